@@ -1,0 +1,117 @@
+/*
+ * raft.h -- C ABI of libraft.so, the MI355X (gfx950) implementation of
+ * ssc-raft's parallel-beam reconstruction hot path: FBP (ramp-type filter +
+ * backprojection), the Radon ray-tracing projector and the emission EM
+ * (eEM) iteration.
+ *
+ * The first three functions keep the exact signatures that sscRaft's ctypes
+ * layer binds, so the reference's Python API and array layouts stay a
+ * drop-in.  All of their arguments are HOST pointers to C-contiguous arrays
+ * (float = float32, int = int32); the calls are synchronous.  Unlike the
+ * reference (which prints and exit()s on any error), failures are reported
+ * through raft_last_error()/raft_last_error_string() and the call returns.
+ *
+ * The raft_*_device entry points take DEVICE pointers (data already resident
+ * in HBM) and a hipStream_t (passed as void*, NULL = the default stream);
+ * they are asynchronous with respect to the host except where noted.  They
+ * have no counterpart in the reference: they exist for callers that keep data
+ * on the GPU (the benchmark, pipelines, the EM warm-start work).
+ *
+ * Layouts: tomogram/sinogram [slices][angles][rays]; object [slices][y][x];
+ * angles [angles] in radians.
+ */
+#ifndef RAFT_H
+#define RAFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference entry points (host pointers) ------------------------------ */
+
+/* Filtered backprojection, RT method.
+ * Replaces getFBPMultiGPU, cuda/src/geometries/parallel/reconstruction_methods/fbp.cu:238-302
+ * (declared in cuda/inc/geometries/parallel/fbp.hpp; bound at sscRaft/rafttypes.py:177-183,
+ * called from sscRaft/geometries/parallel/fbp_methods.py:99-101).
+ * parami = [nrays, nangles, nslices, objsize(=nrays), pad, pady(ignored), padz(ignored),
+ *           filter_type (0 none,1 gaussian,2 lorentz,3 cosine,4 rectangle,5 hann,6 hamming,7 ramp),
+ *           0, blocksize (0 = automatic)]
+ * paramf = [beta/delta inverse or 0, regularization, energy[eV], z2[m], pixel_x, pixel_y, axis_offset]
+ * obj: output [nslices][objsize][objsize]. */
+void getFBPMultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf, int* parami);
+
+/* Emission EM with ray-tracing operators.
+ * Replaces get_eEM_RT_MultiGPU, cuda/src/geometries/parallel/reconstruction_methods/em_rt.cu:201-262
+ * (declared in cuda/inc/geometries/parallel/em.hpp:17-19; bound at sscRaft/rafttypes.py:139-144,
+ * called from sscRaft/geometries/parallel/em_methods.py:66-68).
+ * parami = [nrays, nangles, nslices, objsize, 0, 0, 0, nflats, iterations, blocksize]
+ * paramf = unused.  obj: in/out [nslices][objsize][objsize], the initial guess on entry. */
+void get_eEM_RT_MultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf,
+                         int* parami);
+
+/* Radon transform, ray tracing.
+ * Replaces getRadonRTMultiGPU, cuda/src/geometries/parallel/forward_methods/radon_rt.cu:252-295
+ * (bound at sscRaft/rafttypes.py:312-320, called from sscRaft/geometries/parallel/radon_methods.py:48-52).
+ * obj [nslices][sizeImagey][sizeImagex] -> projection [nslices][nangles][nrays]; ax, ay = half
+ * extents of the image square (the Python wrapper passes 1.0).  The wrapper's nrays*pixel/2
+ * rescale (radon_methods.py:58) stays on the Python side, as in the reference. */
+void getRadonRTMultiGPU(int* gpus, int ngpus, float* projection, float* obj, float* angles, int sizeImagex,
+                        int sizeImagey, int nrays, int nangles, int nslices, float ax, float ay);
+
+/* eEM as get_eEM_RT_MultiGPU, plus the FP64 Kullback-Leibler objective
+ * KL(y || A x_k) of every slice at every iteration k (before its update):
+ * kl[nslices][iterations], kl_excluded[nslices][iterations] = bins left out
+ * (A x <= 0 or y < 0).  Either output may be NULL.  No counterpart in the
+ * reference (which computes no objective). */
+void get_eEM_RT_KL_MultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf,
+                            int* parami, double* kl, long long* kl_excluded);
+
+/* ---- error state (thread-local) -------------------------------------------- */
+
+/* 0 after a successful call; otherwise a RAFT_E* code. */
+int raft_last_error(void);
+/* Human-readable text of the last error of this thread ("" if none). */
+const char* raft_last_error_string(void);
+/* Clears this thread's error state. */
+void raft_clear_error(void);
+
+#define RAFT_OK 0
+#define RAFT_EINVAL 1    /* invalid parameters */
+#define RAFT_EDEVICE 2   /* invalid device id / no device */
+#define RAFT_EHIP 3      /* HIP runtime error (allocation, launch, copy) */
+#define RAFT_EUNSUPPORTED 4
+
+/* ---- device-resident entry points (no reference counterpart) ---------------- */
+
+/* storage precision of the filtered rows / ratio rows staged for backprojection */
+#define RAFT_STORE_F32 0
+#define RAFT_STORE_F16 1
+
+/* FBP on one device: d_tomo [nslices][nangles][nrays] -> d_obj [nslices][nrays][nrays],
+ * both device pointers; angles is a HOST array; parameters as getFBPMultiGPU.
+ * store = RAFT_STORE_F32 or RAFT_STORE_F16.  Returns 0 or an error code. */
+int raft_fbp_device(int device, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                    const int* parami, int store, void* stream);
+
+/* Radon on one device: d_obj [nslices][Ny][Nx] -> d_proj [nslices][nangles][nrays]. */
+int raft_radon_device(int device, float* d_proj, const float* d_obj, const float* angles, int Nx, int Ny,
+                      int nrays, int nangles, int nslices, float ax, float ay, void* stream);
+
+/* eEM on one device: d_obj in/out [nslices][objsize][objsize], d_tomo [nslices][nangles][nrays];
+ * parami as get_eEM_RT_MultiGPU.  kl/kl_excluded: HOST arrays [nslices][iterations] or NULL.
+ * Synchronises the stream before returning when kl or kl_excluded is given. */
+int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                    double* kl, long long* kl_excluded, void* stream);
+
+/* Average device time (ms) of the last launch of each kernel family on this
+ * thread's last call, for roofline reporting: which = 0 filter, 1 FBP
+ * backprojection, 2 projector, 3 EM backprojection/update.  Returns -1 if
+ * not measured.  Timing is recorded only when raft_set_profiling(1). */
+void raft_set_profiling(int on);
+double raft_last_kernel_ms(int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAFT_H */

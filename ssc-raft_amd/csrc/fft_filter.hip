@@ -1,0 +1,370 @@
+// fft_filter.hip -- FBP row filter on gfx950: edge-replicate pad, FFT,
+// multiply by the (Hermitian) filter, inverse FFT, window store.
+//
+// One workgroup filters two sinogram rows of one slice at once: the padded
+// rows u0, u1 (length n = npad) form z = u0 + i u1 in LDS; a mixed-radix
+// Stockham FFT (radices 8, 4, 2, 3, 5, 7, 11, 13; twiddles from an FP64-built
+// table) runs entirely in LDS with the butterfly inputs held in registers
+// across one barrier per pass.  Because the filter H is Hermitian
+// (H_{n-k} = conj H_k, the spectrum of a real kernel), conj(FFT(conj(Z H)))
+// returns q0 + i q1, the two filtered rows.  H carries 1/(npad * pixel)
+// (filters.cu:184-186); the store multiplies by dtheta_a * angular scale and
+// writes only the bin window the backprojector reads, zeros outside [0, n).
+//
+// Padded sizes with a prime factor > 13 or n > 16384 use k_filter_direct:
+// q[m] = sum_j g[(m - j) mod n] u[j] with g = IDFT(H) (built on the host in
+// FP64), the same circular convolution.
+//
+// Reference behaviour restated: paddR2R (padding.cu:140-178), filterFBP /
+// convolution_R2C_C2R_1D / fbp_filtering_R2C2R (filters.cu:33-51,68-87,163-190),
+// Filter::apply (filters.cu:395-438).
+#include "raft_internal.hpp"
+
+namespace raft {
+
+namespace {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); } // * (-i)
+
+// in-register forward DFTs of size R (W_R = exp(-2 pi i / R))
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+    __device__ static void run(float2* v, const float2*, int)
+    {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <> struct Dft<3> {
+    __device__ static void run(float2* v, const float2*, int)
+    {
+        const float c1 = -0.5f, s1 = 0.86602540378443864676f; // sin(2 pi/3)
+        float2 a = v[0], b = v[1], c = v[2];
+        float2 sp = cadd(b, c), sm = csub(b, c);
+        float2 m = make_float2(a.x + c1 * sp.x, a.y + c1 * sp.y);
+        // W = c1 - i s1 : y1 = m - i s1 sm, y2 = m + i s1 sm
+        float2 t = make_float2(s1 * sm.y, -s1 * sm.x); // -i s1 sm
+        v[0] = cadd(a, sp);
+        v[1] = cadd(m, t);
+        v[2] = csub(m, t);
+    }
+};
+
+template <> struct Dft<4> {
+    __device__ static void run(float2* v, const float2*, int)
+    {
+        float2 a = v[0], b = v[1], c = v[2], d = v[3];
+        float2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_mi(csub(b, d));
+        v[0] = cadd(s0, s2);
+        v[1] = cadd(s1, s3);
+        v[2] = csub(s0, s2);
+        v[3] = csub(s1, s3);
+    }
+};
+
+template <> struct Dft<8> {
+    __device__ static void run(float2* v, const float2*, int)
+    {
+        const float h = 0.70710678118654752440f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4>::run(e, nullptr, 0);
+        Dft<4>::run(o, nullptr, 0);
+        // o[k] *= W8^k
+        float2 o1 = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)); // * h(1 - i)
+        float2 o2 = mul_mi(o[2]);
+        float2 o3 = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)); // * h(-1 - i)
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+// odd prime radices: direct sum with W_R^m = tw[m * (n / R)]
+template <int R> struct DftGeneric {
+    __device__ static void run(float2* v, const float2* tw, int n)
+    {
+        float2 in[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) in[q] = v[q];
+        const int st = n / R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 acc = in[0];
+#pragma unroll
+            for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(in[q], tw[((q * r) % R) * st]));
+            v[r] = acc;
+        }
+    }
+};
+template <> struct Dft<5> : DftGeneric<5> {};
+template <> struct Dft<7> : DftGeneric<7> {};
+template <> struct Dft<11> : DftGeneric<11> {};
+template <> struct Dft<13> : DftGeneric<13> {};
+
+// One Stockham pass (decimation in time, autosorting):
+//   butterfly j (0 <= j < n/R), jm = j mod Ns:
+//   in  x_r = z[j + r n/R] * W_{Ns R}^{jm r}
+//   out z[(j - jm) R + jm + r Ns] = DFT_R(x)_r
+// n / THREADS <= 16, so a thread holds at most ceil(16/R) butterflies.
+template <int R, int THREADS>
+__device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ tw, int n, int Ns)
+{
+    constexpr int MAXB = (16 + R - 1) / R;
+    const int nb = n / R;
+    const int twstep = n / (Ns * R);
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = threadIdx.x + b * THREADS;
+        if (j < nb) {
+            const int jm = j % Ns;
+            const int ts = jm * twstep;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float2 x = z[j + r * nb];
+                if (r) x = cmul(x, tw[r * ts]);
+                v[b][r] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = threadIdx.x + b * THREADS;
+        if (j < nb) {
+            Dft<R>::run(v[b], tw, n);
+            const int jm = j % Ns;
+            const int d = (j - jm) * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) z[d + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+struct FftK {
+    int n, nstages;
+    int radix[24];
+};
+
+template <int THREADS>
+__device__ void fft_run(float2* z, const float2* __restrict__ tw, const FftK& p)
+{
+    int Ns = 1;
+    for (int st = 0; st < p.nstages; ++st) {
+        const int R = p.radix[st];
+        switch (R) {
+        case 8: fft_pass<8, THREADS>(z, tw, p.n, Ns); break;
+        case 4: fft_pass<4, THREADS>(z, tw, p.n, Ns); break;
+        case 2: fft_pass<2, THREADS>(z, tw, p.n, Ns); break;
+        case 3: fft_pass<3, THREADS>(z, tw, p.n, Ns); break;
+        case 5: fft_pass<5, THREADS>(z, tw, p.n, Ns); break;
+        case 7: fft_pass<7, THREADS>(z, tw, p.n, Ns); break;
+        case 11: fft_pass<11, THREADS>(z, tw, p.n, Ns); break;
+        default: fft_pass<13, THREADS>(z, tw, p.n, Ns); break;
+        }
+        Ns *= R;
+    }
+}
+
+template <typename OT> __device__ __forceinline__ OT from_f32(float v);
+template <> __device__ __forceinline__ float from_f32<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half from_f32<__half>(float v) { return __float2half(v); }
+
+struct FilterK {
+    const float* tomo;
+    void* q;
+    const float2* H;
+    const float2* tw;
+    const float* g;
+    const float* wa;
+    int N, A, npad, L, tlo, W, none, hp;
+};
+
+template <typename OT>
+__device__ __forceinline__ void store_pair(const FilterK& a, const float2* z, int s, int a0, bool has2, float sign1)
+{
+    OT* q0 = static_cast<OT*>(a.q) + ((size_t)s * a.A + a0) * a.W;
+    OT* q1 = q0 + a.W;
+    const float w0 = a.wa[a0];
+    const float w1 = has2 ? a.wa[a0 + 1] : 0.0f;
+    for (int w = threadIdx.x; w < a.W; w += blockDim.x) {
+        const int T = a.tlo + w;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (T >= 0 && T < a.npad) {
+            const float2 zv = z[T];
+            v0 = zv.x * w0;
+            v1 = sign1 * zv.y * w1;
+        }
+        q0[w] = from_f32<OT>(v0);
+        if (has2) q1[w] = from_f32<OT>(v1);
+    }
+}
+
+template <typename OT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
+{
+    extern __shared__ float2 z[];
+    const int s = blockIdx.x / a.hp;
+    const int a0 = 2 * (blockIdx.x % a.hp);
+    const bool has2 = a0 + 1 < a.A;
+    const float* r0 = a.tomo + ((size_t)s * a.A + a0) * a.N;
+    const float* r1 = r0 + a.N;
+    for (int m = threadIdx.x; m < a.npad; m += THREADS) {
+        int ii = m - a.L;
+        ii = ii < 0 ? 0 : (ii > a.N - 1 ? a.N - 1 : ii);
+        z[m] = make_float2(r0[ii], has2 ? r1[ii] : 0.0f);
+    }
+    __syncthreads();
+    if (a.none) {
+        store_pair<OT>(a, z, s, a0, has2, 1.0f);
+        return;
+    }
+    fft_run<THREADS>(z, a.tw, p);
+    for (int k = threadIdx.x; k < a.npad; k += THREADS) {
+        const float2 v = cmul(z[k], a.H[k]);
+        z[k] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    fft_run<THREADS>(z, a.tw, p);
+    store_pair<OT>(a, z, s, a0, has2, -1.0f);
+}
+
+// direct circular convolution for sizes the LDS FFT does not cover
+template <typename OT>
+__global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
+{
+    extern __shared__ float2 z[]; // u0 + i u1, length npad
+    const int s = blockIdx.x / a.hp;
+    const int a0 = 2 * (blockIdx.x % a.hp);
+    const bool has2 = a0 + 1 < a.A;
+    const float* r0 = a.tomo + ((size_t)s * a.A + a0) * a.N;
+    const float* r1 = r0 + a.N;
+    for (int m = threadIdx.x; m < a.npad; m += blockDim.x) {
+        int ii = m - a.L;
+        ii = ii < 0 ? 0 : (ii > a.N - 1 ? a.N - 1 : ii);
+        z[m] = make_float2(r0[ii], has2 ? r1[ii] : 0.0f);
+    }
+    __syncthreads();
+    OT* q0 = static_cast<OT*>(a.q) + ((size_t)s * a.A + a0) * a.W;
+    OT* q1 = q0 + a.W;
+    const float w0 = a.wa[a0];
+    const float w1 = has2 ? a.wa[a0 + 1] : 0.0f;
+    for (int w = threadIdx.x; w < a.W; w += blockDim.x) {
+        const int T = a.tlo + w;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (T >= 0 && T < a.npad) {
+            if (a.none) {
+                v0 = z[T].x;
+                v1 = z[T].y;
+            } else {
+                float s0 = 0.0f, s1 = 0.0f;
+                int d = T; // (T - j) mod n, j = 0..n-1
+                for (int j = 0; j < a.npad; ++j) {
+                    const float gv = a.g[d];
+                    s0 = fmaf(gv, z[j].x, s0);
+                    s1 = fmaf(gv, z[j].y, s1);
+                    d = (d == 0) ? a.npad - 1 : d - 1;
+                }
+                v0 = s0;
+                v1 = s1;
+            }
+        }
+        q0[w] = from_f32<OT>(v0 * w0);
+        if (has2) q1[w] = from_f32<OT>(v1 * w1);
+    }
+}
+
+template <typename OT, int THREADS>
+void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
+{
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_fft<OT, THREADS>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_filter_fft<OT, THREADS>), dim3(blocks), dim3(THREADS), lds, st, k, p);
+}
+
+template <typename OT>
+void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
+{
+    switch (threads) {
+    case 64: launch_fft_t<OT, 64>(k, p, blocks, lds, st); break;
+    case 128: launch_fft_t<OT, 128>(k, p, blocks, lds, st); break;
+    case 256: launch_fft_t<OT, 256>(k, p, blocks, lds, st); break;
+    case 512: launch_fft_t<OT, 512>(k, p, blocks, lds, st); break;
+    default: launch_fft_t<OT, 1024>(k, p, blocks, lds, st); break;
+    }
+}
+
+} // namespace
+
+FftPlan fft_plan(int n)
+{
+    FftPlan p;
+    p.n = n;
+    if (n < 2 || n > 16384) return p;
+    int m = n;
+    static const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    for (int r : order) {
+        while (m % r == 0 && m > 1) {
+            if (p.nstages >= 24) return p;
+            p.radix[p.nstages++] = r;
+            m /= r;
+        }
+    }
+    if (m != 1) return p;
+    int t = 64;
+    while (t < 1024 && t * 16 < n) t *= 2;
+    p.threads = t;
+    p.ok = true;
+    return p;
+}
+
+void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st)
+{
+    if (a.S <= 0 || a.A <= 0) return;
+    FilterK k;
+    k.tomo = a.tomo; k.q = a.q; k.H = a.H; k.tw = a.tw; k.g = a.g; k.wa = a.wa;
+    k.N = a.N; k.A = a.A; k.npad = a.npad; k.L = a.L; k.tlo = a.tlo; k.W = a.W; k.none = a.none;
+    k.hp = (a.A + 1) / 2;
+    const long long blocks = (long long)a.S * k.hp;
+    if (blocks > 0x7fffffffLL) fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one filter launch");
+    const size_t lds = sizeof(float2) * (size_t)a.npad;
+    if (plan.ok || a.none) {
+        FftK p;
+        p.n = plan.n;
+        p.nstages = plan.nstages;
+        for (int i = 0; i < 24; ++i) p.radix[i] = plan.radix[i];
+        const int threads = plan.ok ? plan.threads : 256;
+        if (store == RAFT_STORE_F16) launch_fft<__half>(k, p, threads, (int)blocks, lds, st);
+        else launch_fft<float>(k, p, threads, (int)blocks, lds, st);
+    } else {
+        if (lds > 160 * 1024) fail(RAFT_EUNSUPPORTED, "padded row too long for the direct filter path");
+        if (store == RAFT_STORE_F16) {
+            RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_direct<__half>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_filter_direct<__half>), dim3((int)blocks), dim3(256), lds, st, k);
+        } else {
+            RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_direct<float>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_filter_direct<float>), dim3((int)blocks), dim3(256), lds, st, k);
+        }
+    }
+    RAFT_HIP(hipGetLastError());
+}
+
+} // namespace raft

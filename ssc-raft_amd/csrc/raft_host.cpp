@@ -1,0 +1,853 @@
+// raft_host.cpp -- host side of libraft: error state, parameter decode,
+// geometry tables, per-device caches, the reference C ABI
+// (getFBPMultiGPU / get_eEM_RT_MultiGPU / getRadonRTMultiGPU) and the
+// device-pointer entry points.
+//
+// Design (MI355X, one process, one host thread per device in the call):
+//   * the reference's slab split over gpus[] (fbp.cu:267-300, em_rt.cu:221-259,
+//     radon_rt.cu:259-294) is kept: contiguous slabs of slices, one std::thread
+//     per device, each binding its own device for the whole slab;
+//   * inside a device, slices are processed in blocks sized from free HBM
+//     (or parami[9] when the caller sets it); device buffers live in a
+//     per-device grow-only workspace instead of being allocated per call;
+//   * geometry (cos/sin of the float32 angles, bin windows, the filter
+//     spectrum, FFT twiddles) is built on the host in FP64.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "raft_internal.hpp"
+
+namespace raft {
+
+// ---------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------
+static thread_local int g_err = RAFT_OK;
+static thread_local std::string g_msg;
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(RAFT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t n)
+{
+    release();
+    if (n == 0) return;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(RAFT_EHIP, "hipMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e));
+    }
+    p = q;
+    bytes = n;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+template <typename F> static int guarded(F&& f)
+{
+    try {
+        f();
+        g_err = RAFT_OK;
+        g_msg.clear();
+        return RAFT_OK;
+    } catch (const Error& e) {
+        g_err = e.code;
+        g_msg = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = RAFT_EHIP;
+        g_msg = e.what();
+        return RAFT_EHIP;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// profiling (events around kernel families, resolved lazily)
+// ---------------------------------------------------------------------------
+static thread_local bool g_prof = false;
+struct ProfRec {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KF_COUNT];
+};
+static thread_local ProfRec g_prof_rec;
+
+static void prof_clear()
+{
+    for (auto& v : g_prof_rec.ev) {
+        for (auto& pr : v) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        v.clear();
+    }
+}
+
+ProfScope::ProfScope(int family, hipStream_t stream) : fam(family), st(stream)
+{
+    if (!g_prof) return;
+    RAFT_HIP(hipEventCreate(&e0));
+    RAFT_HIP(hipEventCreate(&e1));
+    RAFT_HIP(hipEventRecord(e0, st));
+}
+
+ProfScope::~ProfScope()
+{
+    if (!e0) return;
+    (void)hipEventRecord(e1, st);
+    g_prof_rec.ev[fam].push_back({e0, e1});
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+static double angle_step(const float* ang, int A, int a)
+{
+    if (a == A - 1) return (double)ang[a] - (double)ang[a - 1];
+    return (double)ang[a + 1] - (double)ang[a];
+}
+
+// fbp.cu:24-28: float32 comparison, value pi/theta_last
+static double fbp_angle_scale(const float* ang, int A)
+{
+    const float last = ang[A - 1];
+    return (last > (float)M_PI) ? M_PI / (double)last : 1.0;
+}
+
+void bp_window(BpGeom& g)
+{
+    const int tiles = (g.n + BP_TILE - 1) / BP_TILE;
+    const double lo = -g.xc, hi = (double)(tiles * BP_TILE - 1) - g.xc;
+    long long tmin = 0x7fffffffLL, tmax = -0x7fffffffLL;
+    for (int a = 0; a < g.nangles; ++a) {
+        const double u[4] = {std::fma(lo, g.cA[a], std::fma(lo, g.cB[a], g.c0)),
+                             std::fma(hi, g.cA[a], std::fma(lo, g.cB[a], g.c0)),
+                             std::fma(lo, g.cA[a], std::fma(hi, g.cB[a], g.c0)),
+                             std::fma(hi, g.cA[a], std::fma(hi, g.cB[a], g.c0))};
+        const double umin = std::min(std::min(u[0], u[1]), std::min(u[2], u[3]));
+        const double umax = std::max(std::max(u[0], u[1]), std::max(u[2], u[3]));
+        tmin = std::min(tmin, (long long)std::floor(umin) - 1);
+        tmax = std::max(tmax, (long long)std::floor(umax) + 2);
+    }
+    // the per-tile window of the kernel may reach two bins past these bounds
+    // only if rounding disagrees; keep a margin of 2 more on each side
+    tmin -= 2;
+    tmax += 2;
+    g.tlo = (int)tmin;
+    g.width = (int)(tmax - tmin + 1);
+}
+
+RayGeom ray_geom_radon(int Nx, int Ny, int nrays, int nangles, double ax, double ay)
+{
+    RayGeom g;
+    g.Nx = Nx; g.Ny = Ny; g.nrays = nrays; g.nangles = nangles; g.nk = Ny;
+    g.ci = (double)(Nx - 1) / 2.0;
+    g.ck = (double)(Ny - 1) / 2.0;
+    g.ucx = (double)(Nx - 1) / (2.0 * ax);
+    g.ucy = (double)(Ny - 1) / (2.0 * ay);
+    g.ry = (ay * (double)(Nx - 1)) / (ax * (double)(Ny - 1));
+    g.rx = (ax * (double)(Ny - 1)) / (ay * (double)(Nx - 1));
+    g.scale = (float)(2.0 * ay / (double)(Ny - 1));
+    return g;
+}
+
+RayGeom ray_geom_em(int nrays, int nangles, int n)
+{
+    RayGeom g;
+    g.Nx = n; g.Ny = n; g.nrays = nrays; g.nangles = nangles; g.nk = nrays;
+    g.ci = (double)(nrays - 1) / 2.0;
+    g.ck = g.ci;
+    g.ucx = g.ci;
+    g.ucy = g.ci;
+    g.ry = 1.0;
+    g.rx = 1.0;
+    g.scale = (float)(2.0 / (double)(nrays - 1));
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// FBP parameters (parameters.cu:131-219, Filter::apply filters.cu:395-438)
+// ---------------------------------------------------------------------------
+struct FbpCfg {
+    int N, A, S, P, filter, blocksize;
+    int npad, L, M;
+    double pixel, reg, paganin, offset;
+};
+
+static FbpCfg decode_fbp(const float* paramf, const int* parami)
+{
+    if (!paramf || !parami) fail(RAFT_EINVAL, "null parameter array");
+    FbpCfg c;
+    c.N = parami[0];
+    c.A = parami[1];
+    c.S = parami[2];
+    const int objsize = parami[3];
+    c.P = parami[4];
+    c.filter = parami[7];
+    c.blocksize = parami[9];
+    if (c.N < 2) fail(RAFT_EINVAL, "nrays must be >= 2");
+    if (c.A < 2) fail(RAFT_EINVAL, "nangles must be >= 2 (the angle step of the last angle needs two angles)");
+    if (c.S < 0) fail(RAFT_EINVAL, "nslices must be >= 0");
+    if (c.P < 0) fail(RAFT_EINVAL, "padding must be >= 0");
+    if (objsize != c.N) fail(RAFT_EUNSUPPORTED, "objsize must equal nrays (as sscRaft.fbpGPU passes it)");
+    if (c.filter < 0 || c.filter > 7) fail(RAFT_EINVAL, "filter type must be in 0..7");
+    if ((long long)c.N * (1 + c.P) > (1 << 26)) fail(RAFT_EINVAL, "padded size too large");
+    c.npad = c.N * (1 + c.P);
+    c.L = (c.P * c.N) / 2;
+    c.M = c.npad / 2 + 1;
+    const double bd = (double)paramf[0];
+    c.reg = (double)paramf[1];
+    const double energy = (double)paramf[2];
+    const double z2 = (double)paramf[3];
+    c.pixel = (double)paramf[4];
+    c.offset = (double)paramf[6];
+    if (!(c.pixel != 0.0) && c.filter != 0) fail(RAFT_EINVAL, "pixel size must be non-zero");
+    const double plank = 4.135667696E-15, vc = 299792458.0;
+    const double lambda = (bd == 0.0) ? 1.0 : (plank * vc) / energy;
+    c.paganin = lambda * z2 * M_PI * bd / (c.pixel * c.pixel);
+    return c;
+}
+
+static double filter_apply(int type, double w, double reg, double pag)
+{
+    const double pg = 1.0 / (1.0 + pag * w * w);
+    switch (type) {
+    case 1: return w * std::exp(-(double)0.693f * reg * w * w) * pg;
+    case 2: return w * (1.0 / (1.0 + reg * w * w)) * pg;
+    case 3: return w * std::cos(M_PI * 0.5 * w) * pg;
+    case 4: {
+        const double prm = std::max(w * reg * M_PI * 0.5, (double)1E-4f);
+        return w * (std::sin(prm) / prm) * pg;
+    }
+    case 5: return w * (0.5 + 0.5 * std::cos(2.0 * M_PI * w)) * pg;
+    case 6: return w * ((double)0.54f + (double)0.46f * std::cos(2.0 * M_PI * w)) * pg;
+    case 7: return w * pg;
+    default: return 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-device state: lock, cached filter tables, grow-only workspace
+// ---------------------------------------------------------------------------
+struct FilterTables {
+    DevBuf H, tw, g; // float2[n], float2[n], float[n]
+    FftPlan plan;
+};
+
+struct Device {
+    std::mutex mu;
+    std::map<std::string, std::unique_ptr<FilterTables>> filters;
+    DevBuf ws[6];
+    void* workspace(int slot, size_t bytes)
+    {
+        if (ws[slot].bytes < bytes) {
+            ws[slot].release();
+            ws[slot].alloc(bytes);
+        }
+        return ws[slot].p;
+    }
+};
+
+static std::mutex g_dev_mu;
+static std::map<int, std::unique_ptr<Device>> g_devices;
+
+static Device& device_state(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto& d = g_devices[dev];
+    if (!d) d.reset(new Device());
+    return *d;
+}
+
+static void bind_device(int dev)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        fail(RAFT_EDEVICE, "no HIP device available");
+    }
+    if (dev < 0 || dev >= count) fail(RAFT_EDEVICE, "invalid device id " + std::to_string(dev));
+    RAFT_HIP(hipSetDevice(dev));
+}
+
+// bytes of device memory a block of slices may use
+static size_t block_budget()
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        return (size_t)8 << 30;
+    }
+    return std::max<size_t>(fr / 2, (size_t)256 << 20);
+}
+
+template <typename T> static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st)
+{
+    b.alloc(sizeof(T) * v.size());
+    RAFT_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+// filter spectrum, twiddles and (for sizes the LDS FFT does not take) the
+// real convolution kernel; cached per (n, filter, reg, paganin, offset, pixel)
+static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
+{
+    char key[256];
+    snprintf(key, sizeof key, "%d/%d/%.17g/%.17g/%.17g/%.17g", c.npad, c.filter, c.reg, c.paganin, c.offset,
+             c.pixel);
+    auto it = D.filters.find(key);
+    if (it != D.filters.end()) return *it->second;
+    std::unique_ptr<FilterTables> F(new FilterTables());
+    const int n = c.npad, M = c.M;
+    std::vector<double> Hr(n), Hi(n);
+    const double scale = 1.0 / ((double)n * c.pixel);
+    for (int k = 0; k < M; ++k) {
+        const double w = 0.5 * (double)std::min(k, M - k) / (double)M;
+        const double amp = filter_apply(c.filter, w, c.reg, c.paganin);
+        const double ph = -2.0 * M_PI * c.offset * (double)k / (double)(2 * M - 2);
+        Hr[k] = amp * std::cos(ph) * scale;
+        Hi[k] = amp * std::sin(ph) * scale;
+    }
+    for (int k = M; k < n; ++k) {
+        Hr[k] = Hr[n - k];
+        Hi[k] = -Hi[n - k];
+    }
+    if (n % 2 == 0) Hi[n / 2] = 0.0;
+    std::vector<float2> H(n), tw(n);
+    for (int k = 0; k < n; ++k) {
+        H[k] = make_float2((float)Hr[k], (float)Hi[k]);
+        const double ang = -2.0 * M_PI * (double)k / (double)n;
+        tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    F->plan = fft_plan(n);
+    upload(F->H, H, st);
+    upload(F->tw, tw, st);
+    if (!F->plan.ok && c.filter != 0) {
+        // g[d] = sum_k H_k exp(2 pi i k d / n): real because H is Hermitian
+        std::vector<float> g(n);
+        for (int d = 0; d < n; ++d) {
+            double acc = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const long long kd = ((long long)k * d) % n;
+                const double ang = 2.0 * M_PI * (double)kd / (double)n;
+                acc += Hr[k] * std::cos(ang) - Hi[k] * std::sin(ang);
+            }
+            g[d] = (float)acc;
+        }
+        upload(F->g, g, st);
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+    auto& ref = *F;
+    D.filters[key] = std::move(F);
+    return ref;
+}
+
+// ---------------------------------------------------------------------------
+// FBP on device pointers
+// ---------------------------------------------------------------------------
+static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                       const int* parami, int store, hipStream_t st)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    if (store != RAFT_STORE_F32 && store != RAFT_STORE_F16) fail(RAFT_EINVAL, "store must be F32 or F16");
+    if (c.S == 0) return;
+    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+
+    const int A = c.A, N = c.N;
+    BpGeom g;
+    g.n = N;
+    g.nangles = A;
+    g.nbins = c.npad;
+    g.c0 = (double)c.npad / 2.0;
+    g.xc = g.c0 - (double)c.L;
+    g.cA.resize(A);
+    g.cB.resize(A);
+    std::vector<float> wa(A);
+    const double ascale = fbp_angle_scale(angles, A);
+    for (int a = 0; a < A; ++a) {
+        const double th = (double)angles[a];
+        g.cA[a] = std::cos(th);
+        g.cB[a] = -std::sin(th);
+        wa[a] = (float)(angle_step(angles, A, a) * ascale);
+    }
+    bp_window(g);
+
+    FilterTables& F = filter_tables(D, c, st);
+    DevBuf dcA, dcB, dwa;
+    upload(dcA, g.cA, st);
+    upload(dcB, g.cB, st);
+    upload(dwa, wa, st);
+
+    const size_t esz = (store == RAFT_STORE_F16) ? 2 : 4;
+    const size_t per_slice = (size_t)A * g.width * esz;
+    int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
+    const size_t budget = block_budget();
+    while (B > 1 && (size_t)B * per_slice > budget) B = (B + 1) / 2;
+    void* q = D.workspace(0, (size_t)B * per_slice);
+
+    for (int s0 = 0; s0 < c.S; s0 += B) {
+        const int nb = std::min(B, c.S - s0);
+        FilterArgs fa;
+        fa.tomo = d_tomo + (size_t)s0 * A * N;
+        fa.q = q;
+        fa.H = F.H.as<float2>();
+        fa.tw = F.tw.as<float2>();
+        fa.g = F.g.as<float>();
+        fa.wa = dwa.as<float>();
+        fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = g.tlo; fa.W = g.width;
+        fa.none = (c.filter == 0);
+        {
+            ProfScope ps(KF_FILTER, st);
+            launch_fbp_filter(fa, F.plan, store, st);
+        }
+        BpArgs ba;
+        ba.q = q;
+        ba.out = d_obj + (size_t)s0 * N * N;
+        ba.x = nullptr; ba.norm = nullptr; ba.xt = nullptr;
+        ba.cA = dcA.as<double>();
+        ba.cB = dcB.as<double>();
+        ba.xc = g.xc; ba.c0 = g.c0;
+        ba.n = N; ba.A = A; ba.S = nb; ba.tlo = g.tlo; ba.W = g.width; ba.mode = 0;
+        {
+            ProfScope ps(KF_BP, st);
+            launch_backproject(ba, store, st);
+        }
+    }
+    // the small tables above are freed when this call returns
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// Radon on device pointers
+// ---------------------------------------------------------------------------
+static void radon_device(int dev, float* d_proj, const float* d_obj, const float* angles, int Nx, int Ny, int nrays,
+                         int nangles, int nslices, float ax, float ay, hipStream_t st)
+{
+    if (Nx < 2 || Ny < 2) fail(RAFT_EINVAL, "image sizes must be >= 2");
+    if (nrays < 1 || nangles < 1 || nslices < 0) fail(RAFT_EINVAL, "invalid projection sizes");
+    if (!(ax != 0.0f) || !(ay != 0.0f)) fail(RAFT_EINVAL, "ax, ay must be non-zero");
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    if (nslices == 0) return;
+    if (!d_proj || !d_obj) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+
+    std::vector<double> cv(nangles), sv(nangles);
+    for (int a = 0; a < nangles; ++a) {
+        cv[a] = std::cos((double)angles[a]);
+        sv[a] = std::sin((double)angles[a]);
+    }
+    DevBuf dc, ds;
+    upload(dc, cv, st);
+    upload(ds, sv, st);
+    RayGeom g = ray_geom_radon(Nx, Ny, nrays, nangles, (double)ax, (double)ay);
+    const size_t img = (size_t)Nx * Ny;
+    int B = nslices;
+    const size_t budget = block_budget();
+    while (B > 1 && (size_t)B * img * 4 > budget) B = (B + 1) / 2;
+    float* t = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
+    for (int s0 = 0; s0 < nslices; s0 += B) {
+        const int nb = std::min(B, nslices - s0);
+        const float* src = d_obj + (size_t)s0 * img;
+        launch_transpose(src, t, nb, Ny, Nx, st);
+        ProjArgs p;
+        p.img = src; p.imgT = t; p.out = d_proj + (size_t)s0 * nangles * nrays;
+        p.y = nullptr; p.wa = nullptr; p.kl_part = nullptr; p.ex_part = nullptr;
+        p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
+        p.g = g; p.S = nb; p.tlo = 0; p.W = nrays;
+        ProfScope ps(KF_PROJ, st);
+        launch_project(p, 0, st);
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// eEM on device pointers
+// ---------------------------------------------------------------------------
+static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                       double* kl, long long* kl_ex, hipStream_t st)
+{
+    if (!parami) fail(RAFT_EINVAL, "null parameter array");
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = parami[8];
+    int blocksize = parami[9];
+    if (R < 2) fail(RAFT_EINVAL, "nrays must be >= 2");
+    if (A < 2) fail(RAFT_EINVAL, "nangles must be >= 2");
+    if (n < 2) fail(RAFT_EINVAL, "objsize must be >= 2");
+    if (S < 0 || K < 0) fail(RAFT_EINVAL, "nslices and iterations must be >= 0");
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    const double rr = (double)(R - 1) / (double)(n - 1);
+    if (rr * 63.0 * std::sqrt(2.0) + 5.0 > (double)BP_SLOT)
+        fail(RAFT_EUNSUPPORTED, "nrays larger than objsize beyond the backprojection tile window");
+    if (S == 0) return;
+    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+
+    std::vector<double> cv(A), sv(A);
+    std::vector<float> wa(A);
+    BpGeom g;
+    g.n = n;
+    g.nangles = A;
+    g.nbins = R;
+    g.xc = (double)(n - 1) / 2.0;
+    g.c0 = (double)(R - 1) / 2.0;
+    g.cA.resize(A);
+    g.cB.resize(A);
+    for (int a = 0; a < A; ++a) {
+        cv[a] = std::cos((double)angles[a]);
+        sv[a] = std::sin((double)angles[a]);
+        g.cA[a] = cv[a] * rr;
+        g.cB[a] = sv[a] * rr;
+        wa[a] = (float)angle_step(angles, A, a);
+    }
+    bp_window(g);
+    // the ratio rows are written at offset (i - tlo): the window must cover [0, R)
+    if (g.tlo > 0 || g.tlo + g.width < R) {
+        const int lo = std::min(g.tlo, 0), hi = std::max(g.tlo + g.width, R);
+        g.tlo = lo;
+        g.width = hi - lo;
+    }
+    DevBuf dc, ds, dcA, dcB, dwa;
+    upload(dc, cv, st);
+    upload(ds, sv, st);
+    upload(dcA, g.cA, st);
+    upload(dcB, g.cB, st);
+    upload(dwa, wa, st);
+    RayGeom rg = ray_geom_em(R, A, n);
+
+    const size_t img = (size_t)n * n;
+    const int nbl = proj_blocks_per_slice(rg);
+    const size_t per_slice = (size_t)A * g.width * 4 + img * 4 + (size_t)nbl * 16;
+    int B = blocksize > 0 ? std::min(blocksize, S) : S;
+    const size_t budget = block_budget();
+    while (B > 1 && (size_t)B * per_slice > budget) B = (B + 1) / 2;
+
+    float* rows = static_cast<float*>(D.workspace(0, (size_t)B * A * g.width * 4));
+    float* xT = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
+    float* ones = static_cast<float*>(D.workspace(2, img * 4 + (size_t)A * g.width * 4));
+    float* onerows = ones + img;
+    double* part = static_cast<double*>(D.workspace(3, (size_t)B * nbl * 16));
+    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * nbl);
+    const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
+    double* dkl = nullptr;
+    unsigned long long* dex = nullptr;
+    if (want_kl && K > 0) {
+        double* buf = static_cast<double*>(D.workspace(4, (size_t)S * K * 16));
+        dkl = buf;
+        dex = reinterpret_cast<unsigned long long*>(buf + (size_t)S * K);
+    }
+
+    // backones = B(1): the same for every slice
+    launch_fill_rows(onerows, dwa.as<float>(), 1, A, g.width, g.tlo, R, st);
+    {
+        BpArgs b1;
+        b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
+        b1.cA = dcA.as<double>(); b1.cB = dcB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
+        b1.n = n; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 2;
+        launch_backproject(b1, RAFT_STORE_F32, st);
+    }
+    // ratio rows: margins stay zero, the projector writes [0, R)
+    RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)B * A * g.width * 4, st));
+
+    for (int s0 = 0; s0 < S; s0 += B) {
+        const int nb = std::min(B, S - s0);
+        float* x = d_obj + (size_t)s0 * img;
+        const float* y = d_tomo + (size_t)s0 * A * R;
+        launch_transpose(x, xT, nb, n, n, st);
+        for (int it = 0; it < K; ++it) {
+            ProjArgs p;
+            p.img = x; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
+            p.kl_part = want_kl ? part : nullptr;
+            p.ex_part = want_kl ? expart : nullptr;
+            p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
+            p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
+            {
+                ProfScope ps(KF_PROJ, st);
+                launch_project(p, 1, st);
+            }
+            if (want_kl) launch_kl_reduce(part, expart, nbl, nb, dkl + (size_t)s0 * K, dex + (size_t)s0 * K, K, it, st);
+            BpArgs b;
+            b.q = rows; b.out = x; b.x = x; b.norm = ones; b.xt = xT;
+            b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
+            b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
+            {
+                ProfScope ps(KF_EMBP, st);
+                launch_backproject(b, RAFT_STORE_F32, st);
+            }
+        }
+    }
+    if (want_kl && K > 0) {
+        std::vector<double> hk((size_t)S * K);
+        std::vector<unsigned long long> he((size_t)S * K);
+        RAFT_HIP(hipMemcpyAsync(hk.data(), dkl, hk.size() * 8, hipMemcpyDeviceToHost, st));
+        RAFT_HIP(hipMemcpyAsync(he.data(), dex, he.size() * 8, hipMemcpyDeviceToHost, st));
+        RAFT_HIP(hipStreamSynchronize(st));
+        if (kl) std::memcpy(kl, hk.data(), hk.size() * 8);
+        if (kl_ex)
+            for (size_t e = 0; e < he.size(); ++e) kl_ex[e] = (long long)he[e];
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// host-pointer wrappers: slab split over gpus[], one host thread per device
+// ---------------------------------------------------------------------------
+static void check_gpus(const int* gpus, int ngpus)
+{
+    if (!gpus || ngpus < 1) fail(RAFT_EDEVICE, "empty GPU list");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        fail(RAFT_EDEVICE, "no HIP device available");
+    }
+    for (int i = 0; i < ngpus; ++i)
+        if (gpus[i] < 0 || gpus[i] >= count) fail(RAFT_EDEVICE, "invalid device number " + std::to_string(gpus[i]));
+}
+
+// run fn(device, first_slice, count) on contiguous slabs, one thread per device
+template <typename F> static void run_slabs(const int* gpus, int ngpus, int nslices, F fn)
+{
+    const int per = (nslices + ngpus - 1) / ngpus;
+    if (ngpus == 1) {
+        fn(gpus[0], 0, nslices);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<Error> errs(ngpus, Error{RAFT_OK, ""});
+    int ptr = 0;
+    for (int i = 0; i < ngpus; ++i) {
+        const int cnt = std::max(0, std::min(nslices - ptr, per));
+        const int first = ptr;
+        ptr += cnt;
+        th.emplace_back([&, i, first, cnt]() {
+            try {
+                if (cnt > 0) fn(gpus[i], first, cnt);
+            } catch (const Error& e) {
+                errs[i] = e;
+            } catch (const std::exception& e) {
+                errs[i] = Error{RAFT_EHIP, e.what()};
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+    for (auto& e : errs)
+        if (e.code != RAFT_OK) throw e;
+}
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { RAFT_HIP(hipStreamCreate(&s)); }
+    ~Stream() { if (s) (void)hipStreamDestroy(s); }
+};
+
+static void fbp_host(int* gpus, int ngpus, float* obj, float* tomo, float* angles, float* paramf, int* parami)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (!obj || !tomo || !angles) fail(RAFT_EINVAL, "null data pointer");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, c.S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t tin = (size_t)c.A * c.N, oout = (size_t)c.N * c.N;
+        // device copies of the slab in blocks that fit next to the workspace
+        int B = c.blocksize > 0 ? std::min(c.blocksize, cnt) : cnt;
+        const size_t need = (tin + oout) * 4 + (size_t)c.A * (size_t)(1.5 * c.N + 64) * 4;
+        const size_t budget = block_budget();
+        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
+        DevBuf dt((size_t)B * tin * 4), dob((size_t)B * oout * 4);
+        std::vector<int> pi(parami, parami + 10);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            RAFT_HIP(hipMemcpyAsync(dt.p, tomo + (size_t)(first + s0) * tin, (size_t)nb * tin * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            pi[2] = nb;
+            pi[9] = 0;
+            fbp_device(dev, dob.as<float>(), dt.as<float>(), angles, paramf, pi.data(), RAFT_STORE_F32, st.s);
+            RAFT_HIP(hipMemcpyAsync(obj + (size_t)(first + s0) * oout, dob.p, (size_t)nb * oout * 4,
+                                    hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+static void radon_host(int* gpus, int ngpus, float* proj, float* obj, float* angles, int Nx, int Ny, int nrays,
+                       int nangles, int nslices, float ax, float ay)
+{
+    if (!proj || !obj || !angles) fail(RAFT_EINVAL, "null data pointer");
+    if (Nx < 2 || Ny < 2 || nrays < 1 || nangles < 1 || nslices < 0) fail(RAFT_EINVAL, "invalid sizes");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, nslices, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t iin = (size_t)Nx * Ny, pout = (size_t)nangles * nrays;
+        int B = cnt;
+        const size_t budget = block_budget();
+        while (B > 1 && (size_t)B * (2 * iin + pout) * 4 > budget) B = (B + 1) / 2;
+        DevBuf di((size_t)B * iin * 4), dp((size_t)B * pout * 4);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            RAFT_HIP(hipMemcpyAsync(di.p, obj + (size_t)(first + s0) * iin, (size_t)nb * iin * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            radon_device(dev, dp.as<float>(), di.as<float>(), angles, Nx, Ny, nrays, nangles, nb, ax, ay, st.s);
+            RAFT_HIP(hipMemcpyAsync(proj + (size_t)(first + s0) * pout, dp.p, (size_t)nb * pout * 4,
+                                    hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+static void eem_host(int* gpus, int ngpus, float* obj, float* tomo, float* angles, int* parami, double* kl,
+                     long long* kl_ex)
+{
+    if (!parami) fail(RAFT_EINVAL, "null parameter array");
+    if (!obj || !tomo || !angles) fail(RAFT_EINVAL, "null data pointer");
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = parami[8];
+    if (R < 2 || A < 2 || n < 2 || S < 0 || K < 0) fail(RAFT_EINVAL, "invalid eEM parameters");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t tin = (size_t)A * R, img = (size_t)n * n;
+        int B = parami[9] > 0 ? std::min(parami[9], cnt) : cnt;
+        const size_t budget = block_budget();
+        const size_t need = (tin + 2 * img) * 4 + (size_t)A * (size_t)(1.5 * R + 64) * 4;
+        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
+        DevBuf dt((size_t)B * tin * 4), dx((size_t)B * img * 4);
+        std::vector<int> pi(parami, parami + 10);
+        std::vector<double> hk;
+        std::vector<long long> he;
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            RAFT_HIP(hipMemcpyAsync(dt.p, tomo + (size_t)(first + s0) * tin, (size_t)nb * tin * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            RAFT_HIP(hipMemcpyAsync(dx.p, obj + (size_t)(first + s0) * img, (size_t)nb * img * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            pi[2] = nb;
+            pi[9] = 0;
+            double* klp = kl ? kl + (size_t)(first + s0) * K : nullptr;
+            long long* exp_ = kl_ex ? kl_ex + (size_t)(first + s0) * K : nullptr;
+            eem_device(dev, dx.as<float>(), dt.as<float>(), angles, pi.data(), klp, exp_, st.s);
+            RAFT_HIP(hipMemcpyAsync(obj + (size_t)(first + s0) * img, dx.p, (size_t)nb * img * 4,
+                                    hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+} // namespace raft
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+using namespace raft;
+
+extern "C" {
+
+void getFBPMultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf, int* parami)
+{
+    guarded([&] { fbp_host(gpus, ngpus, obj, tomogram, angles, paramf, parami); });
+}
+
+void get_eEM_RT_MultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf,
+                         int* parami)
+{
+    (void)paramf;
+    guarded([&] { eem_host(gpus, ngpus, obj, tomogram, angles, parami, nullptr, nullptr); });
+}
+
+void get_eEM_RT_KL_MultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf,
+                            int* parami, double* kl, long long* kl_excluded)
+{
+    (void)paramf;
+    guarded([&] { eem_host(gpus, ngpus, obj, tomogram, angles, parami, kl, kl_excluded); });
+}
+
+void getRadonRTMultiGPU(int* gpus, int ngpus, float* projection, float* obj, float* angles, int sizeImagex,
+                        int sizeImagey, int nrays, int nangles, int nslices, float ax, float ay)
+{
+    guarded([&] {
+        radon_host(gpus, ngpus, projection, obj, angles, sizeImagex, sizeImagey, nrays, nangles, nslices, ax, ay);
+    });
+}
+
+int raft_last_error(void) { return g_err; }
+const char* raft_last_error_string(void) { return g_msg.c_str(); }
+void raft_clear_error(void)
+{
+    g_err = RAFT_OK;
+    g_msg.clear();
+}
+
+int raft_fbp_device(int device, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                    const int* parami, int store, void* stream)
+{
+    return guarded([&] {
+        fbp_device(device, d_obj, d_tomo, angles, paramf, parami, store, static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_radon_device(int device, float* d_proj, const float* d_obj, const float* angles, int Nx, int Ny, int nrays,
+                      int nangles, int nslices, float ax, float ay, void* stream)
+{
+    return guarded([&] {
+        radon_device(device, d_proj, d_obj, angles, Nx, Ny, nrays, nangles, nslices, ax, ay,
+                     static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                    double* kl, long long* kl_excluded, void* stream)
+{
+    return guarded([&] {
+        eem_device(device, d_obj, d_tomo, angles, parami, kl, kl_excluded, static_cast<hipStream_t>(stream));
+    });
+}
+
+void raft_set_profiling(int on)
+{
+    prof_clear();
+    g_prof = (on != 0);
+}
+
+double raft_last_kernel_ms(int which)
+{
+    if (which < 0 || which >= KF_COUNT) return -1.0;
+    auto& v = g_prof_rec.ev[which];
+    if (v.empty()) return -1.0;
+    double tot = 0.0;
+    for (auto& pr : v) {
+        if (hipEventSynchronize(pr.second) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1.0;
+        }
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) != hipSuccess) {
+            (void)hipGetLastError();
+            return -1.0;
+        }
+        tot += ms;
+    }
+    return tot / (double)v.size();
+}
+
+} // extern "C"

@@ -1,0 +1,166 @@
+// raft_internal.hpp -- shared host/device declarations of libraft (gfx950).
+//
+// Error handling: internal code throws raft::Error; every extern "C" entry
+// point catches it and stores code + text in thread-local state
+// (raft_last_error / raft_last_error_string).  Nothing calls exit().
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/raft.h"
+
+namespace raft {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define RAFT_HIP(call) ::raft::hip_check((call), #call)
+
+// device allocation owned by the caller's scope
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    ~DevBuf() { release(); }
+    void alloc(size_t n);
+    void release();
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Backprojection geometry shared by FBP and the eEM update.
+// For pixel (i, j) of an n x n image and angle a:
+//     X_i = i - xc,  Y_j = j - xc         (exact: small integers or halves)
+//     u   = fma(X_i, cA[a], fma(Y_j, cB[a], c0))     (IEEE double)
+//     T   = (int)u (truncation); the sample counts iff 0 <= T < nbins.
+// FBP-RT:  xc = npad/2 - L, cA = cos, cB = -sin, c0 = npad/2, nbins = npad
+//          (fbp.cu:38-69 with the pixel size cancelled and the crop folded in).
+// eEM:     xc = (n-1)/2, cA = cos*rr, cB = sin*rr, c0 = (nrays-1)/2, nbins = nrays,
+//          rr = (nrays-1)/(n-1)   (em_RT_kernel.cu:262-291 about the image centre).
+// Rows staged for the kernel hold the bins [tlo, tlo + width) of each angle;
+// bins outside [0, nbins) are stored as zeros so the kernel needs no range test.
+// ---------------------------------------------------------------------------
+struct BpGeom {
+    int n = 0;        // image side
+    int nangles = 0;
+    int nbins = 0;    // valid bin range [0, nbins)
+    double xc = 0;    // pixel-centre offset
+    double c0 = 0;
+    int tlo = 0;      // first staged bin
+    int width = 0;    // staged bins per angle row
+    std::vector<double> cA, cB; // per angle
+};
+
+// Tile of the backprojection kernel: 64 x 64 pixels, 256 threads, 4 x 4 pixels each.
+constexpr int BP_TILE = 64;
+constexpr int BP_CHUNK = 32;   // angles staged per LDS round
+constexpr int BP_SLOT = 96;    // LDS floats per staged angle window (>= 63*sqrt(2) + 4)
+
+// global bin window covering every pixel of the tile grid (ceil(n/64)*64)^2
+void bp_window(BpGeom& g);
+
+// ---------------------------------------------------------------------------
+// Ray geometry shared by Radon-RT and the eEM forward operator.
+// Ray i at angle a, sample k (centred forms, see oracle/raft_oracle.c):
+//     ir = i - ci, kr = k - ck
+//     ux = fma(kr, -sin*ry, fma(ir, cos, ucx)),  X = (int)ux
+//     uy = fma(kr,  cos,    fma(ir, sin*rx, ucy)), Y = (int)uy
+// sample counts iff 0 <= X < Nx and 0 <= Y < Ny; result = scale * sum.
+// ---------------------------------------------------------------------------
+struct RayGeom {
+    int Nx = 0, Ny = 0, nrays = 0, nangles = 0, nk = 0;
+    double ci = 0, ck = 0, ucx = 0, ucy = 0, ry = 1, rx = 1;
+    float scale = 1;
+};
+RayGeom ray_geom_radon(int Nx, int Ny, int nrays, int nangles, double ax, double ay);
+RayGeom ray_geom_em(int nrays, int nangles, int n);
+
+// per-angle tables on the device
+struct AngleTables {
+    DevBuf cosv, sinv; // double[A]
+};
+
+// ---------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+struct FftPlan {
+    int n = 0;
+    int nstages = 0;
+    int radix[24] = {0};
+    int threads = 256;
+    bool ok = false;   // false -> direct-convolution path
+};
+FftPlan fft_plan(int n);
+
+struct FilterArgs {
+    const float* tomo;  // [S][A][N]
+    void* q;            // [S][A][W] float or __half
+    const float2* H;    // [npad] Hermitian filter incl. 1/(npad*pixel)
+    const float2* tw;   // [npad] exp(-2 pi i t/npad)
+    const float* g;     // direct-convolution kernel (real, [npad]) or null
+    const float* wa;    // [A] per-angle weight (dtheta * angular scale)
+    int N, A, S, npad, L, tlo, W;
+    int none;           // filter 'none': no filtering, no 1/(npad h)
+};
+void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
+
+struct BpArgs {
+    const void* q;      // [S][A][W] staged rows (float or half)
+    float* out;         // [S][n][n]
+    const float* x;     // eEM: current image (may alias out)
+    const float* norm;  // eEM: backones [S][n][n] (null for FBP / backones pass)
+    float* xt;          // eEM: optional transposed copy of the updated image (or null)
+    const double* cA;
+    const double* cB;
+    double xc, c0;
+    int n, A, S, tlo, W;
+    int mode;           // 0 = FBP store, 1 = eEM update x*bp/norm, 2 = plain store (backones)
+};
+void launch_backproject(const BpArgs& a, int store, hipStream_t st);
+
+struct ProjArgs {
+    const float* img;   // [S][Ny][Nx]
+    const float* imgT;  // [S][Nx][Ny] transposed copy (may be null: then no transposed path)
+    float* out;         // Radon: [S][A][nrays]; eEM: ratio rows [S][A][W] at offset -tlo
+    const float* y;     // eEM: measured sinogram [S][A][nrays] (null for Radon)
+    const float* wa;    // eEM: per-angle dtheta
+    double* kl_part;    // eEM: per-block KL partial sums (null = skip)
+    unsigned long long* ex_part; // eEM: per-block excluded counts
+    const double* cosv;
+    const double* sinv;
+    RayGeom g;
+    int S;
+    int tlo, W;         // eEM ratio-row layout
+};
+void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
+int proj_blocks_per_slice(const RayGeom& g);
+
+void launch_transpose(const float* in, float* out, int S, int rows, int cols, hipStream_t st);
+void launch_fill_rows(float* q, const float* wa, int S, int A, int W, int tlo, int nbins, hipStream_t st);
+void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,
+                      unsigned long long* ex, int K, int iter, hipStream_t st);
+
+// profiling hooks (events around kernel families)
+enum KernelFamily { KF_FILTER = 0, KF_BP = 1, KF_PROJ = 2, KF_EMBP = 3, KF_COUNT = 4 };
+struct ProfScope {
+    int fam;
+    hipStream_t st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(int family, hipStream_t stream);
+    ~ProfScope();
+};
+
+} // namespace raft

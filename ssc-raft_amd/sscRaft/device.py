@@ -1,0 +1,101 @@
+"""Device-resident entry points for torch tensors already in HBM
+(raft_fbp_device / raft_radon_device / raft_eem_device).  torch is only the
+allocator and stream provider here; every computation runs in libraft."""
+import numpy
+
+from .rafttypes import CNICE, libraft, raft_check, ctypes
+
+__all__ = ["fbp_device", "radon_device", "eem_device", "set_profiling", "last_kernel_ms",
+           "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
+
+KF_FILTER, KF_BP, KF_PROJ, KF_EMBP = 0, 1, 2, 3
+
+
+def _stream(t, stream):
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(t.device)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _check_tensor(t, name):
+    import torch
+
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous float32 CUDA tensor")
+
+
+def fbp_device(tomo, angles, paramf, parami, out=None, store="f32", stream=None):
+    """FBP of tomo [S, A, N] (CUDA float32) -> out [S, N, N]."""
+    import torch
+
+    _check_tensor(tomo, "tomo")
+    N, A, S = int(parami[0]), int(parami[1]), int(parami[2])
+    if out is None:
+        out = torch.empty((S, N, N), dtype=torch.float32, device=tomo.device)
+    _check_tensor(out, "out")
+    angles = CNICE(numpy.asarray(angles))
+    paramf = CNICE(numpy.asarray(paramf), numpy.float32)
+    parami = CNICE(numpy.asarray(parami), numpy.int32)
+    rc = libraft.raft_fbp_device(tomo.device.index or 0, ctypes.c_void_p(out.data_ptr()),
+                                 ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
+                                 paramf.ctypes.data_as(ctypes.c_void_p), parami.ctypes.data_as(ctypes.c_void_p),
+                                 1 if store == "f16" else 0, _stream(tomo, stream))
+    if rc:
+        raft_check("raft_fbp_device")
+    return out
+
+
+def radon_device(img, angles, nrays=None, ax=1.0, ay=1.0, out=None, stream=None):
+    """Radon (no Python-side rescale) of img [S, Ny, Nx] -> out [S, A, nrays]."""
+    import torch
+
+    _check_tensor(img, "img")
+    S, Ny, Nx = img.shape
+    nrays = Nx if nrays is None else nrays
+    angles = CNICE(numpy.asarray(angles))
+    A = angles.shape[0]
+    if out is None:
+        out = torch.empty((S, A, nrays), dtype=torch.float32, device=img.device)
+    _check_tensor(out, "out")
+    rc = libraft.raft_radon_device(img.device.index or 0, ctypes.c_void_p(out.data_ptr()),
+                                   ctypes.c_void_p(img.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p), Nx, Ny,
+                                   nrays, A, S, ctypes.c_float(ax), ctypes.c_float(ay), _stream(img, stream))
+    if rc:
+        raft_check("raft_radon_device")
+    return out
+
+
+def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None):
+    """eEM of tomo [S, A, R]; obj [S, R, R] is the in/out initial guess (ones if None).
+    Returns (obj, kl [S, K] float64 or None, excluded [S, K] or None)."""
+    import torch
+
+    _check_tensor(tomo, "tomo")
+    S, A, R = tomo.shape
+    if obj is None:
+        obj = torch.ones((S, R, R), dtype=torch.float32, device=tomo.device)
+    _check_tensor(obj, "obj")
+    angles = CNICE(numpy.asarray(angles))
+    parami = CNICE(numpy.array([R, A, S, obj.shape[-1], 0, 0, 0, 1, iterations, 0]), numpy.int32)
+    kl = numpy.zeros((S, max(iterations, 1)), dtype=numpy.float64) if with_kl else None
+    ex = numpy.zeros((S, max(iterations, 1)), dtype=numpy.int64) if with_kl else None
+    rc = libraft.raft_eem_device(tomo.device.index or 0, ctypes.c_void_p(obj.data_ptr()),
+                                 ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
+                                 parami.ctypes.data_as(ctypes.c_void_p),
+                                 kl.ctypes.data_as(ctypes.c_void_p) if with_kl else None,
+                                 ex.ctypes.data_as(ctypes.c_void_p) if with_kl else None, _stream(tomo, stream))
+    if rc:
+        raft_check("raft_eem_device")
+    if with_kl:
+        return obj, kl[:, :iterations], ex[:, :iterations]
+    return obj, None, None
+
+
+def set_profiling(on):
+    libraft.raft_set_profiling(1 if on else 0)
+
+
+def last_kernel_ms(which):
+    return libraft.raft_last_kernel_ms(which)

@@ -1,0 +1,211 @@
+"""GPU parity: libraft (through the reference C ABI) against the FP64 CPU oracle.
+
+Bars: FP32 paths within 1e-5 relative L2 of the FP64 oracle (bin indices are
+computed with the oracle's own FP64 expressions, so only value rounding
+differs); FP16-stored filtered rows within 2e-3 (reported); eEM iterates and
+per-iteration KL tracked against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from sscRaft import phantom as ph
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def noisy_sino(n, a, s, seed, i0=1e4):
+    img = ph.phantoms(n, s, config=9, first=seed)
+    ang = ph.uniform_angles(a)
+    p = oracle.radon(img, ang) * 1.0  # pixel = 2/N: nrays*pixel/2 = 1
+    return ph.poisson_log(p, i0, seed=seed), ang
+
+
+def fbp_dic(ang, pad=2, filt="ramp", pixel=1.0, offset=0.0, reg=0.0, bd=0.0):
+    return {"gpu": [0], "angles[rad]": ang, "filter": filt, "padding": pad, "detectorPixel[m]": pixel,
+            "rotation axis offset": offset, "regularization": reg, "beta/delta": bd, "energy[eV]": 2.0e4,
+            "z2[m]": 0.5}
+
+
+FILTERS = ["none", "gaussian", "lorentz", "cosine", "rectangle", "hann", "hamming", "ramp"]
+
+
+@pytest.mark.parametrize("n,a,pad", [(16, 12, 0), (33, 17, 1), (64, 64, 2), (100, 90, 2), (128, 77, 3),
+                                     (17, 20, 2), (255, 128, 1), (96, 2, 2)])
+def test_fbp_sizes(gpu, n, a, pad):
+    import sscRaft
+
+    y, ang = noisy_sino(n, a, 2, seed=n + a)
+    dic = fbp_dic(ang, pad=pad, pixel=2.0 / n)
+    out = sscRaft.fbp(y, dic=dic)
+    paramf, parami = oracle.fbp_params(n, a, 2, pad=pad, filter_type=7, pixel=2.0 / n)
+    ref = oracle.fbp(y, ang, paramf, parami)
+    e = rel(out, ref)
+    print(f"fbp n={n} a={a} pad={pad} rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("offset", [0.0, 1.5, -0.7])
+def test_fbp_filters(gpu, filt, offset):
+    import sscRaft
+
+    n, a = 64, 48
+    y, ang = noisy_sino(n, a, 1, seed=3)
+    dic = fbp_dic(ang, pad=2, filt=filt, pixel=0.37, offset=offset, reg=0.8, bd=150.0)
+    out = sscRaft.fbp(y, dic=dic)
+    paramf, parami = oracle.fbp_params(n, a, 1, pad=2, filter_type=sscRaft.FilterNumber(filt), pixel=0.37,
+                                       offset=offset, reg=0.8, beta_delta=150.0, energy=2.0e4, z2=0.5)
+    ref = oracle.fbp(y, ang, paramf, parami)
+    e = rel(out, ref)
+    print(f"fbp filter={filt} offset={offset} rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
+def test_fbp_angles_over_pi_and_nonuniform(gpu):
+    import sscRaft
+
+    n, a = 48, 40
+    rng = np.random.default_rng(5)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, a)).astype(np.float32)  # last > pi -> angular scale
+    y = rng.standard_normal((2, a, n)).astype(np.float32)
+    out = sscRaft.fbp(y, dic=fbp_dic(ang, pad=1, pixel=1.0))
+    paramf, parami = oracle.fbp_params(n, a, 2, pad=1, filter_type=7, pixel=1.0)
+    assert rel(out, oracle.fbp(y, ang, paramf, parami)) <= FP32_TOL
+
+
+def test_fbp_2d_input_and_given_obj(gpu):
+    import sscRaft
+
+    n, a = 40, 30
+    y, ang = noisy_sino(n, a, 1, seed=11)
+    obj = np.full((1, n, n), 7.0, dtype=np.float32)
+    out = sscRaft.fbp(y[0], obj=obj, dic=fbp_dic(ang, pixel=2.0 / n))
+    assert out is obj
+    paramf, parami = oracle.fbp_params(n, a, 1, pad=2, filter_type=7, pixel=2.0 / n)
+    assert rel(out, oracle.fbp(y, ang, paramf, parami)) <= FP32_TOL
+
+
+def test_fbp_2048(gpu):
+    """The headline size: one 2048^2 slice, 2048 angles, pad 2, ramp, I0 = 1e4."""
+    import sscRaft
+
+    n = a = 2048
+    img = ph.phantoms(n, 1, config=2)
+    ang = ph.uniform_angles(a)
+    y = ph.poisson_log(oracle.radon(img, ang), 1e4, seed=2000)
+    out = sscRaft.fbp(y, dic=fbp_dic(ang, pixel=2.0 / n))
+    paramf, parami = oracle.fbp_params(n, a, 1, pad=2, filter_type=7, pixel=2.0 / n)
+    ref = oracle.fbp(y, ang, paramf, parami)
+    e = rel(out, ref)
+    print(f"fbp 2048^2 rel-L2 vs FP64 oracle {e:.3e}, max abs {np.abs(out - ref).max():.3e}")
+    assert e <= FP32_TOL
+
+
+def test_fbp_fp16_storage(gpu):
+    import torch
+    from sscRaft import device as dv
+
+    n, a, s = 256, 256, 2
+    y, ang = noisy_sino(n, a, s, seed=21)
+    paramf, parami = oracle.fbp_params(n, a, s, pad=2, filter_type=7, pixel=2.0 / n)
+    ref = oracle.fbp(y, ang, paramf, parami)
+    t = torch.from_numpy(y).cuda()
+    o32 = dv.fbp_device(t, ang, paramf, parami, store="f32")
+    o16 = dv.fbp_device(t, ang, paramf, parami, store="f16")
+    torch.cuda.synchronize()
+    e32 = rel(o32.cpu().numpy(), ref)
+    e16 = rel(o16.cpu().numpy(), ref)
+    print(f"fbp 256^2 device path: f32 rel-L2 {e32:.2e}, f16-stored rows rel-L2 {e16:.2e}")
+    assert e32 <= FP32_TOL
+    assert e16 <= 2e-3
+
+
+@pytest.mark.parametrize("ny,nx,a", [(16, 16, 12), (33, 33, 17), (64, 64, 90), (20, 32, 19), (100, 100, 64),
+                                     (256, 256, 256)])
+def test_radon(gpu, ny, nx, a):
+    import sscRaft
+
+    rng = np.random.default_rng(ny * nx + a)
+    img = rng.random((2, ny, nx)).astype(np.float32)
+    ang = ph.uniform_angles(a)
+    out = sscRaft.radon_RT(img, ang, [0], pixel=1.0)
+    ref = oracle.radon(img, ang) * (nx * 1.0 / 2)
+    e = rel(out, ref)
+    print(f"radon {ny}x{nx} a={a} rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
+def test_radon_random_angles(gpu):
+    import sscRaft
+
+    rng = np.random.default_rng(1)
+    img = ph.phantoms(128, 2, config=4)
+    ang = np.sort(rng.uniform(-np.pi, 2 * np.pi, 50)).astype(np.float32)
+    out = sscRaft.radon_RT(img, ang, [0], pixel=2.0 / 128)
+    ref = oracle.radon(img, ang)
+    assert rel(out, ref) <= FP32_TOL
+
+
+@pytest.mark.parametrize("n,a,k", [(16, 16, 1), (33, 17, 5), (64, 64, 20), (47, 60, 5)])
+def test_eem(gpu, n, a, k):
+    import sscRaft
+
+    img = ph.phantoms(n, 2, config=3, first=n)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n), 0)
+    x, kl, ex = sscRaft.eEMRT_GPU_KL(y, ang, k, [0], 0)
+    xr, klr, exr = oracle.eem(y, ang, k)
+    e = rel(x, xr)
+    ek = float(np.max(np.abs(kl - klr) / np.abs(klr)))
+    print(f"eem n={n} a={a} K={k}: x rel-L2 {e:.2e}, KL max rel {ek:.2e}, KL {kl[0, 0]:.4e}->{kl[0, -1]:.4e}")
+    assert e <= (FP32_TOL if k <= 5 else 1e-4)
+    assert ek <= 1e-4
+    np.testing.assert_array_equal(ex, exr)
+
+
+def test_eem_plain_abi_matches_kl_variant(gpu):
+    import sscRaft
+
+    n, a = 40, 36
+    img = ph.phantoms(n, 1, config=3)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=1), 0)
+    x1 = sscRaft.em(y, angles=ang, dic={"gpu": [0], "method": "eEMRT", "iterations": 4})
+    x2, _, _ = sscRaft.eEMRT_GPU_KL(y, ang, 4, [0], 0)
+    np.testing.assert_array_equal(x1, x2)
+
+
+def test_eem_initial_guess_in_place(gpu):
+    import sscRaft
+
+    n, a = 32, 30
+    img = ph.phantoms(n, 1, config=3, first=5)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=2), 0)
+    x0 = np.full((1, n, n), 0.5, dtype=np.float32)
+    out = sscRaft.eEMRT_GPU_(y, ang, 3, [0], 0, obj=x0)
+    assert out is x0
+    xr, _, _ = oracle.eem(y, ang, 3, obj=np.full((1, n, n), 0.5))
+    assert rel(out, xr) <= FP32_TOL
+
+
+def test_errors(gpu):
+    import sscRaft
+
+    y = np.zeros((1, 1, 16), np.float32)
+    with pytest.raises(ValueError):
+        sscRaft.fbp(y, dic={"gpu": [0], "angles[rad]": np.zeros(1, np.float32)})
+    with pytest.raises(RuntimeError):
+        sscRaft.fbp(np.zeros((1, 8, 16), np.float32), dic={"gpu": [99], "angles[rad]": ph.uniform_angles(8)})
+    # library still usable afterwards
+    out = sscRaft.fbp(np.zeros((1, 8, 16), np.float32), dic={"gpu": [0], "angles[rad]": ph.uniform_angles(8)})
+    assert np.all(out == 0)
