@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the parallel-beam hot path on one MI355X.
+
+Headline (`value`): FBP slices/s on 64 slices of 2048 x 2048, 2048 angles on
+[0, pi) x 2048 bins, padding 2, ramp filter, detector pixel 2/N, noisy
+sinogram (Poisson, I0 = 1e4) already resident in HBM.  One step = one
+raft_fbp_device call over the 64 slices (pad + FFT filter + backprojection).
+
+Also reported on the same JSON line:
+  roofline      backprojection kernel: gathered-sample bytes (4 B per sample,
+                FP32 rows) + staged-row and image bytes, over its average
+                launch time from HIP events on the launch stream;
+  cpu_baseline  the FP32 restatement in oracle/ timed on this host's cores on
+                a bounded sample (whole slices of the same workload);
+  fp16_rows     the same FBP with FP16-stored filtered rows;
+  check         rel-L2 / max-abs error of the timed output against the FP64
+                oracle on a few slices;
+  em            plain eEM on 16 slices of 1024^2 (1024 angles, I0 = 1e4,
+                uniform start, 200 iterations, FP64 KL per iteration).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ssc-raft_amd"))
+
+METRIC = "FBP slices/s @2048²; EM time-to-KL, learned-init vs plain, 1 GPU"
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        vals = {}
+        with open(path) as fh:
+            for line in fh:
+                if "=" in line and not line.lstrip().startswith("#"):
+                    k, v = line.split("=", 1)
+                    vals[k.strip()] = v.strip()
+        return float(vals["hbm_bytes_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def fbp_bench(args, torch, dv, ph, oracle):
+    N = A = 2048
+    S = 64
+    dev = torch.device("cuda:0")
+    ang = ph.uniform_angles(A)
+    t0 = time.perf_counter()
+    img = ph.phantoms(N, S, config=2, device=dev)
+    p = dv.radon_device(img, ang)
+    y = ph.poisson_log_torch(p, 1.0e4, seed=2000)
+    del img, p
+    torch.cuda.synchronize()
+    log(f"inputs: {S} x {A} x {N} sinogram generated on the GPU in {time.perf_counter() - t0:.1f} s")
+    paramf, parami = oracle.fbp_params(N, A, S, pad=2, filter_type=7, pixel=2.0 / N)
+    tlo, width, fft_path = dv.fbp_plan_info(ang, paramf, parami)
+    out = torch.empty((S, N, N), dtype=torch.float32, device=dev)
+    res = {}
+    for store in ("f32", "f16"):
+        for _ in range(args.warmup):
+            dv.fbp_device(y, ang, paramf, parami, out=out, store=store)
+        torch.cuda.synchronize()
+        dv.set_profiling(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dv.fbp_device(y, ang, paramf, parami, out=out, store=store)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        bp_ms = dv.last_kernel_ms(dv.KF_BP)
+        filt_ms = dv.last_kernel_ms(dv.KF_FILTER)
+        dv.set_profiling(False)
+        res[store] = dict(seconds=dt, ms_per_step=1e3 * dt / args.steps, slices_per_s=S * args.steps / dt,
+                          bp_ms=bp_ms, filter_ms=filt_ms)
+        log(f"FBP {store}: {res[store]['slices_per_s']:.1f} slices/s, step {res[store]['ms_per_step']:.2f} ms, "
+            f"backprojection {bp_ms:.3f} ms/launch, filter {filt_ms:.3f} ms/launch")
+        if store == "f32":
+            out32 = out[: args.check_slices].cpu().numpy() if args.check_slices else None
+    # error of the timed (FP32) output against the FP64 oracle
+    check = None
+    if args.check_slices:
+        import numpy as np
+
+        ys = y[: args.check_slices].cpu().numpy()
+        pf1, pi1 = oracle.fbp_params(N, A, args.check_slices, pad=2, filter_type=7, pixel=2.0 / N)
+        t0 = time.perf_counter()
+        ref = oracle.fbp(ys, ang, pf1, pi1, prec="f64", nthreads=host_cores())
+        e = float(np.linalg.norm(out32 - ref) / np.linalg.norm(ref))
+        check = dict(slices=args.check_slices, rel_l2=e, max_abs=float(np.abs(out32 - ref).max()),
+                     ref_max_abs=float(np.abs(ref).max()))
+        log(f"check vs FP64 oracle on {args.check_slices} slices: rel-L2 {e:.3e} ({time.perf_counter() - t0:.1f} s)")
+    # CPU baseline: FP32 restatement, whole slices of the same workload
+    cpu = None
+    if not args.no_cpu:
+        ys = y[: args.cpu_slices].cpu().numpy()
+        pf1, pi1 = oracle.fbp_params(N, A, args.cpu_slices, pad=2, filter_type=7, pixel=2.0 / N)
+        cores = host_cores()
+        t0 = time.perf_counter()
+        oracle.fbp(ys, ang, pf1, pi1, prec="f32", nthreads=cores)
+        dt = time.perf_counter() - t0
+        cpu = dict(value=args.cpu_slices / dt, unit="slices/s", cores=cores, kind="port",
+                   sample=f"{args.cpu_slices} of the 64 slices (2048^2, 2048 angles, pad 2, ramp), "
+                          f"FP32 C restatement with FP64 bin indices, {dt:.2f} s")
+        log(f"CPU baseline: {cpu['value']:.3f} slices/s on {cores} threads")
+    # roofline of the backprojection kernel (FP32 rows)
+    samples = float(S) * A * N * N
+    bytes_alg = samples * 4 + float(S) * A * width * 4 + float(S) * N * N * 4
+    bp_s = res["f32"]["bp_ms"] * 1e-3
+    achieved = bytes_alg / bp_s
+    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
+                traffic=read_traffic("fbp_backproject_2048_traffic.txt"),
+                kernel="k_backproject<float,0>", bytes_per_launch=bytes_alg, samples_per_launch=samples,
+                launch_ms=res["f32"]["bp_ms"], valu_frac=(2.0 * samples / bp_s) / 157.3e12,
+                staged_row_width=width, fft_path=bool(fft_path))
+    return res, roof, cpu, check
+
+
+def em_bench(args, torch, dv, ph):
+    N = A = 1024
+    S = 16
+    K = args.em_iters
+    dev = torch.device("cuda:0")
+    ang = ph.uniform_angles(A)
+    img = ph.phantoms(N, S, config=3, device=dev)
+    p = dv.radon_device(img, ang)
+    y = torch.clamp(ph.poisson_log_torch(p, 1.0e4, seed=3000), min=0.0).contiguous()
+    del img, p
+    # warm-up (kernels, workspace)
+    x = torch.ones((S, N, N), dtype=torch.float32, device=dev)
+    dv.eem_device(y, ang, 2, obj=x)
+    torch.cuda.synchronize()
+    x = torch.ones((S, N, N), dtype=torch.float32, device=dev)
+    dv.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, kl, ex = dv.eem_device(y, ang, K, obj=x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    proj_ms = dv.last_kernel_ms(dv.KF_PROJ)
+    embp_ms = dv.last_kernel_ms(dv.KF_EMBP)
+    dv.set_profiling(False)
+    samples_it = 2.0 * S * A * N * N
+    r = dict(config="16 x 1024^2, 1024 angles x 1024 bins, I0=1e4, uniform init", iterations=K, seconds=dt,
+             ms_per_iteration=1e3 * dt / K, slice_iterations_per_s=S * K / dt,
+             samples_per_s=samples_it * K / dt, kl_first=float(kl[:, 0].sum()), kl_last=float(kl[:, -1].sum()),
+             kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms,
+             learned_init=None)
+    log(f"EM plain: {K} iterations in {dt:.2f} s ({r['ms_per_iteration']:.2f} ms/it), KL {r['kl_first']:.4e} -> "
+        f"{r['kl_last']:.4e}; projector {proj_ms:.3f} ms, backprojection+update {embp_ms:.3f} ms")
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--check-slices", type=int, default=2)
+    ap.add_argument("--cpu-slices", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-em", action="store_true")
+    ap.add_argument("--em-iters", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the slab sharding over several GPUs is "
+              f"not exercised by this benchmark.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker and the CPU baseline only
+    from sscRaft import device as dv
+    from sscRaft import phantom as ph
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    res, roof, cpu, check = fbp_bench(args, torch, dv, ph, oracle)
+    em = None if args.no_em else em_bench(args, torch, dv, ph)
+    r32 = res["f32"]
+    line = {
+        "metric": METRIC,
+        "value": r32["slices_per_s"],
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-ellipse phantoms, Radon-RT, Poisson I0=1e4)",
+        "config": {"workload": "FBP-RT 64 slices x 2048 angles x 2048 bins -> 64 x 2048^2, pad 2, ramp, "
+                               "pixel 2/N, FP32 rows", "slices": 64, "nrays": 2048, "nangles": 2048, "padding": 2,
+                   "filter": "ramp"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp16_rows": {"value": res["f16"]["slices_per_s"], "unit": "slices/s", "bp_ms": res["f16"]["bp_ms"],
+                      "filter_ms": res["f16"]["filter_ms"]},
+        "filter_ms": r32["filter_ms"],
+        "check": check,
+        "em": em,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

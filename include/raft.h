@@ -103,6 +103,13 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* Host-only planning query (no device work): the bin window the FBP
+ * backprojector stages per angle row for these parameters -- first bin *tlo
+ * and row width *width -- and whether the filter runs on the LDS FFT path
+ * (*fft_path = 1) or the direct-convolution path (0).  Returns 0 or an error. */
+int raft_fbp_plan_info(const float* angles, const float* paramf, const int* parami, int* tlo, int* width,
+                       int* fft_path);
+
 /* Average device time (ms) of the last launch of each kernel family on this
  * thread's last call, for roofline reporting: which = 0 filter, 1 FBP
  * backprojection, 2 projector, 3 EM backprojection/update.  Returns -1 if

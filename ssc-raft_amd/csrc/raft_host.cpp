@@ -358,6 +358,28 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
 // ---------------------------------------------------------------------------
 // FBP on device pointers
 // ---------------------------------------------------------------------------
+// backprojection geometry and per-angle weights dtheta_a * angular scale
+static void fbp_geometry(const FbpCfg& c, const float* angles, BpGeom& g, std::vector<float>& wa)
+{
+    const int A = c.A;
+    g.n = c.N;
+    g.nangles = A;
+    g.nbins = c.npad;
+    g.c0 = (double)c.npad / 2.0;
+    g.xc = g.c0 - (double)c.L;
+    g.cA.resize(A);
+    g.cB.resize(A);
+    wa.resize(A);
+    const double ascale = fbp_angle_scale(angles, A);
+    for (int a = 0; a < A; ++a) {
+        const double th = (double)angles[a];
+        g.cA[a] = std::cos(th);
+        g.cB[a] = -std::sin(th);
+        wa[a] = (float)(angle_step(angles, A, a) * ascale);
+    }
+    bp_window(g);
+}
+
 static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
                        const int* parami, int store, hipStream_t st)
 {
@@ -372,22 +394,8 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
 
     const int A = c.A, N = c.N;
     BpGeom g;
-    g.n = N;
-    g.nangles = A;
-    g.nbins = c.npad;
-    g.c0 = (double)c.npad / 2.0;
-    g.xc = g.c0 - (double)c.L;
-    g.cA.resize(A);
-    g.cB.resize(A);
-    std::vector<float> wa(A);
-    const double ascale = fbp_angle_scale(angles, A);
-    for (int a = 0; a < A; ++a) {
-        const double th = (double)angles[a];
-        g.cA[a] = std::cos(th);
-        g.cB[a] = -std::sin(th);
-        wa[a] = (float)(angle_step(angles, A, a) * ascale);
-    }
-    bp_window(g);
+    std::vector<float> wa;
+    fbp_geometry(c, angles, g, wa);
 
     FilterTables& F = filter_tables(D, c, st);
     DevBuf dcA, dcB, dwa;
@@ -820,6 +828,21 @@ int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* 
 {
     return guarded([&] {
         eem_device(device, d_obj, d_tomo, angles, parami, kl, kl_excluded, static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_fbp_plan_info(const float* angles, const float* paramf, const int* parami, int* tlo, int* width,
+                       int* fft_path)
+{
+    return guarded([&] {
+        const FbpCfg c = decode_fbp(paramf, parami);
+        if (!angles) fail(RAFT_EINVAL, "null angles");
+        BpGeom g;
+        std::vector<float> wa;
+        fbp_geometry(c, angles, g, wa);
+        if (tlo) *tlo = g.tlo;
+        if (width) *width = g.width;
+        if (fft_path) *fft_path = fft_plan(c.npad).ok ? 1 : 0;
     });
 }
 

@@ -5,7 +5,7 @@ import numpy
 
 from .rafttypes import CNICE, libraft, raft_check, ctypes
 
-__all__ = ["fbp_device", "radon_device", "eem_device", "set_profiling", "last_kernel_ms",
+__all__ = ["fbp_device", "radon_device", "eem_device", "set_profiling", "last_kernel_ms", "fbp_plan_info",
            "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
 
 KF_FILTER, KF_BP, KF_PROJ, KF_EMBP = 0, 1, 2, 3
@@ -99,3 +99,17 @@ def set_profiling(on):
 
 def last_kernel_ms(which):
     return libraft.raft_last_kernel_ms(which)
+
+
+def fbp_plan_info(angles, paramf, parami):
+    """(tlo, width, fft_path) of the FBP plan for these parameters (host only)."""
+    angles = CNICE(numpy.asarray(angles))
+    paramf = CNICE(numpy.asarray(paramf), numpy.float32)
+    parami = CNICE(numpy.asarray(parami), numpy.int32)
+    tlo, width, fft = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    rc = libraft.raft_fbp_plan_info(angles.ctypes.data_as(ctypes.c_void_p), paramf.ctypes.data_as(ctypes.c_void_p),
+                                    parami.ctypes.data_as(ctypes.c_void_p), ctypes.byref(tlo), ctypes.byref(width),
+                                    ctypes.byref(fft))
+    if rc:
+        raft_check("raft_fbp_plan_info")
+    return tlo.value, width.value, fft.value
