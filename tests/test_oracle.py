@@ -1,0 +1,417 @@
+"""Pins of the FP64 CPU oracle (oracle/raft_oracle.c) against what the
+mathematics fixes, independently of the oracle's own code:
+
+* exact-rational brute force of the reference's index formulas (Python
+  Fractions, the reference's expressions verbatim) on tiny inputs;
+* closed forms: single-frequency response of every filter (tent weight,
+  1/(npad*pixel), axis-offset shift), constant rows, constant sinograms,
+  delta sinograms, column sums at theta = 0, the eEM fixed point, KL on hand
+  vectors;
+* structural identities: padded-grid-then-crop == crop-only, EM forward ==
+  Radon at matching geometry, linearity;
+* a second independent restatement (oracle/oracle_np.py, numpy + pocketfft).
+A plausible slip (dropped term, wrong sign, index or offset, transposed axis)
+fails at least one of these.
+"""
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+import oracle
+import oracle_np
+
+RNG = np.random.default_rng(1234)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, float) - np.asarray(b, float)) / max(np.linalg.norm(b), 1e-300))
+
+
+def trunc_fr(q):
+    """C (int) cast of an exact rational: truncation toward zero."""
+    return int(q) if q >= 0 else -int(-q)
+
+
+def cs_fr(theta32):
+    th = float(np.float32(theta32))
+    return Fr(math.cos(th)), Fr(math.sin(th))
+
+
+def dtheta(ang):
+    a = [Fr(float(np.float32(v))) for v in ang]
+    return [(a[i + 1] - a[i]) if i < len(a) - 1 else (a[i] - a[i - 1]) for i in range(len(a))]
+
+
+# ----------------------------------------------------------------------------
+# FBP
+# ----------------------------------------------------------------------------
+
+def fbp_exact_none(tomo, ang, pad, pixel):
+    """fbp.cu:38-74 + padding.cu:140-178,245-268 with filter 'none', in exact
+    rationals (the padded object grid, then the crop)."""
+    S, A, N = tomo.shape
+    npad = N * (1 + pad)
+    L = (pad * N) // 2
+    h = Fr(float(np.float32(pixel)))
+    xmin = -h * npad / 2
+    tmin = -h * npad / 2
+    dth = dtheta(ang)
+    scale = Fr(math.pi) / Fr(float(np.float32(ang[-1]))) if np.float32(ang[-1]) > np.float32(math.pi) else Fr(1)
+    out = np.zeros((S, N, N))
+    for a in range(A):
+        c, s = cs_fr(ang[a])
+        for jj in range(N):
+            J = jj + L
+            y = xmin + J * h
+            for ii in range(N):
+                I = ii + L
+                x = xmin + I * h
+                t = x * c - y * s
+                T = trunc_fr((t - tmin) / h)
+                if 0 <= T < npad:
+                    m = min(max(T - L, 0), N - 1)  # padded row value u[T] = p[clamp(T - L)]
+                    out[:, jj, ii] += tomo[:, a, m].astype(np.float64) * float(dth[a] * scale)
+    return out
+
+
+@pytest.mark.parametrize("n,pad", [(7, 0), (8, 1), (9, 2), (6, 3), (10, 2)])
+def test_fbp_indices_exact_rational(n, pad):
+    A = 6
+    ang = np.concatenate([[0.0], np.sort(RNG.uniform(0.05, 3.1, A - 1))]).astype(np.float32)
+    tomo = RNG.standard_normal((2, A, n)).astype(np.float32)
+    pixel = 0.37
+    paramf, parami = oracle.fbp_params(n, A, 2, pad=pad, filter_type=0, pixel=pixel)
+    got = oracle.fbp(tomo, ang, paramf, parami)
+    ref = fbp_exact_none(tomo, ang, pad, pixel)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_fbp_angle_scale_over_pi():
+    n, A = 8, 5
+    ang = np.array([0.0, 1.0, 2.5, 4.0, 5.5], np.float32)  # last > pi -> pi/theta_last (fbp.cu:24-28)
+    tomo = RNG.standard_normal((1, A, n)).astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, A, 1, pad=1, filter_type=0, pixel=1.0)
+    np.testing.assert_allclose(oracle.fbp(tomo, ang, paramf, parami), fbp_exact_none(tomo, ang, 1, 1.0),
+                               rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("pad", [1, 2, 3])
+def test_fbp_constant_sinogram(pad):
+    """filter 'none', q = c: every pixel sees every angle (pad >= 1) -> c * sum(dtheta)."""
+    n, A, c = 24, 30, 0.75
+    ang = np.linspace(0, np.pi, A, endpoint=False).astype(np.float32)
+    tomo = np.full((1, A, n), c, np.float32)
+    paramf, parami = oracle.fbp_params(n, A, 1, pad=pad, filter_type=0, pixel=1.0)
+    out = oracle.fbp(tomo, ang, paramf, parami)
+    expect = c * float(sum(dtheta(ang)))
+    np.testing.assert_allclose(out, expect, rtol=1e-13)
+
+
+def test_fbp_delta_sinogram():
+    """A single nonzero bin lights exactly the pixels whose bin is that bin."""
+    n, A, pad = 16, 5, 1
+    ang = np.array([0.0, 0.4, 1.1, 2.0, 2.9], np.float32)
+    npad = n * (1 + pad)
+    L = (pad * n) // 2
+    for a, m in [(0, 5), (2, 9), (4, 0)]:
+        tomo = np.zeros((1, A, n), np.float32)
+        tomo[0, a, m] = 1.0
+        paramf, parami = oracle.fbp_params(n, A, 1, pad=pad, filter_type=0, pixel=1.0)
+        out = oracle.fbp(tomo, ang, paramf, parami)[0]
+        c, s = cs_fr(ang[a])
+        half = Fr(npad, 2)
+        lit = np.zeros((n, n), bool)
+        for j in range(n):
+            for i in range(n):
+                T = trunc_fr((i + L - half) * c - (j + L - half) * s + half)
+                # padded value at T is p[clamp(T - L)]: nonzero only where clamp(T-L) == m
+                lit[j, i] = min(max(T - L, 0), n - 1) == m
+        assert np.array_equal(out != 0, lit)
+
+
+def test_fbp_padded_grid_equals_crop_only():
+    """fbp.cu's npad x npad object grid followed by remove_paddR2R == the crop-only form."""
+    for n, pad in [(12, 1), (13, 2), (10, 3)]:
+        A = 9
+        ang = np.sort(RNG.uniform(0.0, 3.1, A)).astype(np.float32)
+        tomo = RNG.standard_normal((2, A, n)).astype(np.float32)
+        for ft in (0, 7):
+            paramf, parami = oracle.fbp_params(n, A, 2, pad=pad, filter_type=ft, pixel=0.3)
+            a = oracle.fbp(tomo, ang, paramf, parami)
+            b = oracle.fbp(tomo, ang, paramf, parami, padded=True)
+            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+# ---- filter stage -----------------------------------------------------------
+
+APPLY = {
+    1: lambda w, reg, pag: w * math.exp(-float(np.float32(0.693)) * reg * w * w) / (1 + pag * w * w),
+    2: lambda w, reg, pag: w / (1 + reg * w * w) / (1 + pag * w * w),
+    3: lambda w, reg, pag: w * math.cos(math.pi * 0.5 * w) / (1 + pag * w * w),
+    4: lambda w, reg, pag: w * (math.sin(max(w * reg * math.pi * 0.5, float(np.float32(1e-4))))
+                                / max(w * reg * math.pi * 0.5, float(np.float32(1e-4)))) / (1 + pag * w * w),
+    5: lambda w, reg, pag: w * (0.5 + 0.5 * math.cos(2 * math.pi * w)) / (1 + pag * w * w),
+    6: lambda w, reg, pag: w * (float(np.float32(0.54)) + float(np.float32(0.46)) * math.cos(2 * math.pi * w))
+    / (1 + pag * w * w),
+    7: lambda w, reg, pag: w / (1 + pag * w * w),
+}
+
+
+@pytest.mark.parametrize("ftype", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("k0", [1, 5, 11, 17])
+def test_filter_single_frequency(ftype, k0):
+    """pad 0, p[m] = cos(2 pi k0 m / n): q[m] = apply(w_k0)/h * cos(2 pi k0 (m - c)/n),
+    w_k = 0.5 min(k, M-k)/M, M = n/2 + 1 (filters.cu:44-50, 184-186)."""
+    n, h, c, reg = 48, 0.37, 1.3, 0.9
+    bd_inv, energy, z2 = 1.0 / 40.0, 2.0e4, 0.5
+    m = np.arange(n)
+    tomo = np.cos(2 * np.pi * k0 * m / n)[None, None, :].astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, 2, 1, pad=0, filter_type=ftype, pixel=h, offset=c, reg=reg,
+                                       beta_delta=40.0, energy=energy, z2=z2)
+    q = oracle.fbp_filter(np.repeat(tomo, 2, axis=1), paramf, parami)[0, 0]
+    M = n // 2 + 1
+    w = 0.5 * min(k0, M - k0) / M
+    lam = 4.135667696e-15 * 299792458.0 / float(np.float32(energy))
+    h32 = float(np.float32(h))
+    pag = lam * float(np.float32(z2)) * math.pi * float(np.float32(bd_inv)) / (h32 * h32)
+    amp = APPLY[ftype](w, float(np.float32(reg)), pag)
+    c32 = float(np.float32(c))
+    expect = amp / h32 * np.cos(2 * np.pi * k0 * (m - c32) / n)
+    # the input is float32-rounded cos: tolerance covers that rounding
+    np.testing.assert_allclose(q, expect, rtol=0, atol=2e-6 * max(abs(amp / h32), 1e-30) * 10)
+
+
+def test_filter_constant_row_is_zero_and_rows_sum_to_zero():
+    n, A = 40, 3
+    for ft in range(1, 8):
+        for pad in (0, 1, 2):
+            tomo = np.full((1, A, n), 2.5, np.float32)
+            paramf, parami = oracle.fbp_params(n, A, 1, pad=pad, filter_type=ft, pixel=1.0, reg=0.4)
+            q = oracle.fbp_filter(tomo, paramf, parami)
+            assert np.abs(q).max() < 1e-13
+            tomo = RNG.standard_normal((1, A, n)).astype(np.float32)
+            q = oracle.fbp_filter(tomo, paramf, parami)
+            assert np.abs(q.sum(axis=-1)).max() < 1e-11 * np.abs(q).sum()
+
+
+@pytest.mark.parametrize("n,pad", [(9, 0), (10, 1), (11, 2), (8, 3)])
+def test_filter_none_is_edge_replicate_pad(n, pad):
+    """'none': no filtering and no scaling (fbp.cu:102-103): the padded row itself."""
+    tomo = RNG.standard_normal((2, 3, n)).astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, 3, 2, pad=pad, filter_type=0, pixel=0.5)
+    q = oracle.fbp_filter(tomo, paramf, parami)
+    npad, L = n * (1 + pad), (pad * n) // 2
+    for mm in range(npad):
+        np.testing.assert_array_equal(q[..., mm], tomo[..., min(max(mm - L, 0), n - 1)])
+
+
+def test_filter_linearity():
+    n, A = 30, 4
+    # small dyadic values so that a*p1 + b*p2 is exact in float32
+    p1 = (RNG.integers(-100, 100, (1, A, n)) / 8.0).astype(np.float32)
+    p2 = (RNG.integers(-100, 100, (1, A, n)) / 8.0).astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, A, 1, pad=2, filter_type=5, pixel=0.2, offset=0.3)
+    a, b = 0.5, -2.0  # exact in float32
+    lhs = oracle.fbp_filter((a * p1 + b * p2).astype(np.float32), paramf, parami)
+    rhs = a * oracle.fbp_filter(p1, paramf, parami) + b * oracle.fbp_filter(p2, paramf, parami)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("ftype", range(8))
+@pytest.mark.parametrize("n,pad", [(16, 0), (17, 2), (33, 1), (31, 3), (20, 2)])
+def test_fbp_matches_numpy_restatement(ftype, n, pad):
+    A = 11
+    ang = np.sort(RNG.uniform(0, np.pi, A)).astype(np.float32)
+    tomo = RNG.standard_normal((2, A, n)).astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, A, 2, pad=pad, filter_type=ftype, pixel=0.21, offset=-0.6, reg=0.7,
+                                       beta_delta=25.0, energy=1.5e4, z2=0.3)
+    np.testing.assert_allclose(oracle.fbp_filter(tomo, paramf, parami), oracle_np.filter_rows(tomo, paramf, parami),
+                               rtol=1e-9, atol=1e-11)
+    assert rel(oracle.fbp(tomo, ang, paramf, parami), oracle_np.fbp(tomo, ang, paramf, parami)) < 1e-12
+
+
+def test_fbp_f32_restatement_close_to_f64():
+    n, A = 64, 48
+    ang = np.linspace(0, np.pi, A, endpoint=False).astype(np.float32)
+    tomo = RNG.standard_normal((1, A, n)).astype(np.float32)
+    paramf, parami = oracle.fbp_params(n, A, 1, pad=2, filter_type=7, pixel=2.0 / n)
+    assert rel(oracle.fbp(tomo, ang, paramf, parami, prec="f32"), oracle.fbp(tomo, ang, paramf, parami)) < 1e-5
+
+
+# ----------------------------------------------------------------------------
+# Radon
+# ----------------------------------------------------------------------------
+
+def radon_exact(img, ang, ax=1.0, ay=1.0):
+    """radon_rt.cu:131-174 verbatim in exact rationals."""
+    S, Ny, Nx = img.shape
+    ax, ay = Fr(float(np.float32(ax))), Fr(float(np.float32(ay)))
+    dx, dy = 2 * ax / (Nx - 1), 2 * ay / (Ny - 1)
+    out = np.zeros((S, len(ang), Nx))
+    for a, th in enumerate(ang):
+        c, s = cs_fr(th)
+        for i in range(Nx):
+            t = -ax + i * dx
+            acc = np.zeros(S)
+            for k in range(Ny):
+                sk = -ay + k * dy
+                x = t * c - sk * s
+                y = t * s + sk * c
+                X = trunc_fr((x + 1) / dx)
+                Y = trunc_fr((y + 1) / dy)
+                if 0 <= X < Nx and 0 <= Y < Ny:
+                    acc += img[:, Y, X]
+            out[:, a, i] = acc * float(dy)
+    return out
+
+
+@pytest.mark.parametrize("ny,nx", [(7, 7), (8, 8), (6, 9), (9, 5)])
+def test_radon_exact_rational(ny, nx):
+    ang = np.concatenate([[0.0], RNG.uniform(-1.0, 7.0, 5)]).astype(np.float32)
+    img = RNG.standard_normal((2, ny, nx)).astype(np.float32)
+    np.testing.assert_allclose(oracle.radon(img, ang), radon_exact(img, ang), rtol=1e-12, atol=1e-12)
+
+
+def test_radon_theta0_is_column_sums():
+    for n in (8, 9, 32, 33):
+        img = RNG.random((2, n, n)).astype(np.float32)
+        p = oracle.radon(img, np.array([0.0], np.float32))
+        np.testing.assert_allclose(p[:, 0, :], img.astype(np.float64).sum(axis=1) * 2.0 / (n - 1), rtol=1e-12)
+
+
+def test_radon_disk_line_integrals():
+    """Loose analytic check of the scaling: a centred disk of radius r integrates to
+    2 sqrt(r^2 - t^2) along a line at distance t (pixel = 2/N in the Python wrapper)."""
+    n, r = 257, 0.6
+    g = -1 + np.arange(n) * 2.0 / (n - 1)
+    disk = ((g[None, :] ** 2 + g[:, None] ** 2) <= r * r).astype(np.float32)[None]
+    ang = np.array([0.0, 0.7, 1.9], np.float32)
+    p = oracle.radon(disk, ang)
+    t = g
+    expect = 2 * np.sqrt(np.maximum(r * r - t * t, 0))
+    inner = np.abs(t) < r - 0.05  # the chord is steep at the rim
+    for a in range(3):
+        assert np.abs(p[0, a] - expect)[inner].max() < 0.03
+        assert abs(p[0, a].sum() * 2.0 / (n - 1) - math.pi * r * r) < 0.01 * math.pi * r * r
+
+
+def test_radon_matches_numpy_restatement():
+    for ny, nx, A in [(16, 16, 20), (33, 33, 17), (20, 32, 19), (31, 17, 23)]:
+        img = RNG.random((2, ny, nx)).astype(np.float32)
+        ang = np.sort(RNG.uniform(0, 2 * np.pi, A)).astype(np.float32)
+        np.testing.assert_allclose(oracle.radon(img, ang), oracle_np.radon(img, ang), rtol=1e-12, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------
+# eEM
+# ----------------------------------------------------------------------------
+
+def backones_exact(ang, n, nrays):
+    """kernel_backprojectionOfOnes (em_RT_kernel.cu:303-350) in exact rationals."""
+    dxy, dt = Fr(2, n - 1), Fr(2, nrays - 1)
+    dth = dtheta(ang)
+    out = np.zeros((n, n))
+    for a, th in enumerate(ang):
+        c, s = cs_fr(th)
+        for j in range(n):
+            for i in range(n):
+                x, y = -1 + i * dxy, -1 + j * dxy
+                T = trunc_fr((x * c + y * s + 1) / dt)
+                if 0 <= T < nrays:
+                    out[j, i] += float(dth[a])
+    return out
+
+
+@pytest.mark.parametrize("n", [7, 8, 9])
+def test_em_backones_exact_rational(n):
+    ang = np.concatenate([[0.0], np.sort(RNG.uniform(0.1, 3.0, 4))]).astype(np.float32)
+    got = oracle.em_backproject(np.zeros((5, n)), ang, n, ones=True)
+    np.testing.assert_allclose(got, backones_exact(ang, n, n), rtol=1e-13, atol=1e-13)
+
+
+def test_em_backprojection_exact_rational():
+    n = 8
+    ang = np.concatenate([[0.0], np.sort(RNG.uniform(0.1, 3.0, 4))]).astype(np.float32)
+    q = RNG.standard_normal((5, n))
+    dxy = dt = Fr(2, n - 1)
+    dth = dtheta(ang)
+    ref = np.zeros((n, n))
+    for a, th in enumerate(ang):
+        c, s = cs_fr(th)
+        for j in range(n):
+            for i in range(n):
+                T = trunc_fr(((-1 + i * dxy) * c + (-1 + j * dxy) * s + 1) / dt)
+                if 0 <= T < n:
+                    ref[j, i] += q[a, T] * float(dth[a])
+    np.testing.assert_allclose(oracle.em_backproject(q, ang, n), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_em_forward_is_radon_geometry():
+    """kernel_radonWithDivision samples exactly like Radon_RT with a = 1 and nrays = N."""
+    for n in (16, 17):
+        x = RNG.random((n, n))
+        ang = np.sort(RNG.uniform(0, np.pi, 13)).astype(np.float32)
+        v = oracle.em_forward(x, ang, n)
+        r = oracle.radon(x.astype(np.float32)[None], ang)[0]
+        # radon takes float32 input; use a float32-exact image for the comparison
+        x32 = x.astype(np.float32).astype(np.float64)
+        np.testing.assert_allclose(oracle.em_forward(x32, ang, n), r, rtol=1e-12, atol=1e-12)
+        assert v.shape == r.shape
+
+
+def test_em_fixed_point():
+    """y = A x*: one update keeps x* up to the 1e-8 smoothing of APPROXINVX."""
+    n, A = 24, 30
+    ang = np.linspace(0, np.pi, A, endpoint=False).astype(np.float32)
+    g = -1 + np.arange(n) * 2.0 / (n - 1)
+    xs = (1.0 + 0.5 * np.cos(3 * g[None, :]) * np.sin(2 * g[:, None]))
+    y = oracle.em_forward(xs, ang, n).astype(np.float32)
+    y64 = y.astype(np.float64)
+    # rebuild x* exactly consistent with the float32 y is not possible; use the
+    # float32 y as data and start from x*: the ratio is y/(A x*) ~ 1 +- 1e-7
+    x1, kl, _ = oracle.eem(y[None], ang, 1, obj=xs[None])
+    assert rel(x1[0], xs) < 1e-6
+    assert kl[0, 0] < 1e-10 * y64.sum()
+
+
+def test_kl_closed_form():
+    y = np.array([0.0, 1.0, 2.0, 3.0, -1.0, 4.0])
+    v = np.array([0.5, 1.0, 4.0, 0.0, 2.0, 2.0])
+    val, ex = oracle.kl(y, v)
+    expect = 0.5 + 0.0 + (4 - 2 + 2 * math.log(2 / 4)) + (2 - 4 + 4 * math.log(4 / 2))
+    assert ex == 2  # v = 0 and y < 0
+    assert abs(val - expect) < 1e-14
+    assert oracle_np.kl(y, v) == pytest.approx((expect, 2))
+
+
+def test_kl_order_independent():
+    y = RNG.random(100000) * 5
+    v = y * (1 + 0.01 * RNG.standard_normal(100000)) + 1e-3
+    a, _ = oracle.kl(y, v)
+    p = RNG.permutation(y.size)
+    b, _ = oracle.kl(y[p], v[p])
+    assert abs(a - b) <= 1e-14 * abs(a)
+
+
+def test_eem_matches_numpy_restatement():
+    for n, A in [(16, 16), (17, 23)]:
+        ang = np.linspace(0, np.pi, A, endpoint=False).astype(np.float32)
+        y = RNG.random((2, A, n)).astype(np.float32)
+        x1, k1, _ = oracle.eem(y, ang, 4)
+        x2, k2 = oracle_np.eem(y, ang, 4)
+        assert rel(x1, x2) < 1e-12
+        np.testing.assert_allclose(k1, k2, rtol=1e-12)
+
+
+def test_eem_kl_decreases_on_phantom():
+    """Soft sanity only: B != A^T, so MLEM's monotone KL is not guaranteed."""
+    from sscRaft import phantom as ph
+
+    n = A = 32
+    img = ph.phantoms(n, 1, config=7)
+    ang = ph.uniform_angles(A)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=3), 0)
+    _, kl, _ = oracle.eem(y, ang, 8)
+    assert kl[0, -1] < kl[0, 0]
