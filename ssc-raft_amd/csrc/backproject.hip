@@ -1,17 +1,22 @@
 // backproject.hip -- pixel-driven backprojection for FBP-RT and the eEM update
-// (gfx950, wave64).
+// (gfx950, wave64), plus the layout kernels around it.
 //
-// One workgroup = 256 threads = one 64 x 64 pixel tile of one slice; each
-// thread owns 4 x 4 pixels at (tx + 16p, ty + 16q), so a wave covers a 16 x 4
-// pixel patch for every (p, q) and its 64 bin lookups of one angle fall into
-// at most 16|cos| + 4|sin| + 2 consecutive LDS words (bank-conflict free).
-// Angles are processed in chunks of BP_CHUNK: each wave stages the bin
-// window that the tile touches for its share of the chunk's angles into LDS
-// (<= BP_SLOT words per angle, coalesced global loads), then every thread
-// sweeps the chunk:
-//     u = fma(X_p, cA, fma(Y_q, cB, c0));  T = (int)u;  acc += win[T]
+// Slice interleaving.  Every slice of a call shares the geometry, so the
+// kernel computes each bin index once and gathers the same bin of a group of
+// slices with one 16-byte LDS read: staged rows are stored slice-interleaved,
+// [group][angle][bin][SB] with SB = 4 (FP32 rows) or 8 (FP16 rows).
+//
+// One workgroup = 256 threads = one 64 x 32 pixel tile of one slice group;
+// each thread owns 4 x 2 pixels at (tx + 16p, ty + 16q), so a wave covers a
+// 16 x 4 pixel patch for every (p, q) and its 64 bin lookups of one angle fall
+// into at most 16|cos| + 4|sin| + 2 consecutive 16-byte LDS entries.  Angles
+// are processed in chunks of BP_CHUNK: the first threads compute the bin
+// window the tile touches for one angle each (from the tile corners), then
+// all threads copy the windows (<= BP_SLOT entries per angle, coalesced
+// 16-byte loads) into LDS, then every thread sweeps the chunk:
+//     u = fma(X_p, cA, fma(Y_q, cB, c0));  T = (int)u;  acc[0..SB) += win[T]
 // in IEEE double for the bin coordinate (bit-identical to the CPU oracle's
-// expression), FP32 for the values.  The staged rows already carry
+// expression) and FP32 for the values.  The staged rows already carry
 // dtheta_a * angular scale and zeros outside the valid bin range, so the
 // inner loop has no range test and no multiply.  Per-chunk partial sums are
 // added to the running total (two-level summation over angles).
@@ -25,67 +30,102 @@ namespace raft {
 
 namespace {
 
-template <typename QT> __device__ __forceinline__ float to_f32(QT v);
-template <> __device__ __forceinline__ float to_f32<float>(float v) { return v; }
-template <> __device__ __forceinline__ float to_f32<__half>(__half v) { return __half2float(v); }
-
-template <typename QT> __device__ __forceinline__ QT zero_q();
-template <> __device__ __forceinline__ float zero_q<float>() { return 0.0f; }
-template <> __device__ __forceinline__ __half zero_q<__half>() { return __float2half(0.0f); }
-
 struct BpKArgs {
-    const void* q;
-    float* out;
-    const float* x;
-    const float* norm;
-    float* xt;
+    const uint4* q;     // [G][A][W] entries of SB values
+    float* out;         // MODE 0: [S][n][n]; MODE 1: interleaved [G][n][n][4]
+    const float* x;     // MODE 1: interleaved current image [G][n][n][4] (may alias out)
+    const float* norm;  // MODE 1: backones [n][n] (shared by all slices)
+    float* xt;          // MODE 1: interleaved transposed image [G][n][n][4] (or null)
     const double* cA;
     const double* cB;
     double xc, c0;
-    int n, A, tlo, W;
+    int n, A, tlo, W, S;
     int tiles_x;
-    long long norm_stride; // elements between slices of norm (0 = shared)
+};
+
+// accumulate one 16-byte staged entry into SB running sums
+template <typename QT> struct Acc;
+template <> struct Acc<float> {
+    static constexpr int SB = 4;
+    __device__ static __forceinline__ void add(float* acc, const uint4& e)
+    {
+        acc[0] += __uint_as_float(e.x);
+        acc[1] += __uint_as_float(e.y);
+        acc[2] += __uint_as_float(e.z);
+        acc[3] += __uint_as_float(e.w);
+    }
+};
+// FP16 entries: acc += (float)half in one v_fma_mix_f32 (f16 source selected from
+// the low or high half of the dword, times 1.0, plus the f32 accumulator)
+__device__ __forceinline__ float mix_lo(unsigned v, float acc)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(v), "v"(acc));
+    return r;
+}
+__device__ __forceinline__ float mix_hi(unsigned v, float acc)
+{
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(v), "v"(acc));
+    return r;
+}
+template <> struct Acc<__half> {
+    static constexpr int SB = 8;
+    __device__ static __forceinline__ void add(float* acc, const uint4& e)
+    {
+        acc[0] = mix_lo(e.x, acc[0]);
+        acc[1] = mix_hi(e.x, acc[1]);
+        acc[2] = mix_lo(e.y, acc[2]);
+        acc[3] = mix_hi(e.y, acc[3]);
+        acc[4] = mix_lo(e.z, acc[4]);
+        acc[5] = mix_hi(e.z, acc[5]);
+        acc[6] = mix_lo(e.w, acc[6]);
+        acc[7] = mix_hi(e.w, acc[7]);
+    }
 };
 
 template <typename QT, int MODE>
 __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
 {
-    __shared__ QT win[BP_CHUNK * BP_SLOT];
+    constexpr int SB = Acc<QT>::SB;
+    constexpr int TX = BP_TILE_X, TY = BP_TILE_Y;
+    __shared__ uint4 win[BP_CHUNK * BP_SLOT];
     __shared__ double s_cA[BP_CHUNK], s_cB[BP_CHUNK];
-    __shared__ int s_base[BP_CHUNK];
+    __shared__ int s_tl[BP_CHUNK], s_w[BP_CHUNK], s_boff[BP_CHUNK];
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    const int wave = tid >> 6, lane = tid & 63;
     const int tile = blockIdx.x;
-    const int s = blockIdx.y;
-    const int i0 = (tile % a.tiles_x) * BP_TILE;
-    const int j0 = (tile / a.tiles_x) * BP_TILE;
+    const int g = blockIdx.y;
+    const int i0 = (tile % a.tiles_x) * TX;
+    const int j0 = (tile / a.tiles_x) * TY;
 
-    double X[4], Y[4];
+    double X[4], Y[2];
 #pragma unroll
     for (int p = 0; p < 4; ++p) X[p] = (double)(i0 + tx + 16 * p) - a.xc;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Y[q] = (double)(j0 + ty + 16 * q) - a.xc;
-    const double Xmin = (double)i0 - a.xc, Xmax = (double)(i0 + BP_TILE - 1) - a.xc;
-    const double Ymin = (double)j0 - a.xc, Ymax = (double)(j0 + BP_TILE - 1) - a.xc;
+    for (int q = 0; q < 2; ++q) Y[q] = (double)(j0 + ty + 16 * q) - a.xc;
 
-    const QT* qs = static_cast<const QT*>(a.q) + (size_t)s * a.A * a.W;
+    const uint4* qg = a.q + (size_t)g * a.A * a.W;
     const int thi_g = a.tlo + a.W - 1;
 
-    float tot[4][4];
+    float tot[2][4][SB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int p = 0; p < 4; ++p) tot[q][p] = 0.0f;
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int s = 0; s < SB; ++s) tot[q][p][s] = 0.0f;
 
     for (int a0 = 0; a0 < a.A; a0 += BP_CHUNK) {
         const int nA = min(BP_CHUNK, a.A - a0);
         __syncthreads();
-        // ---- stage the chunk's bin windows (wave w: angles w, w+4, ...) ----
-        for (int al = wave; al < nA; al += 4) {
-            const int ang = a0 + al;
+        // ---- per-angle bin windows of this tile (one thread per angle) ----
+        if (tid < nA) {
+            const int ang = a0 + tid;
             const double cA = a.cA[ang], cB = a.cB[ang];
+            const double Xmin = (double)i0 - a.xc, Xmax = (double)(i0 + TX - 1) - a.xc;
+            const double Ymin = (double)j0 - a.xc, Ymax = (double)(j0 + TY - 1) - a.xc;
             const double u00 = fma(Xmin, cA, fma(Ymin, cB, a.c0));
             const double u10 = fma(Xmax, cA, fma(Ymin, cB, a.c0));
             const double u01 = fma(Xmin, cA, fma(Ymax, cB, a.c0));
@@ -93,102 +133,163 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
             const double umin = fmin(fmin(u00, u10), fmin(u01, u11));
             const double umax = fmax(fmax(u00, u10), fmax(u01, u11));
             // any pixel's trunc(u) lies in [floor(umin)-1, floor(umax)+2]
-            int tl = (int)floor(umin) - 1;
-            int th = (int)floor(umax) + 2;
-            tl = max(tl, a.tlo);
-            th = min(th, thi_g);
-            const int width = th - tl + 1;
-            const QT* src = qs + (size_t)ang * a.W + (tl - a.tlo);
-            QT* dst = win + al * BP_SLOT;
-            dst[lane] = lane < width ? src[lane] : zero_q<QT>();
-            if (lane + 64 < BP_SLOT) dst[lane + 64] = (lane + 64) < width ? src[lane + 64] : zero_q<QT>();
-            if (lane == 0) {
-                s_cA[al] = cA;
-                s_cB[al] = cB;
-                s_base[al] = al * BP_SLOT - tl;
-            }
+            const int tl = max((int)floor(umin) - 1, a.tlo);
+            const int th = min((int)floor(umax) + 2, thi_g);
+            s_cA[tid] = cA;
+            s_cB[tid] = cB;
+            s_tl[tid] = tl;
+            s_w[tid] = th - tl + 1;
+            s_boff[tid] = (tid * BP_SLOT - tl) * (int)sizeof(uint4); // wraps for large tl: used modulo 2^32
+
+        }
+        __syncthreads();
+        // ---- stage the windows: 16-byte coalesced loads ----
+        for (int e = tid; e < nA * BP_SLOT; e += 256) {
+            const int al = e / BP_SLOT;
+            const int w = e - al * BP_SLOT;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (w < s_w[al]) v = qg[(size_t)(a0 + al) * a.W + (s_tl[al] - a.tlo) + w];
+            win[e] = v;
         }
         __syncthreads();
         // ---- sweep ----
-        float part[4][4];
+        float part[2][4][SB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) part[q][p] = 0.0f;
-#pragma unroll 2
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int s = 0; s < SB; ++s) part[q][p][s] = 0.0f;
+        const char* wbase = reinterpret_cast<const char*>(win);
         for (int al = 0; al < nA; ++al) {
             const double cA = s_cA[al], cB = s_cB[al];
-            const QT* wb = win + s_base[al];
+            // byte offset of bin 0 of this angle's window: entry (al, T - tl)
+            const unsigned boff = (unsigned)s_boff[al];
+            int T[2][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < 2; ++q) {
                 const double rt = fma(Y[q], cB, a.c0);
 #pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    const int T = (int)fma(X[p], cA, rt);
-                    part[q][p] += to_f32<QT>(wb[T]);
-                }
+                for (int p = 0; p < 4; ++p) T[q][p] = (int)fma(X[p], cA, rt);
             }
+            uint4 v[2][4];
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+                    v[q][p] = *reinterpret_cast<const uint4*>(wbase + (((unsigned)T[q][p] << 4) + boff));
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int p = 0; p < 4; ++p) Acc<QT>::add(part[q][p], v[q][p]);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int p = 0; p < 4; ++p) tot[q][p] += part[q][p];
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int s = 0; s < SB; ++s) tot[q][p][s] += part[q][p][s];
     }
 
     // ---- epilogue ----
-    const size_t so = (size_t)s * a.n * a.n;
+    const size_t nn = (size_t)a.n * a.n;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2; ++q) {
         const int j = j0 + ty + 16 * q;
         if (j >= a.n) continue;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             const int i = i0 + tx + 16 * p;
             if (i >= a.n) continue;
-            const size_t idx = so + (size_t)j * a.n + i;
             if (MODE == 1) {
-                const float nv = a.norm[(size_t)s * a.norm_stride + (size_t)j * a.n + i];
-                const float xv = a.x[idx] * tot[q][p] / nv;
-                a.out[idx] = xv;
-                if (a.xt) a.xt[so + (size_t)i * a.n + j] = xv;
+                // eEM update on the interleaved image (SB == 4 here)
+                const size_t pix = ((size_t)g * nn + (size_t)j * a.n + i) * 4;
+                const float nv = a.norm[(size_t)j * a.n + i];
+                float4 xv = *reinterpret_cast<const float4*>(a.x + pix);
+                xv.x = xv.x * tot[q][p][0] / nv;
+                xv.y = xv.y * tot[q][p][1] / nv;
+                xv.z = xv.z * tot[q][p][2] / nv;
+                xv.w = xv.w * tot[q][p][3] / nv;
+                *reinterpret_cast<float4*>(a.out + pix) = xv;
+                if (a.xt) *reinterpret_cast<float4*>(a.xt + ((size_t)g * nn + (size_t)i * a.n + j) * 4) = xv;
             } else {
-                a.out[idx] = tot[q][p];
+#pragma unroll
+                for (int s = 0; s < SB; ++s) {
+                    const int sl = g * SB + s;
+                    if (sl < a.S) a.out[(size_t)sl * nn + (size_t)j * a.n + i] = tot[q][p][s];
+                }
             }
         }
     }
 }
 
-// rows for B(1): q[a][w] = wa[a] where 0 <= tlo + w < nbins, else 0
-__global__ void k_fill_rows(float* q, const float* wa, int S, int A, int W, int tlo, int nbins)
+// rows for B(1), interleaved by 4: q[a][w][0..4) = wa[a] where 0 <= tlo + w < nbins
+__global__ void k_fill_rows(float4* q, const float* wa, int A, int W, int tlo, int nbins)
 {
-    const size_t total = (size_t)S * A * W;
+    const size_t total = (size_t)A * W;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
         const int w = (int)(e % W);
-        const int ang = (int)((e / W) % A);
+        const int ang = (int)(e / W);
         const int T = tlo + w;
-        q[e] = (T >= 0 && T < nbins) ? wa[ang] : 0.0f;
+        const float v = (T >= 0 && T < nbins) ? wa[ang] : 0.0f;
+        q[e] = make_float4(v, v, v, v);
     }
 }
 
-// out[s][c][r] = in[s][r][c], 32 x 32 LDS tiles
-__global__ __launch_bounds__(256) void k_transpose(const float* in, float* out, int rows, int cols)
+// planar [S][rows][cols] <-> interleaved [G][rows][cols][4] (optionally transposed:
+// interleaved [G][cols][rows][4]); missing slices of the last group read as 0.
+// 32 x 32 tiles through LDS.
+template <int TO_IL, int TRANSPOSE>
+__global__ __launch_bounds__(256) void k_layout(const float* in, float* out, int S, int rows, int cols)
 {
-    __shared__ float t[32][33];
-    const int s = blockIdx.z;
+    __shared__ float4 t[32][33];
+    const int g = blockIdx.z;
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
-    const float* src = in + (size_t)s * rows * cols;
-    float* dst = out + (size_t)s * rows * cols;
+    const size_t plane = (size_t)rows * cols;
+    if (TO_IL) {
 #pragma unroll
-    for (int k = 0; k < 32; k += 8) {
-        const int r = r0 + ty + k, c = c0 + tx;
-        if (r < rows && c < cols) t[ty + k][tx] = src[(size_t)r * cols + c];
-    }
-    __syncthreads();
+        for (int k = 0; k < 32; k += 8) {
+            const int r = r0 + ty + k, c = c0 + tx;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < rows && c < cols) {
+                const size_t o = (size_t)r * cols + c;
+                const int s = g * 4;
+                v.x = (s + 0 < S) ? in[(size_t)(s + 0) * plane + o] : 0.f;
+                v.y = (s + 1 < S) ? in[(size_t)(s + 1) * plane + o] : 0.f;
+                v.z = (s + 2 < S) ? in[(size_t)(s + 2) * plane + o] : 0.f;
+                v.w = (s + 3 < S) ? in[(size_t)(s + 3) * plane + o] : 0.f;
+            }
+            t[ty + k][tx] = v;
+        }
+        __syncthreads();
+        float4* dst = reinterpret_cast<float4*>(out) + (size_t)g * plane;
 #pragma unroll
-    for (int k = 0; k < 32; k += 8) {
-        const int c = c0 + ty + k, r = r0 + tx;
-        if (r < rows && c < cols) dst[(size_t)c * rows + r] = t[tx][ty + k];
+        for (int k = 0; k < 32; k += 8) {
+            if (TRANSPOSE) {
+                const int c = c0 + ty + k, r = r0 + tx;
+                if (r < rows && c < cols) dst[(size_t)c * rows + r] = t[tx][ty + k];
+            } else {
+                const int r = r0 + ty + k, c = c0 + tx;
+                if (r < rows && c < cols) dst[(size_t)r * cols + c] = t[ty + k][tx];
+            }
+        }
+    } else {
+        // interleaved [G][rows][cols][4] -> planar [S][rows][cols] (no transpose)
+        const float4* src = reinterpret_cast<const float4*>(in) + (size_t)g * plane;
+#pragma unroll
+        for (int k = 0; k < 32; k += 8) {
+            const int r = r0 + ty + k, c = c0 + tx;
+            if (r < rows && c < cols) {
+                const float4 v = src[(size_t)r * cols + c];
+                const size_t o = (size_t)r * cols + c;
+                const int s = g * 4;
+                if (s + 0 < S) out[(size_t)(s + 0) * plane + o] = v.x;
+                if (s + 1 < S) out[(size_t)(s + 1) * plane + o] = v.y;
+                if (s + 2 < S) out[(size_t)(s + 2) * plane + o] = v.z;
+                if (s + 3 < S) out[(size_t)(s + 3) * plane + o] = v.w;
+            }
+        }
     }
 }
 
@@ -198,37 +299,54 @@ void launch_backproject(const BpArgs& b, int store, hipStream_t st)
 {
     if (b.S <= 0 || b.n <= 0) return;
     BpKArgs k;
-    k.q = b.q; k.out = b.out; k.x = b.x; k.norm = b.norm; k.xt = b.xt;
+    k.q = static_cast<const uint4*>(b.q);
+    k.out = b.out; k.x = b.x; k.norm = b.norm; k.xt = b.xt;
     k.cA = b.cA; k.cB = b.cB; k.xc = b.xc; k.c0 = b.c0;
-    k.n = b.n; k.A = b.A; k.tlo = b.tlo; k.W = b.W;
-    k.tiles_x = (b.n + BP_TILE - 1) / BP_TILE;
-    k.norm_stride = 0;
-    dim3 grid(k.tiles_x * k.tiles_x, b.S);
+    k.n = b.n; k.A = b.A; k.tlo = b.tlo; k.W = b.W; k.S = b.S;
+    k.tiles_x = (b.n + BP_TILE_X - 1) / BP_TILE_X;
+    const int tiles_y = (b.n + BP_TILE_Y - 1) / BP_TILE_Y;
+    const int sb = (store == RAFT_STORE_F16) ? 8 : 4;
+    const int groups = (b.S + sb - 1) / sb;
+    if (groups > 65535) fail(RAFT_EUNSUPPORTED, "too many slice groups for one backprojection launch");
+    dim3 grid(k.tiles_x * tiles_y, groups);
     dim3 block(256);
     if (b.mode == 1) {
-        if (store == RAFT_STORE_F16) hipLaunchKernelGGL((k_backproject<__half, 1>), grid, block, 0, st, k);
-        else hipLaunchKernelGGL((k_backproject<float, 1>), grid, block, 0, st, k);
+        if (store != RAFT_STORE_F32) fail(RAFT_EINVAL, "eEM update runs on FP32 rows");
+        hipLaunchKernelGGL((k_backproject<float, 1>), grid, block, 0, st, k);
+    } else if (store == RAFT_STORE_F16) {
+        hipLaunchKernelGGL((k_backproject<__half, 0>), grid, block, 0, st, k);
     } else {
-        if (store == RAFT_STORE_F16) hipLaunchKernelGGL((k_backproject<__half, 0>), grid, block, 0, st, k);
-        else hipLaunchKernelGGL((k_backproject<float, 0>), grid, block, 0, st, k);
+        hipLaunchKernelGGL((k_backproject<float, 0>), grid, block, 0, st, k);
     }
     RAFT_HIP(hipGetLastError());
 }
 
-void launch_fill_rows(float* q, const float* wa, int S, int A, int W, int tlo, int nbins, hipStream_t st)
+void launch_fill_rows(float* q, const float* wa, int A, int W, int tlo, int nbins, hipStream_t st)
 {
-    const size_t total = (size_t)S * A * W;
+    const size_t total = (size_t)A * W;
     if (!total) return;
     int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_fill_rows, dim3(blocks), dim3(256), 0, st, q, wa, S, A, W, tlo, nbins);
+    hipLaunchKernelGGL(k_fill_rows, dim3(blocks), dim3(256), 0, st, reinterpret_cast<float4*>(q), wa, A, W, tlo,
+                       nbins);
     RAFT_HIP(hipGetLastError());
 }
 
-void launch_transpose(const float* in, float* out, int S, int rows, int cols, hipStream_t st)
+void launch_to_interleaved(const float* in, float* out, int S, int rows, int cols, bool transpose, hipStream_t st)
 {
     if (S <= 0 || rows <= 0 || cols <= 0) return;
-    dim3 grid((cols + 31) / 32, (rows + 31) / 32, S);
-    hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, st, in, out, rows, cols);
+    const int G = (S + 3) / 4;
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32, G);
+    if (transpose) hipLaunchKernelGGL((k_layout<1, 1>), grid, dim3(256), 0, st, in, out, S, rows, cols);
+    else hipLaunchKernelGGL((k_layout<1, 0>), grid, dim3(256), 0, st, in, out, S, rows, cols);
+    RAFT_HIP(hipGetLastError());
+}
+
+void launch_from_interleaved(const float* in, float* out, int S, int rows, int cols, hipStream_t st)
+{
+    if (S <= 0 || rows <= 0 || cols <= 0) return;
+    const int G = (S + 3) / 4;
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32, G);
+    hipLaunchKernelGGL((k_layout<0, 0>), grid, dim3(256), 0, st, in, out, S, rows, cols);
     RAFT_HIP(hipGetLastError());
 }
 

@@ -1,15 +1,17 @@
 // fft_filter.hip -- FBP row filter on gfx950: edge-replicate pad, FFT,
 // multiply by the (Hermitian) filter, inverse FFT, window store.
 //
-// One workgroup filters two sinogram rows of one slice at once: the padded
-// rows u0, u1 (length n = npad) form z = u0 + i u1 in LDS; a mixed-radix
-// Stockham FFT (radices 8, 4, 2, 3, 5, 7, 11, 13; twiddles from an FP64-built
-// table) runs entirely in LDS with the butterfly inputs held in registers
-// across one barrier per pass.  Because the filter H is Hermitian
-// (H_{n-k} = conj H_k, the spectrum of a real kernel), conj(FFT(conj(Z H)))
-// returns q0 + i q1, the two filtered rows.  H carries 1/(npad * pixel)
-// (filters.cu:184-186); the store multiplies by dtheta_a * angular scale and
-// writes only the bin window the backprojector reads, zeros outside [0, n).
+// One workgroup filters two sinogram rows at once -- the same angle of two
+// consecutive slices: the padded rows u0, u1 (length n = npad) form
+// z = u0 + i u1 in LDS; a mixed-radix Stockham FFT (radices 8, 4, 2, 3, 5, 7,
+// 11, 13; twiddles from an FP64-built table) runs entirely in LDS with the
+// butterfly inputs held in registers across one barrier per pass.  Because
+// the filter H is Hermitian (H_{n-k} = conj H_k, the spectrum of a real
+// kernel), conj(FFT(conj(Z H))) returns q0 + i q1, the two filtered rows.  H
+// carries 1/(npad * pixel) (filters.cu:184-186); the store multiplies by
+// dtheta_a * angular scale and writes only the bin window the backprojector
+// reads, zeros outside [0, n), into the slice-interleaved layout
+// [group][angle][bin][SB] (SB = 4 for FP32 rows, 8 for FP16 rows).
 //
 // Padded sizes with a prime factor > 13 or n > 16384 use k_filter_direct:
 // q[m] = sum_j g[(m - j) mod n] u[j] with g = IDFT(H) (built on the host in
@@ -182,9 +184,6 @@ __device__ void fft_run(float2* z, const float2* __restrict__ tw, const FftK& p)
     }
 }
 
-template <typename OT> __device__ __forceinline__ OT from_f32(float v);
-template <> __device__ __forceinline__ float from_f32<float>(float v) { return v; }
-template <> __device__ __forceinline__ __half from_f32<__half>(float v) { return __float2half(v); }
 
 struct FilterK {
     const float* tomo;
@@ -193,26 +192,69 @@ struct FilterK {
     const float2* tw;
     const float* g;
     const float* wa;
-    int N, A, npad, L, tlo, W, none, hp;
+    int N, A, S, npad, L, tlo, W, none;
+    int SB; // slices per interleaved group
 };
 
-template <typename OT>
-__device__ __forceinline__ void store_pair(const FilterK& a, const float2* z, int s, int a0, bool has2, float sign1)
+// block -> (group, angle, pair): rows (s0 = g*SB + 2*pair, a) and (s0 + 1, a)
+struct PairIdx {
+    int g, a, s0;
+};
+__device__ __forceinline__ PairIdx pair_index(const FilterK& k)
 {
-    OT* q0 = static_cast<OT*>(a.q) + ((size_t)s * a.A + a0) * a.W;
-    OT* q1 = q0 + a.W;
-    const float w0 = a.wa[a0];
-    const float w1 = has2 ? a.wa[a0 + 1] : 0.0f;
-    for (int w = threadIdx.x; w < a.W; w += blockDim.x) {
-        const int T = a.tlo + w;
+    const int pairs = k.SB / 2;
+    PairIdx r;
+    const int b = blockIdx.x;
+    const int pr = b % pairs;
+    const int ga = b / pairs;
+    r.a = ga % k.A;
+    r.g = ga / k.A;
+    r.s0 = r.g * k.SB + 2 * pr;
+    return r;
+}
+
+// padded rows of slices s0, s0+1 (missing slices read as zero) into z
+template <int THREADS>
+__device__ __forceinline__ void load_pair(const FilterK& k, const PairIdx& ix, float2* z)
+{
+    const bool v0 = ix.s0 < k.S, v1 = ix.s0 + 1 < k.S;
+    const float* r0 = k.tomo + ((size_t)ix.s0 * k.A + ix.a) * k.N;
+    const float* r1 = r0 + (size_t)k.A * k.N;
+    for (int m = threadIdx.x; m < k.npad; m += THREADS) {
+        int ii = m - k.L;
+        ii = ii < 0 ? 0 : (ii > k.N - 1 ? k.N - 1 : ii);
+        z[m] = make_float2(v0 ? r0[ii] : 0.0f, v1 ? r1[ii] : 0.0f);
+    }
+}
+
+template <typename OT> struct Pair2;
+template <> struct Pair2<float> {
+    using T = float2;
+    __device__ static __forceinline__ T make(float a, float b) { return make_float2(a, b); }
+};
+template <> struct Pair2<__half> {
+    using T = __half2;
+    __device__ static __forceinline__ T make(float a, float b) { return __floats2half2_rn(a, b); }
+};
+
+// window store: bins [tlo, tlo + W) of both rows, times the angle weight
+template <typename OT, int THREADS>
+__device__ __forceinline__ void store_pair(const FilterK& k, const PairIdx& ix, const float2* z, float sign1)
+{
+    using P2 = typename Pair2<OT>::T;
+    P2* dst = reinterpret_cast<P2*>(static_cast<OT*>(k.q) + (((size_t)ix.g * k.A + ix.a) * k.W) * k.SB +
+                                    (ix.s0 - ix.g * k.SB));
+    const int stride = k.SB / 2; // in pairs
+    const float wv = k.wa[ix.a];
+    for (int w = threadIdx.x; w < k.W; w += THREADS) {
+        const int T = k.tlo + w;
         float v0 = 0.0f, v1 = 0.0f;
-        if (T >= 0 && T < a.npad) {
+        if (T >= 0 && T < k.npad) {
             const float2 zv = z[T];
-            v0 = zv.x * w0;
-            v1 = sign1 * zv.y * w1;
+            v0 = zv.x * wv;
+            v1 = sign1 * zv.y * wv;
         }
-        q0[w] = from_f32<OT>(v0);
-        if (has2) q1[w] = from_f32<OT>(v1);
+        dst[(size_t)w * stride] = Pair2<OT>::make(v0, v1);
     }
 }
 
@@ -220,19 +262,11 @@ template <typename OT, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
 {
     extern __shared__ float2 z[];
-    const int s = blockIdx.x / a.hp;
-    const int a0 = 2 * (blockIdx.x % a.hp);
-    const bool has2 = a0 + 1 < a.A;
-    const float* r0 = a.tomo + ((size_t)s * a.A + a0) * a.N;
-    const float* r1 = r0 + a.N;
-    for (int m = threadIdx.x; m < a.npad; m += THREADS) {
-        int ii = m - a.L;
-        ii = ii < 0 ? 0 : (ii > a.N - 1 ? a.N - 1 : ii);
-        z[m] = make_float2(r0[ii], has2 ? r1[ii] : 0.0f);
-    }
+    const PairIdx ix = pair_index(a);
+    load_pair<THREADS>(a, ix, z);
     __syncthreads();
     if (a.none) {
-        store_pair<OT>(a, z, s, a0, has2, 1.0f);
+        store_pair<OT, THREADS>(a, ix, z, 1.0f);
         return;
     }
     fft_run<THREADS>(z, a.tw, p);
@@ -242,51 +276,40 @@ __global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
     }
     __syncthreads();
     fft_run<THREADS>(z, a.tw, p);
-    store_pair<OT>(a, z, s, a0, has2, -1.0f);
+    store_pair<OT, THREADS>(a, ix, z, -1.0f);
 }
 
-// direct circular convolution for sizes the LDS FFT does not cover
+// direct circular convolution for sizes the LDS FFT does not cover:
+// q[m] = sum_j g[(m - j) mod n] u[j]
 template <typename OT>
 __global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
 {
-    extern __shared__ float2 z[]; // u0 + i u1, length npad
-    const int s = blockIdx.x / a.hp;
-    const int a0 = 2 * (blockIdx.x % a.hp);
-    const bool has2 = a0 + 1 < a.A;
-    const float* r0 = a.tomo + ((size_t)s * a.A + a0) * a.N;
-    const float* r1 = r0 + a.N;
-    for (int m = threadIdx.x; m < a.npad; m += blockDim.x) {
-        int ii = m - a.L;
-        ii = ii < 0 ? 0 : (ii > a.N - 1 ? a.N - 1 : ii);
-        z[m] = make_float2(r0[ii], has2 ? r1[ii] : 0.0f);
-    }
+    extern __shared__ float2 z[]; // u0 + i u1; the filtered values go straight to the staged rows
+    const PairIdx ix = pair_index(a);
+    load_pair<256>(a, ix, z);
     __syncthreads();
-    OT* q0 = static_cast<OT*>(a.q) + ((size_t)s * a.A + a0) * a.W;
-    OT* q1 = q0 + a.W;
-    const float w0 = a.wa[a0];
-    const float w1 = has2 ? a.wa[a0 + 1] : 0.0f;
-    for (int w = threadIdx.x; w < a.W; w += blockDim.x) {
+    if (a.none) {
+        store_pair<OT, 256>(a, ix, z, 1.0f);
+        return;
+    }
+    using P2 = typename Pair2<OT>::T;
+    P2* dst = reinterpret_cast<P2*>(static_cast<OT*>(a.q) + (((size_t)ix.g * a.A + ix.a) * a.W) * a.SB +
+                                    (ix.s0 - ix.g * a.SB));
+    const int stride = a.SB / 2;
+    const float wv = a.wa[ix.a];
+    for (int w = threadIdx.x; w < a.W; w += 256) {
         const int T = a.tlo + w;
-        float v0 = 0.0f, v1 = 0.0f;
+        float s0 = 0.0f, s1 = 0.0f;
         if (T >= 0 && T < a.npad) {
-            if (a.none) {
-                v0 = z[T].x;
-                v1 = z[T].y;
-            } else {
-                float s0 = 0.0f, s1 = 0.0f;
-                int d = T; // (T - j) mod n, j = 0..n-1
-                for (int j = 0; j < a.npad; ++j) {
-                    const float gv = a.g[d];
-                    s0 = fmaf(gv, z[j].x, s0);
-                    s1 = fmaf(gv, z[j].y, s1);
-                    d = (d == 0) ? a.npad - 1 : d - 1;
-                }
-                v0 = s0;
-                v1 = s1;
+            int d = T; // (T - j) mod n, j = 0..n-1
+            for (int j = 0; j < a.npad; ++j) {
+                const float gv = a.g[d];
+                s0 = fmaf(gv, z[j].x, s0);
+                s1 = fmaf(gv, z[j].y, s1);
+                d = (d == 0) ? a.npad - 1 : d - 1;
             }
         }
-        q0[w] = from_f32<OT>(v0 * w0);
-        if (has2) q1[w] = from_f32<OT>(v1 * w1);
+        dst[(size_t)w * stride] = Pair2<OT>::make(s0 * wv, s1 * wv);
     }
 }
 
@@ -339,9 +362,10 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
     if (a.S <= 0 || a.A <= 0) return;
     FilterK k;
     k.tomo = a.tomo; k.q = a.q; k.H = a.H; k.tw = a.tw; k.g = a.g; k.wa = a.wa;
-    k.N = a.N; k.A = a.A; k.npad = a.npad; k.L = a.L; k.tlo = a.tlo; k.W = a.W; k.none = a.none;
-    k.hp = (a.A + 1) / 2;
-    const long long blocks = (long long)a.S * k.hp;
+    k.N = a.N; k.A = a.A; k.S = a.S; k.npad = a.npad; k.L = a.L; k.tlo = a.tlo; k.W = a.W; k.none = a.none;
+    k.SB = (store == RAFT_STORE_F16) ? 8 : 4;
+    const long long groups = (a.S + k.SB - 1) / k.SB;
+    const long long blocks = groups * a.A * (k.SB / 2);
     if (blocks > 0x7fffffffLL) fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one filter launch");
     const size_t lds = sizeof(float2) * (size_t)a.npad;
     if (plan.ok || a.none) {

@@ -129,14 +129,15 @@ static double fbp_angle_scale(const float* ang, int A)
 
 void bp_window(BpGeom& g)
 {
-    const int tiles = (g.n + BP_TILE - 1) / BP_TILE;
-    const double lo = -g.xc, hi = (double)(tiles * BP_TILE - 1) - g.xc;
+    const int tx = (g.n + BP_TILE_X - 1) / BP_TILE_X, ty = (g.n + BP_TILE_Y - 1) / BP_TILE_Y;
+    const double xlo = -g.xc, xhi = (double)(tx * BP_TILE_X - 1) - g.xc;
+    const double ylo = -g.xc, yhi = (double)(ty * BP_TILE_Y - 1) - g.xc;
     long long tmin = 0x7fffffffLL, tmax = -0x7fffffffLL;
     for (int a = 0; a < g.nangles; ++a) {
-        const double u[4] = {std::fma(lo, g.cA[a], std::fma(lo, g.cB[a], g.c0)),
-                             std::fma(hi, g.cA[a], std::fma(lo, g.cB[a], g.c0)),
-                             std::fma(lo, g.cA[a], std::fma(hi, g.cB[a], g.c0)),
-                             std::fma(hi, g.cA[a], std::fma(hi, g.cB[a], g.c0))};
+        const double u[4] = {std::fma(xlo, g.cA[a], std::fma(ylo, g.cB[a], g.c0)),
+                             std::fma(xhi, g.cA[a], std::fma(ylo, g.cB[a], g.c0)),
+                             std::fma(xlo, g.cA[a], std::fma(yhi, g.cB[a], g.c0)),
+                             std::fma(xhi, g.cA[a], std::fma(yhi, g.cB[a], g.c0))};
         const double umin = std::min(std::min(u[0], u[1]), std::min(u[2], u[3]));
         const double umax = std::max(std::max(u[0], u[1]), std::max(u[2], u[3]));
         tmin = std::min(tmin, (long long)std::floor(umin) - 1);
@@ -403,12 +404,14 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
     upload(dcB, g.cB, st);
     upload(dwa, wa, st);
 
-    const size_t esz = (store == RAFT_STORE_F16) ? 2 : 4;
-    const size_t per_slice = (size_t)A * g.width * esz;
+    const int SB = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
+    const size_t per_group = (size_t)A * g.width * 16; // SB values per bin, 16 bytes
+    // blocks are whole slice groups (except the last)
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
+    B = std::min(c.S, ((B + SB - 1) / SB) * SB);
     const size_t budget = block_budget();
-    while (B > 1 && (size_t)B * per_slice > budget) B = (B + 1) / 2;
-    void* q = D.workspace(0, (size_t)B * per_slice);
+    while (B > SB && (size_t)((B + SB - 1) / SB) * per_group > budget) B = (((B + 1) / 2 + SB - 1) / SB) * SB;
+    void* q = D.workspace(0, (size_t)((B + SB - 1) / SB) * per_group);
 
     for (int s0 = 0; s0 < c.S; s0 += B) {
         const int nb = std::min(B, c.S - s0);
@@ -468,16 +471,18 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
     upload(ds, sv, st);
     RayGeom g = ray_geom_radon(Nx, Ny, nrays, nangles, (double)ax, (double)ay);
     const size_t img = (size_t)Nx * Ny;
-    int B = nslices;
+    int B = ((nslices + 3) / 4) * 4;
     const size_t budget = block_budget();
-    while (B > 1 && (size_t)B * img * 4 > budget) B = (B + 1) / 2;
-    float* t = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
+    while (B > 4 && (size_t)B * img * 8 > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
+    float* il = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
+    float* ilT = static_cast<float*>(D.workspace(2, (size_t)B * img * 4));
     for (int s0 = 0; s0 < nslices; s0 += B) {
         const int nb = std::min(B, nslices - s0);
         const float* src = d_obj + (size_t)s0 * img;
-        launch_transpose(src, t, nb, Ny, Nx, st);
+        launch_to_interleaved(src, il, nb, Ny, Nx, false, st);
+        launch_to_interleaved(src, ilT, nb, Ny, Nx, true, st);
         ProjArgs p;
-        p.img = src; p.imgT = t; p.out = d_proj + (size_t)s0 * nangles * nrays;
+        p.img = il; p.imgT = ilT; p.out = d_proj + (size_t)s0 * nangles * nrays;
         p.y = nullptr; p.wa = nullptr; p.kl_part = nullptr; p.ex_part = nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
         p.g = g; p.S = nb; p.tlo = 0; p.W = nrays;
@@ -502,7 +507,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     if (S < 0 || K < 0) fail(RAFT_EINVAL, "nslices and iterations must be >= 0");
     if (!angles) fail(RAFT_EINVAL, "null angles");
     const double rr = (double)(R - 1) / (double)(n - 1);
-    if (rr * 63.0 * std::sqrt(2.0) + 5.0 > (double)BP_SLOT)
+    if (rr * std::hypot((double)(BP_TILE_X - 1), (double)(BP_TILE_Y - 1)) + 5.0 > (double)BP_SLOT)
         fail(RAFT_EUNSUPPORTED, "nrays larger than objsize beyond the backprojection tile window");
     if (S == 0) return;
     if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
@@ -543,47 +548,51 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     RayGeom rg = ray_geom_em(R, A, n);
 
     const size_t img = (size_t)n * n;
-    const int nbl = proj_blocks_per_slice(rg);
-    const size_t per_slice = (size_t)A * g.width * 4 + img * 4 + (size_t)nbl * 16;
+    // per slice: interleaved image + transpose, staged ratio rows, KL partials
+    const size_t per_slice = img * 8 + (size_t)A * g.width * 4 + (size_t)A * 16;
     int B = blocksize > 0 ? std::min(blocksize, S) : S;
+    B = ((B + 3) / 4) * 4;
     const size_t budget = block_budget();
-    while (B > 1 && (size_t)B * per_slice > budget) B = (B + 1) / 2;
+    while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
+    const int Gb = B / 4;
 
-    float* rows = static_cast<float*>(D.workspace(0, (size_t)B * A * g.width * 4));
-    float* xT = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
-    float* ones = static_cast<float*>(D.workspace(2, img * 4 + (size_t)A * g.width * 4));
+    float* rows = static_cast<float*>(D.workspace(0, (size_t)Gb * A * g.width * 16));
+    float* xil = static_cast<float*>(D.workspace(1, (size_t)Gb * img * 16));
+    float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
+    float* ones = static_cast<float*>(D.workspace(3, img * 4 + (size_t)A * g.width * 16));
     float* onerows = ones + img;
-    double* part = static_cast<double*>(D.workspace(3, (size_t)B * nbl * 16));
-    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * nbl);
+    double* part = static_cast<double*>(D.workspace(4, (size_t)B * A * 16));
+    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * A);
     const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
     double* dkl = nullptr;
     unsigned long long* dex = nullptr;
     if (want_kl && K > 0) {
-        double* buf = static_cast<double*>(D.workspace(4, (size_t)S * K * 16));
+        double* buf = static_cast<double*>(D.workspace(5, (size_t)S * K * 16));
         dkl = buf;
         dex = reinterpret_cast<unsigned long long*>(buf + (size_t)S * K);
     }
 
     // backones = B(1): the same for every slice
-    launch_fill_rows(onerows, dwa.as<float>(), 1, A, g.width, g.tlo, R, st);
+    launch_fill_rows(onerows, dwa.as<float>(), A, g.width, g.tlo, R, st);
     {
         BpArgs b1;
         b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
         b1.cA = dcA.as<double>(); b1.cB = dcB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
-        b1.n = n; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 2;
+        b1.n = n; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 0;
         launch_backproject(b1, RAFT_STORE_F32, st);
     }
     // ratio rows: margins stay zero, the projector writes [0, R)
-    RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)B * A * g.width * 4, st));
+    RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)Gb * A * g.width * 16, st));
 
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
         float* x = d_obj + (size_t)s0 * img;
         const float* y = d_tomo + (size_t)s0 * A * R;
-        launch_transpose(x, xT, nb, n, n, st);
+        launch_to_interleaved(x, xil, nb, n, n, false, st);
+        launch_to_interleaved(x, xT, nb, n, n, true, st);
         for (int it = 0; it < K; ++it) {
             ProjArgs p;
-            p.img = x; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
+            p.img = xil; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
             p.kl_part = want_kl ? part : nullptr;
             p.ex_part = want_kl ? expart : nullptr;
             p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
@@ -592,9 +601,9 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
                 ProfScope ps(KF_PROJ, st);
                 launch_project(p, 1, st);
             }
-            if (want_kl) launch_kl_reduce(part, expart, nbl, nb, dkl + (size_t)s0 * K, dex + (size_t)s0 * K, K, it, st);
+            if (want_kl) launch_kl_reduce(part, expart, A, nb, dkl + (size_t)s0 * K, dex + (size_t)s0 * K, K, it, st);
             BpArgs b;
-            b.q = rows; b.out = x; b.x = x; b.norm = ones; b.xt = xT;
+            b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
             b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
             b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
             {
@@ -602,6 +611,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
                 launch_backproject(b, RAFT_STORE_F32, st);
             }
         }
+        launch_from_interleaved(xil, x, nb, n, n, st);
     }
     if (want_kl && K > 0) {
         std::vector<double> hk((size_t)S * K);

@@ -64,12 +64,17 @@ struct BpGeom {
     std::vector<double> cA, cB; // per angle
 };
 
-// Tile of the backprojection kernel: 64 x 64 pixels, 256 threads, 4 x 4 pixels each.
-constexpr int BP_TILE = 64;
+// Tile of the backprojection kernel: 64 x 32 pixels, 256 threads, 4 x 2 pixels each.
+constexpr int BP_TILE_X = 64;
+constexpr int BP_TILE_Y = 32;
 constexpr int BP_CHUNK = 32;   // angles staged per LDS round
-constexpr int BP_SLOT = 96;    // LDS floats per staged angle window (>= 63*sqrt(2) + 4)
+constexpr int BP_SLOT = 80;    // LDS entries per staged angle window (>= |(63, 31)| + 5)
+// staged rows hold the bins of BP_SLICES_F32 (FP32) or BP_SLICES_F16 (FP16)
+// slices side by side: [group][angle][bin][slice]
+constexpr int BP_SLICES_F32 = 4;
+constexpr int BP_SLICES_F16 = 8;
 
-// global bin window covering every pixel of the tile grid (ceil(n/64)*64)^2
+// global bin window covering every pixel of the tile grid
 void bp_window(BpGeom& g);
 
 // ---------------------------------------------------------------------------
@@ -107,7 +112,7 @@ FftPlan fft_plan(int n);
 
 struct FilterArgs {
     const float* tomo;  // [S][A][N]
-    void* q;            // [S][A][W] float or __half
+    void* q;            // [G][A][W][SB] float (SB = 4) or __half (SB = 8)
     const float2* H;    // [npad] Hermitian filter incl. 1/(npad*pixel)
     const float2* tw;   // [npad] exp(-2 pi i t/npad)
     const float* g;     // direct-convolution kernel (real, [npad]) or null
@@ -118,27 +123,27 @@ struct FilterArgs {
 void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
 
 struct BpArgs {
-    const void* q;      // [S][A][W] staged rows (float or half)
-    float* out;         // [S][n][n]
-    const float* x;     // eEM: current image (may alias out)
-    const float* norm;  // eEM: backones [S][n][n] (null for FBP / backones pass)
-    float* xt;          // eEM: optional transposed copy of the updated image (or null)
+    const void* q;      // [G][A][W][SB] staged rows (float SB = 4, or half SB = 8)
+    float* out;         // mode 0: planar [S][n][n]; mode 1: interleaved [G][n][n][4]
+    const float* x;     // mode 1: interleaved current image (may alias out)
+    const float* norm;  // mode 1: backones [n][n], shared by all slices
+    float* xt;          // mode 1: interleaved transposed copy of the updated image (or null)
     const double* cA;
     const double* cB;
     double xc, c0;
     int n, A, S, tlo, W;
-    int mode;           // 0 = FBP store, 1 = eEM update x*bp/norm, 2 = plain store (backones)
+    int mode;           // 0 = store the sums (FBP, backones), 1 = eEM update x * bp / norm
 };
 void launch_backproject(const BpArgs& a, int store, hipStream_t st);
 
 struct ProjArgs {
-    const float* img;   // [S][Ny][Nx]
-    const float* imgT;  // [S][Nx][Ny] transposed copy (may be null: then no transposed path)
-    float* out;         // Radon: [S][A][nrays]; eEM: ratio rows [S][A][W] at offset -tlo
-    const float* y;     // eEM: measured sinogram [S][A][nrays] (null for Radon)
+    const float* img;   // interleaved [G][Ny][Nx][4]
+    const float* imgT;  // interleaved transpose [G][Nx][Ny][4] (null: no transposed path)
+    float* out;         // Radon: planar [S][A][nrays]; eEM: interleaved ratio rows [G][A][W][4] at (i - tlo)
+    const float* y;     // eEM: measured sinogram, planar [S][A][nrays] (null for Radon)
     const float* wa;    // eEM: per-angle dtheta
-    double* kl_part;    // eEM: per-block KL partial sums (null = skip)
-    unsigned long long* ex_part; // eEM: per-block excluded counts
+    double* kl_part;    // eEM: per-(slice, angle) KL partial sums [S][A] (null = skip)
+    unsigned long long* ex_part; // eEM: per-(slice, angle) excluded counts
     const double* cosv;
     const double* sinv;
     RayGeom g;
@@ -146,10 +151,13 @@ struct ProjArgs {
     int tlo, W;         // eEM ratio-row layout
 };
 void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
-int proj_blocks_per_slice(const RayGeom& g);
 
-void launch_transpose(const float* in, float* out, int S, int rows, int cols, hipStream_t st);
-void launch_fill_rows(float* q, const float* wa, int S, int A, int W, int tlo, int nbins, hipStream_t st);
+// planar [S][rows][cols] -> interleaved [G][rows][cols][4] (or its transpose [G][cols][rows][4])
+void launch_to_interleaved(const float* in, float* out, int S, int rows, int cols, bool transpose, hipStream_t st);
+// interleaved [G][rows][cols][4] -> planar [S][rows][cols]
+void launch_from_interleaved(const float* in, float* out, int S, int rows, int cols, hipStream_t st);
+// rows for B(1): interleaved [A][W][4], value wa[a] on valid bins, 0 elsewhere
+void launch_fill_rows(float* q, const float* wa, int A, int W, int tlo, int nbins, hipStream_t st);
 void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,
                       unsigned long long* ex, int K, int iter, hipStream_t st);
 

@@ -95,18 +95,20 @@ def test_plan_fft_path_selection():
 @given(n=st.integers(2, 300), a=st.integers(2, 64), pad=st.integers(0, 3), seed=st.integers(0, 10 ** 6),
        spread=st.sampled_from([np.pi, 2 * np.pi, 7.0]))
 def test_plan_window_covers_every_bin(n, a, pad, seed, spread):
-    """Every bin T = (int)u of every pixel of the 64x64 tile grid lies in the staged window."""
+    """Every bin T = (int)u of every pixel of the backprojector's tile grid (64 x 32
+    pixel tiles, so ceil(n/64)*64 columns by ceil(n/32)*32 rows) lies in the
+    staged window."""
     rng = np.random.default_rng(seed)
     ang = np.sort(rng.uniform(-1.0, spread, a)).astype(np.float32)
     paramf, parami = oracle.fbp_params(n, a, 1, pad=pad)
     tlo, width, _ = plan(ang, paramf, parami)
     npad, L = n * (1 + pad), (pad * n) // 2
     half = npad / 2.0
-    g = -(-n // 64) * 64
-    I = np.arange(g) + L - half
+    I = np.arange(-(-n // 64) * 64) + L - half
+    J = np.arange(-(-n // 32) * 32) + L - half
     th = ang.astype(np.float64)
     for c, s in zip(np.cos(th), np.sin(th)):
-        u = I[None, :] * c - I[:, None] * s + half
+        u = I[None, :] * c - J[:, None] * s + half
         T = np.trunc(u)
         assert T.min() >= tlo and T.max() < tlo + width
 
