@@ -70,6 +70,13 @@ def fbp_bench(args, torch, dv, ph, oracle):
     del img, p
     torch.cuda.synchronize()
     log(f"inputs: {S} x {A} x {N} sinogram generated on the GPU in {time.perf_counter() - t0:.1f} s")
+    if args.copy_calibration:
+        # elementwise copy kernel: reads and writes exactly 4 * S * A * N bytes
+        for _ in range(3):
+            y2 = y.clone()
+        torch.cuda.synchronize()
+        log(f"copy calibration: {3} copies of {4 * y.numel() / 2**20:.1f} MiB")
+        del y2
     paramf, parami = oracle.fbp_params(N, A, S, pad=2, filter_type=7, pixel=2.0 / N)
     tlo, width, fft_path = dv.fbp_plan_info(ang, paramf, parami)
     out = torch.empty((S, N, N), dtype=torch.float32, device=dev)
@@ -178,7 +185,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-em", action="store_true")
     ap.add_argument("--em-iters", type=int, default=200)
+    ap.add_argument("--only", choices=["all", "fbp", "em"], default="all",
+                    help="profiling runs: time only one workload (the JSON line keeps the same fields)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of the 64 x 2048 x 2048 sinogram (known byte count) so "
+                         "that rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled")
     args = ap.parse_args()
+    if args.only == "em":
+        args.no_cpu, args.check_slices = True, 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the slab sharding over several GPUs is "
               f"not exercised by this benchmark.", file=sys.stderr)
@@ -193,8 +207,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.only == "em":
+        em = em_bench(args, torch, dv, ph)
+        print(json.dumps({"metric": METRIC, "em": em}), flush=True)
+        return
     res, roof, cpu, check = fbp_bench(args, torch, dv, ph, oracle)
-    em = None if args.no_em else em_bench(args, torch, dv, ph)
+    em = None if (args.no_em or args.only == "fbp") else em_bench(args, torch, dv, ph)
     r32 = res["f32"]
     line = {
         "metric": METRIC,

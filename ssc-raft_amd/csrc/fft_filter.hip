@@ -13,7 +13,7 @@
 // reads, zeros outside [0, n), into the slice-interleaved layout
 // [group][angle][bin][SB] (SB = 4 for FP32 rows, 8 for FP16 rows).
 //
-// Padded sizes with a prime factor > 13 or n > 16384 use k_filter_direct:
+// Padded sizes with a prime factor > 5 or n > 16384 use k_filter_direct:
 // q[m] = sum_j g[(m - j) mod n] u[j] with g = IDFT(H) (built on the host in
 // FP64), the same circular convolution.
 //
@@ -34,11 +34,25 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); } // * (-i)
 
+// Twiddles W_n^m = exp(-2 pi i m / n), m < n <= 16384, from two small tables
+// kept in LDS: W^m = hi[m >> 7] * lo[m & 127] (one FP32 product of two
+// FP64-built values; error ~2 ulp).
+struct Tw {
+    const float2* lo; // [128]
+    const float2* hi; // [n / 128 + 1]
+    __device__ __forceinline__ float2 operator()(int m) const { return cmul(hi[m >> 7], lo[m & 127]); }
+};
+
+// LDS index with one complex of padding per 16: Stockham passes with small
+// strides (Ns = 1, 8) would otherwise put consecutive lanes on the same banks.
+__device__ __forceinline__ int pix(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr size_t lds_elems(int n) { return (size_t)n + (size_t)(n >> 4) + 1; }
+
 // in-register forward DFTs of size R (W_R = exp(-2 pi i / R))
 template <int R> struct Dft;
 
 template <> struct Dft<2> {
-    __device__ static void run(float2* v, const float2*, int)
+    __device__ static void run(float2* v, const Tw&, int)
     {
         float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -47,7 +61,7 @@ template <> struct Dft<2> {
 };
 
 template <> struct Dft<3> {
-    __device__ static void run(float2* v, const float2*, int)
+    __device__ static void run(float2* v, const Tw&, int)
     {
         const float c1 = -0.5f, s1 = 0.86602540378443864676f; // sin(2 pi/3)
         float2 a = v[0], b = v[1], c = v[2];
@@ -62,7 +76,7 @@ template <> struct Dft<3> {
 };
 
 template <> struct Dft<4> {
-    __device__ static void run(float2* v, const float2*, int)
+    __device__ static void run(float2* v, const Tw&, int)
     {
         float2 a = v[0], b = v[1], c = v[2], d = v[3];
         float2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_mi(csub(b, d));
@@ -74,13 +88,13 @@ template <> struct Dft<4> {
 };
 
 template <> struct Dft<8> {
-    __device__ static void run(float2* v, const float2*, int)
+    __device__ static void run(float2* v, const Tw&, int)
     {
         const float h = 0.70710678118654752440f;
         float2 e[4] = {v[0], v[2], v[4], v[6]};
         float2 o[4] = {v[1], v[3], v[5], v[7]};
-        Dft<4>::run(e, nullptr, 0);
-        Dft<4>::run(o, nullptr, 0);
+        Dft<4>::run(e, Tw{}, 0);
+        Dft<4>::run(o, Tw{}, 0);
         // o[k] *= W8^k
         float2 o1 = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)); // * h(1 - i)
         float2 o2 = mul_mi(o[2]);
@@ -96,9 +110,9 @@ template <> struct Dft<8> {
     }
 };
 
-// odd prime radices: direct sum with W_R^m = tw[m * (n / R)]
+// odd prime radices: direct sum with W_R^m = W_n^(m n / R)
 template <int R> struct DftGeneric {
-    __device__ static void run(float2* v, const float2* tw, int n)
+    __device__ static void run(float2* v, const Tw& tw, int n)
     {
         float2 in[R];
 #pragma unroll
@@ -108,38 +122,38 @@ template <int R> struct DftGeneric {
         for (int r = 0; r < R; ++r) {
             float2 acc = in[0];
 #pragma unroll
-            for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(in[q], tw[((q * r) % R) * st]));
+            for (int q = 1; q < R; ++q) acc = cadd(acc, cmul(in[q], tw(((q * r) % R) * st)));
             v[r] = acc;
         }
     }
 };
 template <> struct Dft<5> : DftGeneric<5> {};
-template <> struct Dft<7> : DftGeneric<7> {};
-template <> struct Dft<11> : DftGeneric<11> {};
-template <> struct Dft<13> : DftGeneric<13> {};
 
 // One Stockham pass (decimation in time, autosorting):
 //   butterfly j (0 <= j < n/R), jm = j mod Ns:
 //   in  x_r = z[j + r n/R] * W_{Ns R}^{jm r}
 //   out z[(j - jm) R + jm + r Ns] = DFT_R(x)_r
-// n / THREADS <= 16, so a thread holds at most ceil(16/R) butterflies.
-template <int R, int THREADS>
-__device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ tw, int n, int Ns)
+// n / THREADS <= E, so a thread holds at most ceil(E/R) butterflies.
+// Ns is a power of two for every pass before the first odd radix (radices
+// are ordered 8, 4, 2 first), so jm is a mask there.
+template <int R, int THREADS, int E>
+__device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns)
 {
-    constexpr int MAXB = (16 + R - 1) / R;
+    constexpr int MAXB = (E + R - 1) / R;
     const int nb = n / R;
     const int twstep = n / (Ns * R);
+    const bool pow2 = (Ns & (Ns - 1)) == 0;
     float2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = threadIdx.x + b * THREADS;
         if (j < nb) {
-            const int jm = j % Ns;
+            const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
             const int ts = jm * twstep;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float2 x = z[j + r * nb];
-                if (r) x = cmul(x, tw[r * ts]);
+                float2 x = z[pix(j + r * nb)];
+                if (r) x = cmul(x, tw(r * ts));
                 v[b][r] = x;
             }
         }
@@ -150,10 +164,10 @@ __device__ __forceinline__ void fft_pass(float2* z, const float2* __restrict__ t
         const int j = threadIdx.x + b * THREADS;
         if (j < nb) {
             Dft<R>::run(v[b], tw, n);
-            const int jm = j % Ns;
+            const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
             const int d = (j - jm) * R + jm;
 #pragma unroll
-            for (int r = 0; r < R; ++r) z[d + r * Ns] = v[b][r];
+            for (int r = 0; r < R; ++r) z[pix(d + r * Ns)] = v[b][r];
         }
     }
     __syncthreads();
@@ -164,32 +178,28 @@ struct FftK {
     int radix[24];
 };
 
-template <int THREADS>
-__device__ void fft_run(float2* z, const float2* __restrict__ tw, const FftK& p)
+template <int THREADS, int E>
+__device__ void fft_run(float2* z, const Tw& tw, const FftK& p)
 {
     int Ns = 1;
     for (int st = 0; st < p.nstages; ++st) {
         const int R = p.radix[st];
         switch (R) {
-        case 8: fft_pass<8, THREADS>(z, tw, p.n, Ns); break;
-        case 4: fft_pass<4, THREADS>(z, tw, p.n, Ns); break;
-        case 2: fft_pass<2, THREADS>(z, tw, p.n, Ns); break;
-        case 3: fft_pass<3, THREADS>(z, tw, p.n, Ns); break;
-        case 5: fft_pass<5, THREADS>(z, tw, p.n, Ns); break;
-        case 7: fft_pass<7, THREADS>(z, tw, p.n, Ns); break;
-        case 11: fft_pass<11, THREADS>(z, tw, p.n, Ns); break;
-        default: fft_pass<13, THREADS>(z, tw, p.n, Ns); break;
+        case 8: fft_pass<8, THREADS, E>(z, tw, p.n, Ns); break;
+        case 4: fft_pass<4, THREADS, E>(z, tw, p.n, Ns); break;
+        case 2: fft_pass<2, THREADS, E>(z, tw, p.n, Ns); break;
+        case 3: fft_pass<3, THREADS, E>(z, tw, p.n, Ns); break;
+        default: fft_pass<5, THREADS, E>(z, tw, p.n, Ns); break;
         }
         Ns *= R;
     }
 }
 
-
 struct FilterK {
     const float* tomo;
     void* q;
     const float2* H;
-    const float2* tw;
+    const float2* tw; // [128 + n/128 + 1]: lo table then hi table
     const float* g;
     const float* wa;
     int N, A, S, npad, L, tlo, W, none;
@@ -223,7 +233,7 @@ __device__ __forceinline__ void load_pair(const FilterK& k, const PairIdx& ix, f
     for (int m = threadIdx.x; m < k.npad; m += THREADS) {
         int ii = m - k.L;
         ii = ii < 0 ? 0 : (ii > k.N - 1 ? k.N - 1 : ii);
-        z[m] = make_float2(v0 ? r0[ii] : 0.0f, v1 ? r1[ii] : 0.0f);
+        z[pix(m)] = make_float2(v0 ? r0[ii] : 0.0f, v1 ? r1[ii] : 0.0f);
     }
 }
 
@@ -250,7 +260,7 @@ __device__ __forceinline__ void store_pair(const FilterK& k, const PairIdx& ix, 
         const int T = k.tlo + w;
         float v0 = 0.0f, v1 = 0.0f;
         if (T >= 0 && T < k.npad) {
-            const float2 zv = z[T];
+            const float2 zv = z[pix(T)];
             v0 = zv.x * wv;
             v1 = sign1 * zv.y * wv;
         }
@@ -258,10 +268,14 @@ __device__ __forceinline__ void store_pair(const FilterK& k, const PairIdx& ix, 
     }
 }
 
-template <typename OT, int THREADS>
+template <typename OT, int THREADS, int E>
 __global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
 {
-    extern __shared__ float2 z[];
+    extern __shared__ float2 z[]; // [lds_elems(npad)] padded data, then the twiddle tables
+    float2* tws = z + lds_elems(a.npad);
+    const int ntw = 128 + a.npad / 128 + 1;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
+    const Tw tw{tws, tws + 128};
     const PairIdx ix = pair_index(a);
     load_pair<THREADS>(a, ix, z);
     __syncthreads();
@@ -269,13 +283,13 @@ __global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
         store_pair<OT, THREADS>(a, ix, z, 1.0f);
         return;
     }
-    fft_run<THREADS>(z, a.tw, p);
+    fft_run<THREADS, E>(z, tw, p);
     for (int k = threadIdx.x; k < a.npad; k += THREADS) {
-        const float2 v = cmul(z[k], a.H[k]);
-        z[k] = make_float2(v.x, -v.y);
+        const float2 v = cmul(z[pix(k)], a.H[k]);
+        z[pix(k)] = make_float2(v.x, -v.y);
     }
     __syncthreads();
-    fft_run<THREADS>(z, a.tw, p);
+    fft_run<THREADS, E>(z, tw, p);
     store_pair<OT, THREADS>(a, ix, z, -1.0f);
 }
 
@@ -304,8 +318,9 @@ __global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
             int d = T; // (T - j) mod n, j = 0..n-1
             for (int j = 0; j < a.npad; ++j) {
                 const float gv = a.g[d];
-                s0 = fmaf(gv, z[j].x, s0);
-                s1 = fmaf(gv, z[j].y, s1);
+                const float2 zj = z[pix(j)];
+                s0 = fmaf(gv, zj.x, s0);
+                s1 = fmaf(gv, zj.y, s1);
                 d = (d == 0) ? a.npad - 1 : d - 1;
             }
         }
@@ -313,23 +328,29 @@ __global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
     }
 }
 
-template <typename OT, int THREADS>
+template <typename OT, int THREADS, int E>
 void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
 {
-    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_fft<OT, THREADS>),
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_fft<OT, THREADS, E>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_filter_fft<OT, THREADS>), dim3(blocks), dim3(THREADS), lds, st, k, p);
+    hipLaunchKernelGGL((k_filter_fft<OT, THREADS, E>), dim3(blocks), dim3(THREADS), lds, st, k, p);
 }
 
+// THREADS = max(64, pow2 >= n/8) up to 1024 with E = 8 elements per thread;
+// n in (8192, 16384] runs 1024 threads with E = 16
 template <typename OT>
 void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
 {
+    if (p.n > 8192) {
+        launch_fft_t<OT, 1024, 16>(k, p, blocks, lds, st);
+        return;
+    }
     switch (threads) {
-    case 64: launch_fft_t<OT, 64>(k, p, blocks, lds, st); break;
-    case 128: launch_fft_t<OT, 128>(k, p, blocks, lds, st); break;
-    case 256: launch_fft_t<OT, 256>(k, p, blocks, lds, st); break;
-    case 512: launch_fft_t<OT, 512>(k, p, blocks, lds, st); break;
-    default: launch_fft_t<OT, 1024>(k, p, blocks, lds, st); break;
+    case 64: launch_fft_t<OT, 64, 8>(k, p, blocks, lds, st); break;
+    case 128: launch_fft_t<OT, 128, 8>(k, p, blocks, lds, st); break;
+    case 256: launch_fft_t<OT, 256, 8>(k, p, blocks, lds, st); break;
+    case 512: launch_fft_t<OT, 512, 8>(k, p, blocks, lds, st); break;
+    default: launch_fft_t<OT, 1024, 8>(k, p, blocks, lds, st); break;
     }
 }
 
@@ -341,7 +362,7 @@ FftPlan fft_plan(int n)
     p.n = n;
     if (n < 2 || n > 16384) return p;
     int m = n;
-    static const int order[] = {8, 4, 2, 3, 5, 7, 11, 13};
+    static const int order[] = {8, 4, 2, 3, 5};
     for (int r : order) {
         while (m % r == 0 && m > 1) {
             if (p.nstages >= 24) return p;
@@ -351,7 +372,7 @@ FftPlan fft_plan(int n)
     }
     if (m != 1) return p;
     int t = 64;
-    while (t < 1024 && t * 16 < n) t *= 2;
+    while (t < 1024 && t * 8 < n) t *= 2;
     p.threads = t;
     p.ok = true;
     return p;
@@ -367,15 +388,16 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
     const long long groups = (a.S + k.SB - 1) / k.SB;
     const long long blocks = groups * a.A * (k.SB / 2);
     if (blocks > 0x7fffffffLL) fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one filter launch");
-    const size_t lds = sizeof(float2) * (size_t)a.npad;
+    const size_t lds = sizeof(float2) * lds_elems(a.npad);
+    const size_t lds_fft = lds + sizeof(float2) * (size_t)(128 + a.npad / 128 + 1);
     if (plan.ok || a.none) {
         FftK p;
         p.n = plan.n;
         p.nstages = plan.nstages;
         for (int i = 0; i < 24; ++i) p.radix[i] = plan.radix[i];
         const int threads = plan.ok ? plan.threads : 256;
-        if (store == RAFT_STORE_F16) launch_fft<__half>(k, p, threads, (int)blocks, lds, st);
-        else launch_fft<float>(k, p, threads, (int)blocks, lds, st);
+        if (store == RAFT_STORE_F16) launch_fft<__half>(k, p, threads, (int)blocks, lds_fft, st);
+        else launch_fft<float>(k, p, threads, (int)blocks, lds_fft, st);
     } else {
         if (lds > 160 * 1024) fail(RAFT_EUNSUPPORTED, "padded row too long for the direct filter path");
         if (store == RAFT_STORE_F16) {

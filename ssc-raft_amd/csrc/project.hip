@@ -7,13 +7,15 @@
 // interleaved transpose instead, so the fast-varying direction across rays is
 // always the contiguous one.
 //
-// Work split: one workgroup (4 waves) per (angle, slice group); each wave
-// takes blocks of R consecutive rays and splits every ray over Kp = 64/R lanes
-// that take its samples k0+kp, k0+kp+Kp, ...  At each step a wave therefore
-// reads an R x Kp patch of the rotated sampling grid, which maps to a compact
-// patch of the image (R is 64 for angles within ~2 degrees of an axis and 8
-// near 45 degrees).  The Kp partial sums of a ray are combined with lane
-// shuffles.
+// Work split: one workgroup per (block of PROJ_ANGLES consecutive angles,
+// slice group), one wave per angle.  The waves walk the same blocks of R
+// consecutive rays in lockstep (a barrier per block), so the 8 nearby angles
+// read nearly the same image region at the same time and the reuse is served
+// by L1/L2.  Every ray is split over Kp = 64/R lanes that take its samples
+// k0+kp, k0+kp+Kp, ...; at each step a wave therefore reads an R x Kp patch of
+// the rotated sampling grid, which maps to a compact patch of the image (R is
+// 64 for angles within ~2 degrees of an axis and 8 near 45 degrees).  The Kp
+// partial sums of a ray are combined with lane shuffles.
 //
 // Sample coordinates are IEEE-double affine functions of the sample index,
 //     ux = fma(k - ck, -sin*ry, fma(i - ci, cos, ucx)),  X = (int)ux
@@ -79,6 +81,17 @@ __device__ __forceinline__ float4 add4(float4 a, float4 b)
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
+// 16-byte gather through a buffer resource: 32-bit byte offsets, and offsets
+// at or past the end of the buffer read as zero (used for the padding samples
+// of the last unrolled step).  Word 3 = 0x00020000 is the gfx9-family raw
+// buffer format.
+__device__ __forceinline__ float4 gather(__amdgpu_buffer_rsrc_t rs, unsigned off)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+constexpr unsigned OFF_NONE = 0xFFFFFFF0u;
+
 __device__ __forceinline__ float ratio(float yv, float v)
 {
     const float e = 1.0e-4f;
@@ -88,25 +101,31 @@ __device__ __forceinline__ float ratio(float yv, float v)
 }
 
 template <int EM>
-__global__ __launch_bounds__(256) void k_project(ProjK a)
+__global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
 {
-    const int ang = blockIdx.x;
-    const int g = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double cs = a.cosv[ang], sn = a.sinv[ang];
-    const double acs = fabs(cs), asn = fabs(sn);
-    const bool steep = asn > acs && a.imgT != nullptr;
-    const size_t plane = (size_t)a.Nx * a.Ny;
-    const float4* __restrict__ img = (steep ? a.imgT : a.img) + (size_t)g * plane;
-    const int sY = steep ? 1 : a.Nx; // element = img[Y * sY + X * sX]
-    const int sX = steep ? a.Ny : 1;
-    // patch shape: R rays x Kp samples per wave step
-    const double tn = steep ? acs / asn : (acs > asn ? asn / acs : 1.0);
+    const int a0 = blockIdx.x * PROJ_ANGLES;
+    const int ang = a0 + wave;
+    const bool have = ang < a.nangles;
+    const int g = blockIdx.y;
+    // layout and patch shape are chosen once per workgroup, from its middle angle
+    const int amid = min(a0 + PROJ_ANGLES / 2, a.nangles - 1);
+    const double cm = fabs(a.cosv[amid]), sm = fabs(a.sinv[amid]);
+    const bool steep = sm > cm && a.imgT != nullptr;
+    const double tn = steep ? cm / sm : (cm > sm ? sm / cm : 1.0);
     const int lgR = tn < 0.031 ? 6 : (tn < 0.125 ? 5 : (tn < 0.5 ? 4 : 3));
     const int R = 1 << lgR, Kp = 64 >> lgR;
     const int r = lane & (R - 1), kp = lane >> lgR;
     const int nblocks = (a.nrays + R - 1) >> lgR;
 
+    const size_t plane = (size_t)a.Nx * a.Ny;
+    const float4* __restrict__ img = (steep ? a.imgT : a.img) + (size_t)g * plane;
+    const int sY = steep ? 1 : a.Nx; // element = img[Y * sY + X * sX]
+    const int sX = steep ? a.Ny : 1;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(img), 0, (int)(plane * sizeof(float4)), 0x00020000);
+
+    const double cs = have ? a.cosv[ang] : 1.0, sn = have ? a.sinv[ang] : 0.0;
     const double bxk = -sn * a.ry;
     const double byi = sn * a.rx;
     // slopes below 1e-9 (but not 0) make the interval estimate ill-conditioned:
@@ -116,10 +135,10 @@ __global__ __launch_bounds__(256) void k_project(ProjK a)
     double klsum[4] = {0.0, 0.0, 0.0, 0.0};
     unsigned long long exc[4] = {0, 0, 0, 0};
 
-    for (int rb = wave; rb < nblocks; rb += 4) {
+    for (int rb = 0; rb < nblocks; ++rb) {
         const int i = (rb << lgR) + r;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < a.nrays) {
+        if (have && i < a.nrays) {
             const double ir = (double)i - a.ci;
             const double uxi = fma(ir, cs, a.ucx);
             const double uyi = fma(ir, byi, a.ucy);
@@ -148,13 +167,24 @@ __global__ __launch_bounds__(256) void k_project(ProjK a)
                     while (k0 > 0 && in_image((double)(k0 - 1) - a.ck, bxk, uxi, cs, uyi, a.Nx, a.Ny)) --k0;
                     while (k1 < a.nk && in_image((double)k1 - a.ck, bxk, uxi, cs, uyi, a.Nx, a.Ny)) ++k1;
                 }
+                // samples k0+kp, k0+kp+Kp, ... < k1, four gathers in flight per step;
+                // kr + u*Kp is exact, so each coordinate is the oracle's expression
+                const int cnt = (k1 > k0 + kp) ? (k1 - k0 - kp + Kp - 1) / Kp : 0;
                 double kr = (double)(k0 + kp) - a.ck;
-                const double kstep = (double)Kp;
-                for (int k = k0 + kp; k < k1; k += Kp) {
-                    const int X = (int)fma(kr, bxk, uxi);
-                    const int Y = (int)fma(kr, cs, uyi);
-                    acc = add4(acc, img[Y * sY + X * sX]);
-                    kr += kstep;
+                const double kstep4 = (double)(4 * Kp);
+                for (int m = 0; m < cnt; m += 4) {
+                    unsigned off[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double kru = kr + (double)(u * Kp);
+                        const int X = (int)fma(kru, bxk, uxi);
+                        const int Y = (int)fma(kru, cs, uyi);
+                        off[u] = (m + u < cnt) ? (unsigned)(Y * sY + X * sX) * 16u : OFF_NONE;
+                    }
+                    const float4 v0 = gather(rs, off[0]), v1 = gather(rs, off[1]);
+                    const float4 v2 = gather(rs, off[2]), v3 = gather(rs, off[3]);
+                    acc = add4(acc, add4(add4(v0, v1), add4(v2, v3)));
+                    kr += kstep4;
                 }
             }
         }
@@ -165,7 +195,7 @@ __global__ __launch_bounds__(256) void k_project(ProjK a)
             acc.z += __shfl_xor(acc.z, off);
             acc.w += __shfl_xor(acc.w, off);
         }
-        if (kp == 0 && i < a.nrays) {
+        if (have && kp == 0 && i < a.nrays) {
             const float v[4] = {acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale};
             if (!EM) {
 #pragma unroll
@@ -191,11 +221,10 @@ __global__ __launch_bounds__(256) void k_project(ProjK a)
                     make_float4(rr[0], rr[1], rr[2], rr[3]);
             }
         }
+        __syncthreads(); // keep the waves (angles) of the workgroup on the same ray block
     }
-    if (!EM || !a.kl_part) return;
-    // deterministic workgroup reduction of the per-slice KL partials
-    __shared__ double sd[4][4];
-    __shared__ unsigned long long se[4][4];
+    if (!EM || !a.kl_part || !have) return;
+    // deterministic per-(slice, angle) KL partial: wave reduction in fixed order
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         double d = klsum[s];
@@ -204,18 +233,10 @@ __global__ __launch_bounds__(256) void k_project(ProjK a)
             d += __shfl_xor(d, off);
             e += __shfl_xor(e, off);
         }
-        if (lane == 0) {
-            sd[wave][s] = d;
-            se[wave][s] = e;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const int s = threadIdx.x;
         const int sl = g * 4 + s;
-        if (sl < a.S) {
-            a.kl_part[(size_t)sl * a.nangles + ang] = ((sd[0][s] + sd[1][s]) + sd[2][s]) + sd[3][s];
-            a.ex_part[(size_t)sl * a.nangles + ang] = se[0][s] + se[1][s] + se[2][s] + se[3][s];
+        if (lane == 0 && sl < a.S) {
+            a.kl_part[(size_t)sl * a.nangles + ang] = d;
+            a.ex_part[(size_t)sl * a.nangles + ang] = e;
         }
     }
 }
@@ -265,9 +286,9 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     k.tlo = p.tlo; k.W = p.W;
     const int G = (p.S + 3) / 4;
     if (G > 65535) fail(RAFT_EUNSUPPORTED, "projector grid too large");
-    dim3 grid(p.g.nangles, G);
-    if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, dim3(256), 0, st, k);
-    else hipLaunchKernelGGL((k_project<0>), grid, dim3(256), 0, st, k);
+    dim3 grid((p.g.nangles + PROJ_ANGLES - 1) / PROJ_ANGLES, G);
+    if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
+    else hipLaunchKernelGGL((k_project<0>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
     RAFT_HIP(hipGetLastError());
 }
 

@@ -327,11 +327,18 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
         Hi[k] = -Hi[n - k];
     }
     if (n % 2 == 0) Hi[n / 2] = 0.0;
-    std::vector<float2> H(n), tw(n);
-    for (int k = 0; k < n; ++k) {
-        H[k] = make_float2((float)Hr[k], (float)Hi[k]);
+    std::vector<float2> H(n);
+    for (int k = 0; k < n; ++k) H[k] = make_float2((float)Hr[k], (float)Hi[k]);
+    // two-level twiddle tables: W^m = hi[m >> 7] * lo[m & 127], W = exp(-2 pi i / n)
+    const int nhi = n / 128 + 1;
+    std::vector<float2> tw(128 + nhi);
+    for (int k = 0; k < 128; ++k) {
         const double ang = -2.0 * M_PI * (double)k / (double)n;
         tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    for (int k = 0; k < nhi; ++k) {
+        const double ang = -2.0 * M_PI * (double)(((long long)k * 128) % n) / (double)n;
+        tw[128 + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     F->plan = fft_plan(n);
     upload(F->H, H, st);
@@ -454,6 +461,7 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
     if (Nx < 2 || Ny < 2) fail(RAFT_EINVAL, "image sizes must be >= 2");
     if (nrays < 1 || nangles < 1 || nslices < 0) fail(RAFT_EINVAL, "invalid projection sizes");
     if (!(ax != 0.0f) || !(ay != 0.0f)) fail(RAFT_EINVAL, "ax, ay must be non-zero");
+    if ((double)Nx * (double)Ny * 16.0 >= 2147483648.0) fail(RAFT_EUNSUPPORTED, "image too large for 32-bit offsets");
     if (!angles) fail(RAFT_EINVAL, "null angles");
     if (nslices == 0) return;
     if (!d_proj || !d_obj) fail(RAFT_EINVAL, "null data pointer");
@@ -507,6 +515,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     if (S < 0 || K < 0) fail(RAFT_EINVAL, "nslices and iterations must be >= 0");
     if (!angles) fail(RAFT_EINVAL, "null angles");
     const double rr = (double)(R - 1) / (double)(n - 1);
+    if ((double)n * (double)n * 16.0 >= 2147483648.0) fail(RAFT_EUNSUPPORTED, "image too large for 32-bit offsets");
     if (rr * std::hypot((double)(BP_TILE_X - 1), (double)(BP_TILE_Y - 1)) + 5.0 > (double)BP_SLOT)
         fail(RAFT_EUNSUPPORTED, "nrays larger than objsize beyond the backprojection tile window");
     if (S == 0) return;

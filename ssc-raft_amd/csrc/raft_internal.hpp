@@ -73,6 +73,8 @@ constexpr int BP_SLOT = 80;    // LDS entries per staged angle window (>= |(63, 
 // slices side by side: [group][angle][bin][slice]
 constexpr int BP_SLICES_F32 = 4;
 constexpr int BP_SLICES_F16 = 8;
+// projector: consecutive angles per workgroup (one wave each)
+constexpr int PROJ_ANGLES = 8;
 
 // global bin window covering every pixel of the tile grid
 void bp_window(BpGeom& g);
@@ -114,7 +116,7 @@ struct FilterArgs {
     const float* tomo;  // [S][A][N]
     void* q;            // [G][A][W][SB] float (SB = 4) or __half (SB = 8)
     const float2* H;    // [npad] Hermitian filter incl. 1/(npad*pixel)
-    const float2* tw;   // [npad] exp(-2 pi i t/npad)
+    const float2* tw;   // two-level twiddles: lo[128] = W^k, hi[npad/128 + 1] = W^(128k), W = exp(-2 pi i/npad)
     const float* g;     // direct-convolution kernel (real, [npad]) or null
     const float* wa;    // [A] per-angle weight (dtheta * angular scale)
     int N, A, S, npad, L, tlo, W;

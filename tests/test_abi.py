@@ -86,7 +86,7 @@ def test_plan_rejects_invalid_parameters():
 
 def test_plan_fft_path_selection():
     ang = np.linspace(0, np.pi, 8, endpoint=False).astype(np.float32)
-    for n, pad, fft in [(2048, 2, 1), (1024, 2, 1), (100, 2, 1), (17, 2, 0), (7000, 2, 0), (33, 0, 1)]:
+    for n, pad, fft in [(2048, 2, 1), (1024, 2, 1), (100, 2, 1), (17, 2, 0), (7000, 2, 0), (33, 0, 0), (45, 0, 1)]:
         paramf, parami = oracle.fbp_params(n, 8, 1, pad=pad)
         assert plan(ang, paramf, parami)[2] == fft, (n, pad)
 
