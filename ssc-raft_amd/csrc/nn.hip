@@ -1,0 +1,831 @@
+// nn.hip -- kernels of the 2-D U-Net that proposes the eEM starting image
+// (gfx950): 3x3 convolution forward / backward-data / backward-weight on BF16
+// MFMA with FP32 accumulation, 2x2 average pooling, nearest 2x upsampling,
+// ReLU masks, the 1x1 output head with its loss, Adam, and the resampling of
+// an FBP image onto the eEM grid.
+//
+// Activations are NHWC bf16 with an explicit channel stride `ld` and channel
+// offset, so a skip concatenation is two writers into one buffer.  Master
+// weights, gradients and optimizer state are FP32; the convolutions read a
+// bf16 copy of their weights, layout [Cout][9][Cin] (tap = 3*(dy+1) + (dx+1)).
+//
+// MFMA v_mfma_f32_32x32x16_bf16, D[32x32] += A[32x16] B[16x32]:
+//   lane l supplies A[l % 32][8(l/32) .. 8(l/32)+7] and B[8(l/32) .. +7][l % 32],
+//   and holds D[8(i/4) + 4(l/32) + (i % 4)][l % 32] in accumulator i (0..15).
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8(l/16) .. +7] and
+//   B[8(l/16) .. +7][l % 16], holds D[4(l/16) + i][l % 16], i < 4.
+// raft_nn_mfma_probe runs both on host-given operands so these layouts are
+// checked before anything relies on them.
+#include "raft_internal.hpp"
+#include "../../include/raft_nn.h"
+
+#include <cmath>
+
+namespace raft {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ u16 bf16_bits(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u)); // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(u16 b) { return __uint_as_float((unsigned)b << 16); }
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
+{
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, 16);
+    return r;
+}
+__device__ __forceinline__ bf16x8 lds8(const u16* p) { return as_bf16x8(*reinterpret_cast<const uint4*>(p)); }
+
+// ---------------------------------------------------------------------------
+// MFMA probe: raw per-lane results of both shapes on row-major operands
+//   A16 [16][32], B16 [32][16] -> out16[lane][4]; A32 [32][16], B32 [16][32] -> out32[lane][16]
+// ---------------------------------------------------------------------------
+__global__ void k_mfma_probe(const u16* A16, const u16* B16, float* out16, const u16* A32, const u16* B32,
+                             float* out32)
+{
+    const int l = threadIdx.x;
+    u16 ta[8], tb[8];
+    for (int j = 0; j < 8; ++j) {
+        ta[j] = A16[(l % 16) * 32 + 8 * (l / 16) + j];
+        tb[j] = B16[(8 * (l / 16) + j) * 16 + (l % 16)];
+    }
+    bf16x8 a, b;
+    __builtin_memcpy(&a, ta, 16);
+    __builtin_memcpy(&b, tb, 16);
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) out16[l * 4 + i] = c[i];
+
+    for (int j = 0; j < 8; ++j) {
+        ta[j] = A32[(l % 32) * 16 + 8 * (l / 32) + j];
+        tb[j] = B32[(8 * (l / 32) + j) * 32 + (l % 32)];
+    }
+    __builtin_memcpy(&a, ta, 16);
+    __builtin_memcpy(&b, tb, 16);
+    f32x16 d;
+    for (int i = 0; i < 16; ++i) d[i] = 0.f;
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
+    for (int i = 0; i < 16; ++i) out32[l * 16 + i] = d[i];
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, forward (and backward-data with flipped weights):
+//   y[n][h][w][cy0 + co] = act(b[co] + sum_{dy,dx,ci} x[n][h+dy][w+dx][cx0 + ci] W[co][tap][ci])
+// zero outside the image.  Block = 4 waves; tile = TH = 4*WR rows x 32 columns
+// of one image x CO_T = 32*WC output channels; wave w owns rows WR*w .. +WR-1.
+// Per 32-channel input chunk the block stages the (TH+2) x 34 halo of x and
+// the chunk's weights in LDS (80-byte pixel / weight-row stride: 32 channels
+// plus 16 bytes of padding), then runs 9 taps x 2 k-halves of 32x32x16 MFMAs
+// with A = weights (rows = output channels) and B = pixels (columns).
+// ---------------------------------------------------------------------------
+constexpr int CONV_PS = 40; // LDS stride (bf16 elements) of a pixel / weight row
+
+struct ConvArgs {
+    const u16* x;
+    const u16* w; // [Cout][9][Cin]
+    const float* b;
+    u16* y;
+    int N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu;
+};
+
+template <int WR, int WC>
+constexpr int conv_lds_bytes()
+{
+    return ((4 * WR + 2) * 34 + 9 * 32 * WC) * CONV_PS * 2;
+}
+
+template <int WR, int WC>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
+{
+    constexpr int TH = 4 * WR, HW = 34, HH = TH + 2, CO_T = 32 * WC;
+    extern __shared__ uint4 conv_lds[];
+    u16* s_in = reinterpret_cast<u16*>(conv_lds);
+    u16* s_w = s_in + HH * HW * CONV_PS;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int tiles_y = (a.H + TH - 1) / TH;
+    const int tx0 = blockIdx.x * 32;
+    const int ty0 = (blockIdx.y % tiles_y) * TH;
+    const int n = blockIdx.y / tiles_y;
+    const int co0 = blockIdx.z * CO_T;
+
+    f32x16 acc[WR][WC];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+        for (int c = 0; c < WC; ++c)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
+
+    const size_t img = (size_t)n * a.H * a.W;
+    for (int c0 = 0; c0 < a.Cin; c0 += 32) {
+        __syncthreads();
+        for (int u = tid; u < HH * HW * 4; u += 256) {
+            const int pix = u >> 2, q = u & 3;
+            const int gy = ty0 + pix / HW - 1, gx = tx0 + pix % HW - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                v = *reinterpret_cast<const uint4*>(a.x + (img + (size_t)gy * a.W + gx) * a.ldx + a.cx0 + c0 + 8 * q);
+            *reinterpret_cast<uint4*>(s_in + pix * CONV_PS + 8 * q) = v;
+        }
+        for (int u = tid; u < 9 * CO_T * 4; u += 256) {
+            const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
+            const uint4 v =
+                *reinterpret_cast<const uint4*>(a.w + ((size_t)(co0 + co) * 9 + tap) * a.Cin + c0 + 8 * q);
+            *reinterpret_cast<uint4*>(s_w + (tap * CO_T + co) * CONV_PS + 8 * q) = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3; // halo offsets
+#pragma unroll
+            for (int kk = 0; kk < 32; kk += 16) {
+                bf16x8 bp[WR], aw[WC];
+#pragma unroll
+                for (int r = 0; r < WR; ++r)
+                    bp[r] = lds8(s_in + ((wave * WR + r + dy) * HW + l32 + dx) * CONV_PS + kk + 8 * hi);
+#pragma unroll
+                for (int c = 0; c < WC; ++c) aw[c] = lds8(s_w + (tap * CO_T + 32 * c + l32) * CONV_PS + kk + 8 * hi);
+#pragma unroll
+                for (int r = 0; r < WR; ++r)
+#pragma unroll
+                    for (int c = 0; c < WC; ++c)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aw[c], bp[r], acc[r][c], 0, 0, 0);
+            }
+        }
+    }
+    // lane holds pixel column tx0 + l32 and channels co0 + 32c + 8g + 4hi + (0..3)
+    if (tx0 + l32 >= a.W) return;
+#pragma unroll
+    for (int r = 0; r < WR; ++r) {
+        if (ty0 + wave * WR + r >= a.H) break;
+        const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
+        u16* yp = a.y + p * a.ldy + a.cy0;
+#pragma unroll
+        for (int c = 0; c < WC; ++c)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int co = co0 + 32 * c + 8 * g + 4 * hi;
+                u16 o[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float v = acc[r][c][4 * g + i] + (a.b ? a.b[co + i] : 0.0f);
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    o[i] = bf16_bits(v);
+                }
+                uint2 pk;
+                pk.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+                pk.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+                *reinterpret_cast<uint2*>(yp + co) = pk;
+            }
+    }
+}
+
+// wt[ci][8 - tap][co] = w[co][tap][ci]: the backward-data weights
+__global__ void k_conv_weight_flip(const u16* w, u16* wt, int Cout, int Cin)
+{
+    const int total = Cout * 9 * Cin;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int co = e / (9 * Cin), tap = (e / Cin) % 9, ci = e % Cin;
+        wt[((size_t)ci * 9 + (8 - tap)) * Cout + co] = w[e];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, backward-weight:
+//   dW[co][tap][ci] = sum_{n,h,w} g[n][h][w][co] x[n][h+dy][w+dx][ci],  db[co] = sum g[..][co]
+// GEMM with M = 32 output channels, N = 32 input channels, K = pixels.  A
+// block owns one (co, ci) tile pair and a contiguous range of 4-row x 32-column
+// pixel tiles; per tile it stages g transposed ([co][pixel]) and three
+// column-shifted transposed copies of the x halo ([dx][ci][row][col]) so that
+// each lane's 8 consecutive K values are one aligned 16-byte LDS read.  Wave w
+// reduces tile row w for all 9 taps; the 4 waves are summed through LDS and
+// the block writes its FP32 partial [9][32][32] (+ 32 bias sums) to
+// part[blockIdx.x]; k_wgrad_reduce adds the partials in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int WG_TH = 4;
+constexpr int WG_ROW = 32 + 8;                  // LDS row stride (elements) of a transposed row
+constexpr int WG_G = 32 * WG_TH * WG_ROW;       // g: [32 co][4 rows][WG_ROW]
+constexpr int WG_X = 32 * (WG_TH + 2) * WG_ROW; // one shifted copy: [32 ci][6 rows][WG_ROW]
+
+struct WgradArgs {
+    const u16* x;
+    const u16* g;
+    float* part; // [gridDim.x][Cout/32][Cin/32][9*32*32 + 32]
+    int N, H, W, Cin, Cout, ldx, cx0, ldg, cg0;
+    int tiles_per_block, total_tiles;
+};
+
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(WgradArgs a)
+{
+    extern __shared__ uint4 wg_lds[];
+    u16* s_g = reinterpret_cast<u16*>(wg_lds);
+    u16* s_x = s_g + WG_G; // three copies
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int cot = blockIdx.y, cit = blockIdx.z;
+    const int tiles_x = (a.W + 31) / 32, tiles_y = (a.H + WG_TH - 1) / WG_TH;
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+    float bsum = 0.0f;
+
+    const int t_begin = blockIdx.x * a.tiles_per_block;
+    const int t_end = min(t_begin + a.tiles_per_block, a.total_tiles);
+    for (int t = t_begin; t < t_end; ++t) {
+        const int tx0 = (t % tiles_x) * 32;
+        const int ty0 = ((t / tiles_x) % tiles_y) * WG_TH;
+        const size_t img = (size_t)(t / (tiles_x * tiles_y)) * a.H * a.W;
+        __syncthreads();
+        // g: 4 rows x 16 column pairs x 4 channel octets = 256 units, one per thread
+        {
+            const int q = tid & 3, pr = (tid >> 2) & 15, row = tid >> 6;
+            const int gx = tx0 + 2 * pr;
+            const size_t p = img + (size_t)(ty0 + row) * a.W + gx;
+            const int off = a.cg0 + 32 * cot + 8 * q;
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+            if (ty0 + row < a.H) {
+                if (gx < a.W) v0 = *reinterpret_cast<const uint4*>(a.g + p * a.ldg + off);
+                if (gx + 1 < a.W) v1 = *reinterpret_cast<const uint4*>(a.g + (p + 1) * a.ldg + off);
+            }
+            const unsigned e0[4] = {v0.x, v0.y, v0.z, v0.w}, e1[4] = {v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const unsigned lo = (e0[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                const unsigned hv = (e1[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                *reinterpret_cast<unsigned*>(s_g + ((8 * q + j) * WG_TH + row) * WG_ROW + 2 * pr) = lo | (hv << 16);
+            }
+        }
+        // x halo: 3 shifts x 6 rows x 16 column pairs x 4 octets = 1152 units
+        for (int u = tid; u < 3 * (WG_TH + 2) * 16 * 4; u += 256) {
+            const int q = u & 3, pr = (u >> 2) & 15, hr = (u >> 6) % (WG_TH + 2), dx = (u >> 6) / (WG_TH + 2);
+            const int gy = ty0 + hr - 1;
+            const int gx0 = tx0 + 2 * pr + dx - 1;
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+            if (gy >= 0 && gy < a.H) {
+                const size_t rowp = img + (size_t)gy * a.W;
+                const int off = a.cx0 + 32 * cit + 8 * q;
+                if (gx0 >= 0 && gx0 < a.W) v0 = *reinterpret_cast<const uint4*>(a.x + (rowp + gx0) * a.ldx + off);
+                if (gx0 + 1 >= 0 && gx0 + 1 < a.W)
+                    v1 = *reinterpret_cast<const uint4*>(a.x + (rowp + gx0 + 1) * a.ldx + off);
+            }
+            const unsigned e0[4] = {v0.x, v0.y, v0.z, v0.w}, e1[4] = {v1.x, v1.y, v1.z, v1.w};
+            u16* dst = s_x + dx * WG_X;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const unsigned lo = (e0[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                const unsigned hv = (e1[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+                *reinterpret_cast<unsigned*>(dst + ((8 * q + j) * (WG_TH + 2) + hr) * WG_ROW + 2 * pr) =
+                    lo | (hv << 16);
+            }
+        }
+        __syncthreads();
+        // wave = tile row; K = its 32 pixels in two halves of 16
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int kk = 0; kk < 32; kk += 16) {
+                const bf16x8 A = lds8(s_g + (l32 * WG_TH + wave) * WG_ROW + kk + 8 * hi);
+                const bf16x8 B = lds8(s_x + dx * WG_X + (l32 * (WG_TH + 2) + wave + dy) * WG_ROW + kk + 8 * hi);
+                acc[tap] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, acc[tap], 0, 0, 0);
+            }
+        }
+        if (cit == 0 && tid < 32) {
+            for (int row = 0; row < WG_TH; ++row)
+                for (int c = 0; c < 32; ++c) bsum += bf16_float(s_g[(tid * WG_TH + row) * WG_ROW + c]);
+        }
+    }
+    // sum the 4 waves' accumulators through LDS (reusing the staging area)
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(wg_lds); // [9][32][32]
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int co = 8 * (i / 4) + 4 * hi + (i % 4);
+                    float* d = red + (tap * 32 + co) * 32 + l32;
+                    *d = (w == 0 ? 0.0f : *d) + acc[tap][i];
+                }
+        }
+        __syncthreads();
+    }
+    float* out = a.part + ((size_t)(blockIdx.x * gridDim.y + cot) * gridDim.z + cit) * (9 * 32 * 32 + 32);
+    for (int e = tid; e < 9 * 32 * 32; e += 256) out[e] = red[e];
+    if (tid < 32) out[9 * 32 * 32 + tid] = bsum;
+}
+
+// dW[co][tap][ci] (+)= sum over partials in order; db likewise
+__global__ void k_wgrad_reduce(const float* part, int nparts, int ncot, int ncit, float* dw, float* db, int Cin,
+                               int accumulate)
+{
+    const int per = 9 * 32 * 32 + 32;
+    const int total = ncot * ncit * per;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int tile = e / per, k = e % per;
+        const int cot = tile / ncit, cit = tile % ncit;
+        float s = 0.0f;
+        for (int b = 0; b < nparts; ++b) s += part[(size_t)b * total + e];
+        if (k < 9 * 32 * 32) {
+            const int tap = k / 1024, co = (k / 32) % 32, ci = k % 32;
+            float* d = dw + ((size_t)(32 * cot + co) * 9 + tap) * Cin + 32 * cit + ci;
+            *d = accumulate ? *d + s : s;
+        } else if (cit == 0 && db) {
+            float* d = db + 32 * cot + (k - 9 * 32 * 32);
+            *d = accumulate ? *d + s : s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise / layout kernels (NHWC bf16, channel stride + offset everywhere)
+// ---------------------------------------------------------------------------
+struct View {
+    u16* p;
+    int ld, c0;
+};
+struct CView {
+    const u16* p;
+    int ld, c0;
+};
+
+// out = g * (act > 0)
+__global__ void k_relu_mask(CView g, CView act, View out, long long P, int C)
+{
+    const long long total = P * C;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long p = e / C;
+        const int c = (int)(e % C);
+        const u16 gv = g.p[p * g.ld + g.c0 + c];
+        out.p[p * out.ld + out.c0 + c] = bf16_float(act.p[p * act.ld + act.c0 + c]) > 0.0f ? gv : (u16)0;
+    }
+}
+
+// 2x2 average pool: y (Ho x Wo) from x (2Ho x 2Wo)
+__global__ void k_pool2(CView x, View y, int N, int Ho, int Wo, int C)
+{
+    const long long total = (long long)N * Ho * Wo * C;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % C);
+        const long long q = e / C;
+        const int w = (int)(q % Wo), h = (int)((q / Wo) % Ho);
+        const long long n = q / ((long long)Wo * Ho);
+        const long long base = (n * 2 * Ho + 2 * h) * 2 * Wo + 2 * w;
+        const float s = bf16_float(x.p[base * x.ld + x.c0 + c]) + bf16_float(x.p[(base + 1) * x.ld + x.c0 + c]) +
+                        bf16_float(x.p[(base + 2 * Wo) * x.ld + x.c0 + c]) +
+                        bf16_float(x.p[(base + 2 * Wo + 1) * x.ld + x.c0 + c]);
+        y.p[q * y.ld + y.c0 + c] = bf16_bits(0.25f * s);
+    }
+}
+
+// Gradient reaching a skip tensor s (post-ReLU) from both of its readers:
+//   out = (gcat + pool_backward(gpool)) * (s > 0),  pool_backward(g)[2h+a][2w+b] = g[h][w] / 4
+__global__ void k_skip_grad(CView gcat, CView gpool, CView act, View out, int N, int H, int W, int C)
+{
+    const long long total = (long long)N * H * W * C;
+    const int Ho = H / 2, Wo = W / 2;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % C);
+        const long long p = e / C;
+        const int w = (int)(p % W), h = (int)((p / W) % H);
+        const long long n = p / ((long long)W * H);
+        const long long pp = (n * Ho + h / 2) * Wo + w / 2;
+        float v = bf16_float(gcat.p[p * gcat.ld + gcat.c0 + c]) +
+                  0.25f * bf16_float(gpool.p[pp * gpool.ld + gpool.c0 + c]);
+        if (!(bf16_float(act.p[p * act.ld + act.c0 + c]) > 0.0f)) v = 0.0f;
+        out.p[p * out.ld + out.c0 + c] = bf16_bits(v);
+    }
+}
+
+// nearest 2x upsampling (Hl x Wl -> 2Hl x 2Wl) and its backward (sum of the 4)
+__global__ void k_up2(CView x, View y, int N, int Hl, int Wl, int C, int backward)
+{
+    const long long total = (long long)N * Hl * Wl * C;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % C);
+        const long long q = e / C;
+        const int w = (int)(q % Wl), h = (int)((q / Wl) % Hl);
+        const long long n = q / ((long long)Wl * Hl);
+        const long long base = (n * 2 * Hl + 2 * h) * 2 * Wl + 2 * w;
+        const long long hp[4] = {base, base + 1, base + 2 * Wl, base + 2 * Wl + 1};
+        if (!backward) {
+            const u16 v = x.p[q * x.ld + x.c0 + c];
+            for (int k = 0; k < 4; ++k) y.p[hp[k] * y.ld + y.c0 + c] = v;
+        } else {
+            float s = 0.0f;
+            for (int k = 0; k < 4; ++k) s += bf16_float(x.p[hp[k] * x.ld + x.c0 + c]);
+            y.p[q * y.ld + y.c0 + c] = bf16_bits(s);
+        }
+    }
+}
+
+// Network input: x[p][0] = bf16(img[p] * scale), channels 1..31 zero (ld = 32)
+__global__ void k_input(const float* img, u16* x, long long P, float scale)
+{
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+        uint4 v = make_uint4((unsigned)bf16_bits(img[p] * scale), 0u, 0u, 0u);
+        uint4* d = reinterpret_cast<uint4*>(x + p * 32);
+        d[0] = v;
+        v.x = 0u;
+        d[1] = v;
+        d[2] = v;
+        d[3] = v;
+    }
+}
+
+// Output head (32 -> 1, 1x1) with residual and MSE loss.
+//   z = b + sum_c w[c] h[p][c];  pred = res[p] + z
+// With a target: loss += sum (pred - t)^2 (FP64), gz = 2 (pred - t) / P,
+// gh[p][c] = gz w[c] (bf16), and per-block partials of dw (= sum gz h) and db.
+// Without a target the prediction may be clamped from below (positivity of an
+// eEM starting image).
+struct HeadArgs {
+    const u16* h;
+    int ldh, ch0;
+    const float* w;   // [32]
+    const float* b;   // [1]
+    const float* res; // [P] or null
+    float* pred;      // [P]
+    const float* target;
+    u16* gh;
+    int ldgh, cgh0;
+    float* part; // [gridDim.x][33]
+    double* loss;
+    long long P;
+    float floor_value;
+    int clamp;
+};
+
+__global__ __launch_bounds__(256) void k_head(HeadArgs a)
+{
+    __shared__ float s_red[4][33];
+    __shared__ double s_loss[4];
+    float w[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) w[c] = a.w[c];
+    const float b = a.b[0];
+    float dw[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) dw[c] = 0.0f;
+    float db = 0.0f;
+    double loss = 0.0;
+    const float invP = 1.0f / (float)a.P;
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < a.P; p += (long long)gridDim.x * 256) {
+        const uint4* hp = reinterpret_cast<const uint4*>(a.h + p * a.ldh + a.ch0);
+        float hv[32];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = hp[k];
+            const unsigned e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                hv[8 * k + 2 * j] = __uint_as_float(e[j] << 16);
+                hv[8 * k + 2 * j + 1] = __uint_as_float(e[j] & 0xffff0000u);
+            }
+        }
+        float z = b;
+#pragma unroll
+        for (int c = 0; c < 32; ++c) z = fmaf(w[c], hv[c], z);
+        float pr = (a.res ? a.res[p] : 0.0f) + z;
+        if (!a.target) {
+            if (a.clamp) pr = fmaxf(pr, a.floor_value);
+            a.pred[p] = pr;
+            continue;
+        }
+        a.pred[p] = pr;
+        const float d = pr - a.target[p];
+        loss += (double)d * (double)d;
+        const float gz = 2.0f * d * invP;
+        db += gz;
+        unsigned o[16];
+#pragma unroll
+        for (int c = 0; c < 32; c += 2) {
+            dw[c] = fmaf(gz, hv[c], dw[c]);
+            dw[c + 1] = fmaf(gz, hv[c + 1], dw[c + 1]);
+            o[c / 2] = (unsigned)bf16_bits(gz * w[c]) | ((unsigned)bf16_bits(gz * w[c + 1]) << 16);
+        }
+        uint4* gp = reinterpret_cast<uint4*>(a.gh + p * a.ldgh + a.cgh0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) gp[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    }
+    if (!a.target) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        float v = dw[c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) s_red[wave][c] = v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        db += __shfl_xor(db, o);
+        loss += __shfl_xor(loss, o);
+    }
+    if (lane == 0) {
+        s_red[wave][32] = db;
+        s_loss[wave] = loss;
+    }
+    __syncthreads();
+    if (threadIdx.x < 33)
+        a.part[blockIdx.x * 33 + threadIdx.x] =
+            s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
+    if (threadIdx.x == 0) a.loss[blockIdx.x + 1] = s_loss[0] + s_loss[1] + s_loss[2] + s_loss[3];
+}
+
+// dw, db from the head partials; loss[0] = sum of the per-block losses (fixed order)
+__global__ void k_head_reduce(const float* part, const int nparts, float* dw, float* db, double* loss, int accumulate)
+{
+    const int k = threadIdx.x;
+    if (k < 33) {
+        float s = 0.0f;
+        for (int b = 0; b < nparts; ++b) s += part[b * 33 + k];
+        float* d = k < 32 ? dw + k : db;
+        *d = accumulate ? *d + s : s;
+    }
+    if (k == 63) {
+        double s = 0.0;
+        for (int b = 0; b < nparts; ++b) s += loss[b + 1];
+        loss[0] = s;
+    }
+}
+
+// Adam (bias-corrected) on FP32 master weights; refreshes the bf16 copy
+__global__ void k_adam(float* p, const float* g, float* m, float* v, u16* pb, long long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, float gscale)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const float gr = g[e] * gscale;
+        const float mm = b1 * m[e] + (1.0f - b1) * gr;
+        const float vv = b2 * v[e] + (1.0f - b2) * gr * gr;
+        m[e] = mm;
+        v[e] = vv;
+        const float np = p[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+        p[e] = np;
+        if (pb) pb[e] = bf16_bits(np);
+    }
+}
+
+__global__ void k_to_bf16(const float* in, u16* out, long long n)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        out[e] = bf16_bits(in[e]);
+}
+
+// Bilinear resampling with edge clamping: out[s][i][j] = img[s](Y(i), X(j)) with
+// X(j) = ax + bx j, Y(i) = ay + by i in input pixel units.
+__global__ void k_resample(const float* img, float* out, int S, int Hi, int Wi, int Ho, int Wo, double ax, double bx,
+                           double ay, double by)
+{
+    const long long total = (long long)S * Ho * Wo;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(e % Wo), i = (int)((e / Wo) % Ho);
+        const long long s = e / ((long long)Wo * Ho);
+        double X = fma((double)j, bx, ax), Y = fma((double)i, by, ay);
+        X = fmin(fmax(X, 0.0), (double)(Wi - 1));
+        Y = fmin(fmax(Y, 0.0), (double)(Hi - 1));
+        const int x0 = (int)X, y0 = (int)Y;
+        const int x1 = min(x0 + 1, Wi - 1), y1 = min(y0 + 1, Hi - 1);
+        const float fx = (float)(X - x0), fy = (float)(Y - y0);
+        const float* b = img + s * (long long)Hi * Wi;
+        const float v00 = b[(long long)y0 * Wi + x0], v01 = b[(long long)y0 * Wi + x1];
+        const float v10 = b[(long long)y1 * Wi + x0], v11 = b[(long long)y1 * Wi + x1];
+        const float top = fmaf(fx, v01 - v00, v00), bot = fmaf(fx, v11 - v10, v10);
+        out[e] = fmaf(fy, bot - top, top);
+    }
+}
+
+int grid_for(long long n, int block = 256)
+{
+    const long long b = (n + block - 1) / block;
+    return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+int launched() { return hipGetLastError() == hipSuccess ? RAFT_OK : RAFT_EHIP; }
+
+template <int WR, int WC>
+int launch_conv(const ConvArgs& a, hipStream_t st)
+{
+    constexpr int lds = conv_lds_bytes<WR, WC>();
+    static bool attr = false;
+    if (!attr) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3<WR, WC>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+            return RAFT_EHIP;
+        attr = true;
+    }
+    dim3 grid((a.W + 31) / 32, ((a.H + 4 * WR - 1) / (4 * WR)) * a.N, a.Cout / (32 * WC));
+    if ((long long)((a.H + 4 * WR - 1) / (4 * WR)) * a.N > 65535) return RAFT_EINVAL;
+    hipLaunchKernelGGL((k_conv3x3<WR, WC>), grid, dim3(256), lds, st, a);
+    return launched();
+}
+
+struct WgradPlan {
+    int nblk, per, tiles, pairs;
+};
+
+bool wgrad_plan(int N, int H, int W, int Cin, int Cout, WgradPlan& p)
+{
+    if (N < 1 || W < 1 || H < 1 || Cin % 32 || Cout % 32 || Cin < 32 || Cout < 32) return false;
+    p.tiles = N * ((H + WG_TH - 1) / WG_TH) * ((W + 31) / 32);
+    p.pairs = (Cout / 32) * (Cin / 32);
+    int nblk = (1024 + p.pairs - 1) / p.pairs;
+    if (nblk > p.tiles) nblk = p.tiles;
+    p.per = (p.tiles + nblk - 1) / nblk;
+    p.nblk = (p.tiles + p.per - 1) / p.per;
+    return true;
+}
+
+} // namespace
+} // namespace raft
+
+using namespace raft;
+
+extern "C" {
+
+int raft_nn_mfma_probe(const unsigned short* A16, const unsigned short* B16, float* out16, const unsigned short* A32,
+                       const unsigned short* B32, float* out32, void* stream)
+{
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, S(stream), A16, B16, out16, A32, B32, out32);
+    return launched();
+}
+
+int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b,
+                    unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
+                    void* stream)
+{
+    if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cout < 32 || Cin % 32 || Cout % 32 || ldx % 8 || cx0 % 8 ||
+        ldy % 4 || cy0 % 4 || cx0 + Cin > ldx || cy0 + Cout > ldy)
+        return RAFT_EINVAL;
+    ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu};
+    if (variant == 0) variant = Cout % 64 == 0 ? 1 : 2;
+    switch (variant) {
+    case 1:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_conv<2, 2>(a, S(stream));
+    case 2:
+        return launch_conv<4, 1>(a, S(stream));
+    case 3:
+        return launch_conv<1, 1>(a, S(stream));
+    case 4:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_conv<4, 2>(a, S(stream));
+    case 5:
+        return launch_conv<2, 1>(a, S(stream));
+    default:
+        return RAFT_EINVAL;
+    }
+}
+
+int raft_nn_conv_weight_flip(const unsigned short* w, unsigned short* wt, int Cout, int Cin, void* stream)
+{
+    hipLaunchKernelGGL(k_conv_weight_flip, dim3(grid_for((long long)Cout * 9 * Cin)), dim3(256), 0, S(stream), w, wt,
+                       Cout, Cin);
+    return launched();
+}
+
+long long raft_nn_wgrad_workspace(int N, int H, int W, int Cin, int Cout)
+{
+    WgradPlan p;
+    if (!wgrad_plan(N, H, W, Cin, Cout, p)) return -1;
+    return (long long)p.nblk * p.pairs * (9 * 32 * 32 + 32);
+}
+
+int raft_nn_conv3x3_wgrad(const unsigned short* x, int ldx, int cx0, const unsigned short* g, int ldg, int cg0,
+                          float* dw, float* db, float* work, long long work_floats, int N, int H, int W, int Cin,
+                          int Cout, int accumulate, void* stream)
+{
+    WgradPlan p;
+    if (!wgrad_plan(N, H, W, Cin, Cout, p) || ldx % 8 || cx0 % 8 || ldg % 8 || cg0 % 8 || cx0 + Cin > ldx ||
+        cg0 + Cout > ldg)
+        return RAFT_EINVAL;
+    if (work_floats < (long long)p.nblk * p.pairs * (9 * 32 * 32 + 32)) return RAFT_EINVAL;
+    WgradArgs a{x, g, work, N, H, W, Cin, Cout, ldx, cx0, ldg, cg0, p.per, p.tiles};
+    constexpr int lds = (WG_G + 3 * WG_X) * 2;
+    static_assert(lds >= 9 * 32 * 32 * 4, "the wave reduction reuses the staging LDS");
+    static bool attr = false;
+    if (!attr) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3_wgrad),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+            return RAFT_EHIP;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(p.nblk, Cout / 32, Cin / 32), dim3(256), lds, S(stream), a);
+    if (hipGetLastError() != hipSuccess) return RAFT_EHIP;
+    const long long outn = (long long)p.pairs * (9 * 32 * 32 + 32);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(outn)), dim3(256), 0, S(stream), work, p.nblk, Cout / 32,
+                       Cin / 32, dw, db, Cin, accumulate);
+    return launched();
+}
+
+int raft_nn_relu_mask(const unsigned short* g, int ldg, int cg0, const unsigned short* act, int lda, int ca0,
+                      unsigned short* out, int ldo, int co0, long long P, int C, void* stream)
+{
+    hipLaunchKernelGGL(k_relu_mask, dim3(grid_for(P * C)), dim3(256), 0, S(stream), CView{g, ldg, cg0},
+                       CView{act, lda, ca0}, View{out, ldo, co0}, P, C);
+    return launched();
+}
+
+int raft_nn_pool2(const unsigned short* x, int ldx, int cx0, unsigned short* y, int ldy, int cy0, int N, int Ho,
+                  int Wo, int C, void* stream)
+{
+    hipLaunchKernelGGL(k_pool2, dim3(grid_for((long long)N * Ho * Wo * C)), dim3(256), 0, S(stream),
+                       CView{x, ldx, cx0}, View{y, ldy, cy0}, N, Ho, Wo, C);
+    return launched();
+}
+
+int raft_nn_skip_grad(const unsigned short* gcat, int ldc, int cc0, const unsigned short* gpool, int ldp, int cp0,
+                      const unsigned short* act, int lda, int ca0, unsigned short* out, int ldo, int co0, int N, int H,
+                      int W, int C, void* stream)
+{
+    if (H % 2 || W % 2) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_skip_grad, dim3(grid_for((long long)N * H * W * C)), dim3(256), 0, S(stream),
+                       CView{gcat, ldc, cc0}, CView{gpool, ldp, cp0}, CView{act, lda, ca0}, View{out, ldo, co0}, N, H,
+                       W, C);
+    return launched();
+}
+
+int raft_nn_up2(const unsigned short* x, int ldx, int cx0, unsigned short* y, int ldy, int cy0, int N, int Hl, int Wl,
+                int C, int backward, void* stream)
+{
+    hipLaunchKernelGGL(k_up2, dim3(grid_for((long long)N * Hl * Wl * C)), dim3(256), 0, S(stream), CView{x, ldx, cx0},
+                       View{y, ldy, cy0}, N, Hl, Wl, C, backward);
+    return launched();
+}
+
+int raft_nn_input(const float* img, unsigned short* x, long long P, float scale, void* stream)
+{
+    hipLaunchKernelGGL(k_input, dim3(grid_for(P)), dim3(256), 0, S(stream), img, x, P, scale);
+    return launched();
+}
+
+// Workspaces of raft_nn_head with a target: floats for the dw/db partials and
+// doubles for the loss (slot 0 receives the total).
+long long raft_nn_head_workspace(long long P) { return (long long)grid_for(P) * 33; }
+long long raft_nn_head_loss_slots(long long P) { return (long long)grid_for(P) + 1; }
+
+int raft_nn_head(const unsigned short* h, int ldh, int ch0, const float* w, const float* b, const float* res,
+                 float* pred, const float* target, unsigned short* gh, int ldgh, int cgh0, float* dw, float* db,
+                 double* loss, float* work, long long P, float floor_value, int clamp, int accumulate, void* stream)
+{
+    if (P < 1 || ldh % 8 || ch0 % 8 || ch0 + 32 > ldh) return RAFT_EINVAL;
+    if (target && (ldgh % 8 || cgh0 % 8 || cgh0 + 32 > ldgh || !gh || !dw || !db || !loss || !work))
+        return RAFT_EINVAL;
+    const int nb = grid_for(P);
+    HeadArgs a{h, ldh, ch0, w, b, res, pred, target, gh, ldgh, cgh0, work, loss, P, floor_value, clamp};
+    hipLaunchKernelGGL(k_head, dim3(nb), dim3(256), 0, S(stream), a);
+    if (hipGetLastError() != hipSuccess) return RAFT_EHIP;
+    if (!target) return RAFT_OK;
+    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, S(stream), work, nb, dw, db, loss, accumulate);
+    return launched();
+}
+
+int raft_nn_adam(float* p, const float* g, float* m, float* v, unsigned short* pb, long long n, float lr, float b1,
+                 float b2, float eps, int t, float gscale, void* stream)
+{
+    if (t < 1 || n < 0) return RAFT_EINVAL;
+    const float c1 = (float)(1.0 - std::pow((double)b1, (double)t));
+    const float c2 = (float)(1.0 - std::pow((double)b2, (double)t));
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, S(stream), p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2,
+                       gscale);
+    return launched();
+}
+
+int raft_nn_to_bf16(const float* in, unsigned short* out, long long n, void* stream)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n)), dim3(256), 0, S(stream), in, out, n);
+    return launched();
+}
+
+int raft_nn_resample(const float* img, float* out, int S_, int Hi, int Wi, int Ho, int Wo, double ax, double bx,
+                     double ay, double by, void* stream)
+{
+    if (S_ < 1 || Hi < 1 || Wi < 1 || Ho < 1 || Wo < 1) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_resample, dim3(grid_for((long long)S_ * Ho * Wo)), dim3(256), 0, S(stream), img, out, S_, Hi,
+                       Wi, Ho, Wo, ax, bx, ay, by);
+    return launched();
+}
+
+} // extern "C"

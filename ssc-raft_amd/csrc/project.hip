@@ -243,7 +243,8 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
 
 // kl[s*K + iter] = sum of the slice's partials, fixed order
 __global__ __launch_bounds__(256) void k_kl_reduce(const double* part, const unsigned long long* expart, int nb,
-                                                   double* kl, unsigned long long* exo, int K, int iter)
+                                                   double* kl, unsigned long long* exo, int K, int iter,
+                                                   const int* iter_ptr)
 {
     const int s = blockIdx.x;
     __shared__ double sd[256];
@@ -264,10 +265,17 @@ __global__ __launch_bounds__(256) void k_kl_reduce(const double* part, const uns
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && iter < K) {
-        kl[(size_t)s * K + iter] = sd[0];
-        exo[(size_t)s * K + iter] = se[0];
+    const int it = iter_ptr ? *iter_ptr : iter;
+    if (threadIdx.x == 0 && it >= 0 && it < K) {
+        kl[(size_t)s * K + it] = sd[0];
+        exo[(size_t)s * K + it] = se[0];
     }
+}
+
+// iteration counter for graph-replayed eEM iterations: op 0 clears, op 1 increments
+__global__ void k_counter(int* c, int op)
+{
+    if (threadIdx.x == 0) *c = op ? *c + 1 : 0;
 }
 
 } // namespace
@@ -293,10 +301,16 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
 }
 
 void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,
-                      unsigned long long* ex, int K, int iter, hipStream_t st)
+                      unsigned long long* ex, int K, int iter, const int* iter_ptr, hipStream_t st)
 {
     if (S <= 0) return;
-    hipLaunchKernelGGL(k_kl_reduce, dim3(S), dim3(256), 0, st, part, expart, nb, kl, ex, K, iter);
+    hipLaunchKernelGGL(k_kl_reduce, dim3(S), dim3(256), 0, st, part, expart, nb, kl, ex, K, iter, iter_ptr);
+    RAFT_HIP(hipGetLastError());
+}
+
+void launch_counter(int* c, int op, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_counter, dim3(1), dim3(64), 0, st, c, op);
     RAFT_HIP(hipGetLastError());
 }
 

@@ -14,6 +14,7 @@
 //     spectrum, FFT twiddles) is built on the host in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -80,6 +81,7 @@ template <typename F> static int guarded(F&& f)
 // profiling (events around kernel families, resolved lazily)
 // ---------------------------------------------------------------------------
 static thread_local bool g_prof = false;
+static bool prof_enabled() { return g_prof; }
 struct ProfRec {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KF_COUNT];
 };
@@ -249,11 +251,13 @@ struct FilterTables {
 };
 
 struct Device {
+    static constexpr int kSlots = 8;
     std::mutex mu;
     std::map<std::string, std::unique_ptr<FilterTables>> filters;
-    DevBuf ws[6];
+    DevBuf ws[kSlots];
     void* workspace(int slot, size_t bytes)
     {
+        if (slot < 0 || slot >= kSlots) fail(RAFT_EINVAL, "internal: workspace slot out of range");
         if (ws[slot].bytes < bytes) {
             ws[slot].release();
             ws[slot].alloc(bytes);
@@ -593,32 +597,72 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // ratio rows: margins stay zero, the projector writes [0, R)
     RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)Gb * A * g.width * 16, st));
 
+    // one iteration: projector + ratio (+ KL partials), KL reduction, backprojection + update
+    int* counter = reinterpret_cast<int*>(static_cast<char*>(D.workspace(6, 256)));
+    auto iteration = [&](int it, int nb, const float* y, double* klo, unsigned long long* exo, bool graph,
+                         hipStream_t s) {
+        ProjArgs p;
+        p.img = xil; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
+        p.kl_part = want_kl ? part : nullptr;
+        p.ex_part = want_kl ? expart : nullptr;
+        p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
+        p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
+        {
+            ProfScope ps(KF_PROJ, s);
+            launch_project(p, 1, s);
+        }
+        if (want_kl) launch_kl_reduce(part, expart, A, nb, klo, exo, K, it, graph ? counter : nullptr, s);
+        BpArgs b;
+        b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
+        b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
+        b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
+        {
+            ProfScope ps(KF_EMBP, s);
+            launch_backproject(b, RAFT_STORE_F32, s);
+        }
+        if (graph) launch_counter(counter, 1, s);
+    };
+    // Iterations are replayed from a captured hipGraph unless per-kernel timing is on
+    // (RAFT_EM_GRAPH=0 disables it).  The graph holds kernels only: the counter it
+    // advances is cleared by a kernel outside it.
+    const char* genv = std::getenv("RAFT_EM_GRAPH");
+    const bool use_graph = K >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
+
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
         float* x = d_obj + (size_t)s0 * img;
         const float* y = d_tomo + (size_t)s0 * A * R;
+        double* klo = want_kl ? dkl + (size_t)s0 * K : nullptr;
+        unsigned long long* exo = want_kl ? dex + (size_t)s0 * K : nullptr;
         launch_to_interleaved(x, xil, nb, n, n, false, st);
         launch_to_interleaved(x, xT, nb, n, n, true, st);
-        for (int it = 0; it < K; ++it) {
-            ProjArgs p;
-            p.img = xil; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
-            p.kl_part = want_kl ? part : nullptr;
-            p.ex_part = want_kl ? expart : nullptr;
-            p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
-            p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
-            {
-                ProfScope ps(KF_PROJ, st);
-                launch_project(p, 1, st);
+        if (use_graph) {
+            launch_counter(counter, 0, st);
+            hipStream_t cs = nullptr;
+            RAFT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                try {
+                    iteration(0, nb, y, klo, exo, true, cs);
+                } catch (...) {
+                    (void)hipStreamEndCapture(cs, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    (void)hipStreamDestroy(cs);
+                    throw;
+                }
+                e = hipStreamEndCapture(cs, &graph);
             }
-            if (want_kl) launch_kl_reduce(part, expart, A, nb, dkl + (size_t)s0 * K, dex + (size_t)s0 * K, K, it, st);
-            BpArgs b;
-            b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
-            b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
-            b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
-            {
-                ProfScope ps(KF_EMBP, st);
-                launch_backproject(b, RAFT_STORE_F32, st);
-            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e == hipSuccess)
+                for (int it = 0; it < K && e == hipSuccess; ++it) e = hipGraphLaunch(exec, st);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipStreamDestroy(cs);
+            RAFT_HIP(e);
+        } else {
+            for (int it = 0; it < K; ++it) iteration(it, nb, y, klo, exo, false, st);
         }
         launch_from_interleaved(xil, x, nb, n, n, st);
     }

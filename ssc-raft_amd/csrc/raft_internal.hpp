@@ -160,8 +160,11 @@ void launch_to_interleaved(const float* in, float* out, int S, int rows, int col
 void launch_from_interleaved(const float* in, float* out, int S, int rows, int cols, hipStream_t st);
 // rows for B(1): interleaved [A][W][4], value wa[a] on valid bins, 0 elsewhere
 void launch_fill_rows(float* q, const float* wa, int A, int W, int tlo, int nbins, hipStream_t st);
+// kl[s*K + it] for it = *iter_ptr (if iter_ptr) or iter
 void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,
-                      unsigned long long* ex, int K, int iter, hipStream_t st);
+                      unsigned long long* ex, int K, int iter, const int* iter_ptr, hipStream_t st);
+// device iteration counter: op 0 clears, op 1 increments (a kernel, so it may be graph-captured)
+void launch_counter(int* c, int op, hipStream_t st);
 
 // profiling hooks (events around kernel families)
 enum KernelFamily { KF_FILTER = 0, KF_BP = 1, KF_PROJ = 2, KF_EMBP = 3, KF_COUNT = 4 };

@@ -1,6 +1,6 @@
 """CPU checks of the C ABI and of the host logic (no GPU compute).
 
-* libraft.so loads and exports every function include/raft.h declares;
+* libraft.so loads and exports every function include/*.h declares;
 * without a GPU the reference entry points fail with an error code and a
   message (the reference exit()s), and the library stays usable;
 * the planning query (raft_fbp_plan_info) rejects what the reference cannot
@@ -10,6 +10,7 @@
   (fbp_methods.py:48-95, em_methods.py:52-63, rafttypes.py:485-519).
 """
 import ctypes
+import glob
 import os
 import re
 
@@ -24,16 +25,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "raft.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text)))
+    names = set()
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
     import sscRaft
 
     names = declared_functions()
-    assert {"getFBPMultiGPU", "get_eEM_RT_MultiGPU", "getRadonRTMultiGPU", "raft_last_error"} <= set(names)
+    assert {"getFBPMultiGPU", "get_eEM_RT_MultiGPU", "getRadonRTMultiGPU", "raft_last_error",
+            "raft_nn_conv3x3"} <= set(names)
     lib = ctypes.CDLL(sscRaft.LIB_PATH)
     for n in names:
         assert hasattr(lib, n), n

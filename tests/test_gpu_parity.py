@@ -209,3 +209,22 @@ def test_errors(gpu):
     # library still usable afterwards
     out = sscRaft.fbp(np.zeros((1, 8, 16), np.float32), dic={"gpu": [0], "angles[rad]": ph.uniform_angles(8)})
     assert np.all(out == 0)
+
+
+def test_eem_graph_replay_matches_direct_launches(gpu, monkeypatch):
+    """hipGraph-replayed iterations (default) and direct launches give identical results."""
+    import sscRaft
+
+    n, a, k = 64, 60, 7
+    img = ph.phantoms(n, 5, config=3, first=9)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=4), 0)
+    monkeypatch.setenv("RAFT_EM_GRAPH", "1")
+    x1, kl1, ex1 = sscRaft.eEMRT_GPU_KL(y, ang, k, [0], 0)
+    monkeypatch.setenv("RAFT_EM_GRAPH", "0")
+    x2, kl2, ex2 = sscRaft.eEMRT_GPU_KL(y, ang, k, [0], 0)
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(kl1, kl2)
+    np.testing.assert_array_equal(ex1, ex2)
+    xr, klr, _ = oracle.eem(y, ang, k)
+    assert rel(x1, xr) <= 1e-5
