@@ -13,16 +13,22 @@ Also reported on the same JSON line:
   cpu_baseline  the FP32 restatement in oracle/ timed on this host's cores on
                 a bounded sample (whole slices of the same workload);
   fp16_rows     the same FBP with FP16-stored filtered rows;
-  check         rel-L2 / max-abs error of the timed output against the FP64
-                oracle on a few slices;
   em            plain eEM on 16 slices of 1024^2 (1024 angles, I0 = 1e4,
-                uniform start, 200 iterations, FP64 KL per iteration).
+                uniform start, 200 iterations, FP64 KL per iteration);
+  em.learned_init  time-to-KL of the U-Net warm start vs plain eEM on held-out
+                512^2 slices (128 angles, I0 = 1e4), with the trained weights
+                in ssc-raft_amd/sscRaft/weights/ (null if they are absent).
+Parity of the timed paths against the FP64 oracle is the job of tests/ (the
+2048^2 FBP case is tests/test_gpu_parity.py::test_fbp_2048); the oracle is
+used here only for the CPU baseline.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ssc-raft_amd"))
@@ -58,7 +64,32 @@ def read_traffic(name):
         return None
 
 
-def fbp_bench(args, torch, dv, ph, oracle):
+def fbp_dic(N):
+    return {"filter": "ramp", "beta/delta": 0.0, "regularization": 1.0, "rotation axis offset": 0.0,
+            "blocksize": 0, "energy[eV]": 1.0, "z2[m]": 0.0, "detectorPixel[m]": 2.0 / N, "padding": 2}
+
+
+def cpu_baseline(args, y, ang, N, A):
+    """The FP32 C restatement (oracle/) on this host's cores, on whole slices."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    from sscRaft.geometries.parallel.fbp_methods import fbp_params
+
+    ys = y[: args.cpu_slices].cpu().numpy()
+    pf1, pi1 = fbp_params(N, A, args.cpu_slices, fbp_dic(N))
+    cores = host_cores()
+    t0 = time.perf_counter()
+    oracle.fbp(ys, ang, pf1, pi1, prec="f32", nthreads=cores)
+    dt = time.perf_counter() - t0
+    cpu = dict(value=args.cpu_slices / dt, unit="slices/s", cores=cores, kind="port",
+               sample=f"{args.cpu_slices} of the 64 slices (2048^2, 2048 angles, pad 2, ramp), "
+                      f"FP32 C restatement with FP64 bin indices, {dt:.2f} s")
+    log(f"CPU baseline: {cpu['value']:.3f} slices/s on {cores} threads")
+    return cpu
+
+
+def fbp_bench(args, torch, dv, ph):
     N = A = 2048
     S = 64
     dev = torch.device("cuda:0")
@@ -77,7 +108,9 @@ def fbp_bench(args, torch, dv, ph, oracle):
         torch.cuda.synchronize()
         log(f"copy calibration: {3} copies of {4 * y.numel() / 2**20:.1f} MiB")
         del y2
-    paramf, parami = oracle.fbp_params(N, A, S, pad=2, filter_type=7, pixel=2.0 / N)
+    from sscRaft.geometries.parallel.fbp_methods import fbp_params
+
+    paramf, parami = fbp_params(N, A, S, fbp_dic(N))
     tlo, width, fft_path = dv.fbp_plan_info(ang, paramf, parami)
     out = torch.empty((S, N, N), dtype=torch.float32, device=dev)
     res = {}
@@ -99,34 +132,7 @@ def fbp_bench(args, torch, dv, ph, oracle):
                           bp_ms=bp_ms, filter_ms=filt_ms)
         log(f"FBP {store}: {res[store]['slices_per_s']:.1f} slices/s, step {res[store]['ms_per_step']:.2f} ms, "
             f"backprojection {bp_ms:.3f} ms/launch, filter {filt_ms:.3f} ms/launch")
-        if store == "f32":
-            out32 = out[: args.check_slices].cpu().numpy() if args.check_slices else None
-    # error of the timed (FP32) output against the FP64 oracle
-    check = None
-    if args.check_slices:
-        import numpy as np
-
-        ys = y[: args.check_slices].cpu().numpy()
-        pf1, pi1 = oracle.fbp_params(N, A, args.check_slices, pad=2, filter_type=7, pixel=2.0 / N)
-        t0 = time.perf_counter()
-        ref = oracle.fbp(ys, ang, pf1, pi1, prec="f64", nthreads=host_cores())
-        e = float(np.linalg.norm(out32 - ref) / np.linalg.norm(ref))
-        check = dict(slices=args.check_slices, rel_l2=e, max_abs=float(np.abs(out32 - ref).max()),
-                     ref_max_abs=float(np.abs(ref).max()))
-        log(f"check vs FP64 oracle on {args.check_slices} slices: rel-L2 {e:.3e} ({time.perf_counter() - t0:.1f} s)")
-    # CPU baseline: FP32 restatement, whole slices of the same workload
-    cpu = None
-    if not args.no_cpu:
-        ys = y[: args.cpu_slices].cpu().numpy()
-        pf1, pi1 = oracle.fbp_params(N, A, args.cpu_slices, pad=2, filter_type=7, pixel=2.0 / N)
-        cores = host_cores()
-        t0 = time.perf_counter()
-        oracle.fbp(ys, ang, pf1, pi1, prec="f32", nthreads=cores)
-        dt = time.perf_counter() - t0
-        cpu = dict(value=args.cpu_slices / dt, unit="slices/s", cores=cores, kind="port",
-                   sample=f"{args.cpu_slices} of the 64 slices (2048^2, 2048 angles, pad 2, ramp), "
-                          f"FP32 C restatement with FP64 bin indices, {dt:.2f} s")
-        log(f"CPU baseline: {cpu['value']:.3f} slices/s on {cores} threads")
+    cpu = None if args.no_cpu else cpu_baseline(args, y, ang, N, A)
     # roofline of the backprojection kernel (FP32 rows)
     samples = float(S) * A * N * N
     bytes_alg = samples * 4 + float(S) * A * width * 4 + float(S) * N * N * 4
@@ -137,7 +143,7 @@ def fbp_bench(args, torch, dv, ph, oracle):
                 kernel="k_backproject<float,0>", bytes_per_launch=bytes_alg, samples_per_launch=samples,
                 launch_ms=res["f32"]["bp_ms"], valu_frac=(2.0 * samples / bp_s) / 157.3e12,
                 staged_row_width=width, fft_path=bool(fft_path))
-    return res, roof, cpu, check
+    return res, roof, cpu
 
 
 def em_bench(args, torch, dv, ph):
@@ -173,10 +179,27 @@ def em_bench(args, torch, dv, ph):
     r = dict(config="16 x 1024^2, 1024 angles x 1024 bins, I0=1e4, uniform init", iterations=K, seconds=dt,
              ms_per_iteration=1e3 * dt / K, slice_iterations_per_s=S * K / dt,
              samples_per_s=samples_it * K / dt, kl_first=float(kl[:, 0].sum()), kl_last=float(kl[:, -1].sum()),
-             kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms,
-             learned_init=None)
+             kl_min=float(kl.min(axis=1).sum()), kl_argmin_median=float(np.median(kl.argmin(axis=1))),
+             kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms)
     log(f"EM plain: {K} iterations in {dt:.2f} s ({r['ms_per_iteration']:.2f} ms/it), KL {r['kl_first']:.4e} -> "
         f"{r['kl_last']:.4e}; projector {proj_ms:.3f} ms, backprojection+update {embp_ms:.3f} ms")
+    return r
+
+
+def learned_bench(args):
+    """Time-to-KL of the U-Net warm start vs plain eEM (sscRaft.learned.evaluate)."""
+    from sscRaft import learned
+
+    path = learned.default_weights_path()
+    if not os.path.exists(path):
+        log(f"learned warm start: no trained weights at {path}")
+        return {"status": "no trained weights", "weights": os.path.relpath(path, ROOT)}
+    from sscRaft.nn import UNet
+
+    net = UNet.load(path)
+    r = learned.evaluate(net, count=args.learned_slices, log=log)
+    r["weights"] = os.path.relpath(path, ROOT)
+    r["config"] = "held-out 512^2 slices, 128 angles x 512 bins, I0=1e4; target = lowest KL of plain eEM in 200 iterations"
     return r
 
 
@@ -185,7 +208,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--check-slices", type=int, default=2)
+    ap.add_argument("--learned-slices", type=int, default=512)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-em", action="store_true")
@@ -197,15 +221,13 @@ def main():
                          "that rocprofv3 FETCH_SIZE / WRITE_SIZE can be scaled")
     args = ap.parse_args()
     if args.only == "em":
-        args.no_cpu, args.check_slices = True, 0
+        args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the slab sharding over several GPUs is "
               f"not exercised by this benchmark.", file=sys.stderr)
         sys.exit(2)
     import torch
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the checker and the CPU baseline only
     from sscRaft import device as dv
     from sscRaft import phantom as ph
 
@@ -214,10 +236,14 @@ def main():
         sys.exit(1)
     if args.only == "em":
         em = em_bench(args, torch, dv, ph)
+        if not args.no_learned:
+            em["learned_init"] = learned_bench(args)
         print(json.dumps({"metric": METRIC, "em": em}), flush=True)
         return
-    res, roof, cpu, check = fbp_bench(args, torch, dv, ph, oracle)
+    res, roof, cpu = fbp_bench(args, torch, dv, ph)
     em = None if (args.no_em or args.only == "fbp") else em_bench(args, torch, dv, ph)
+    if em is not None and not args.no_learned:
+        em["learned_init"] = learned_bench(args)
     r32 = res["f32"]
     line = {
         "metric": METRIC,
@@ -240,7 +266,6 @@ def main():
         "fp16_rows": {"value": res["f16"]["slices_per_s"], "unit": "slices/s", "bp_ms": res["f16"]["bp_ms"],
                       "filter_ms": res["f16"]["filter_ms"]},
         "filter_ms": r32["filter_ms"],
-        "check": check,
         "em": em,
     }
     print(json.dumps(line), flush=True)

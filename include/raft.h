@@ -103,6 +103,24 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* Filter-only export (replaces the reference's getFilterLowPassMultiGPU,
+ * cuda/src/processing/filters/low_pass.cu:87-141, bound at rafttypes.py:190-195,
+ * called from sscRaft/processing/filters.py:84): in place on host
+ * tomogram[nslices][nangles][nrays]; paramf/parami as getFBPMultiGPU; the
+ * axis offset paramf[6] is ignored (the reference filters with offset 0). */
+void getFilterLowPassMultiGPU(int* gpus, int ngpus, float* tomogram, float* paramf, int* parami);
+
+/* Same on one device, in place on d_tomo (stream-ordered, synchronises before returning). */
+int raft_lowpass_device(int device, float* d_tomo, const float* paramf, const int* parami, void* stream);
+
+/* KL(y || A x) per slice of the eEM objective for the given images (one
+ * projection with the eEM projector; d_obj is not modified): kl/kl_excluded
+ * are HOST arrays [nslices]; iterations in parami are ignored.  No
+ * counterpart in the reference; used to compare candidate eEM starting images.
+ * Synchronises the stream before returning. */
+int raft_eem_kl_device(int device, const float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                       double* kl, long long* kl_excluded, void* stream);
+
 /* Host-only planning query (no device work): the bin window the FBP
  * backprojector stages per angle row for these parameters -- first bin *tlo
  * and row width *width -- and whether the filter runs on the LDS FFT path

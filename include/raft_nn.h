@@ -30,8 +30,10 @@ int raft_nn_mfma_probe(const unsigned short* A16, const unsigned short* B16, flo
 
 /* y = act(b + conv3x3(x, w)), zero padding; Cin, Cout multiples of 32;
  * ldx, cx0 multiples of 8; ldy, cy0 multiples of 4; relu 0/1; b may be null.
- * variant 0 = automatic, 1..5 select a tile shape (rows x output channels per
- * block: 8x64, 16x32, 4x32, 16x64, 8x32). */
+ * variant 0 = automatic; 1..5 select a one-tile-per-block shape (rows x output
+ * channels per block: 8x64, 16x32, 4x32, 16x64, 8x32), 6..9 the persistent
+ * form that prefetches the next tile while computing (16x32, 8x64, 16x64,
+ * 8x32).  Any H, W >= 1 (partial tiles are masked). */
 int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b,
                     unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
                     void* stream);
@@ -49,6 +51,9 @@ long long raft_nn_wgrad_workspace(int N, int H, int W, int Cin, int Cout);
 int raft_nn_conv3x3_wgrad(const unsigned short* x, int ldx, int cx0, const unsigned short* g, int ldg, int cg0,
                           float* dw, float* db, float* work, long long work_floats, int N, int H, int W, int Cin,
                           int Cout, int accumulate, void* stream);
+
+/* The elementwise entry points below move 8 channels at a time: C, every
+ * channel stride and every channel offset must be multiples of 8. */
 
 /* out = g * (act > 0) over P pixels x C channels (may run in place). */
 int raft_nn_relu_mask(const unsigned short* g, int ldg, int cg0, const unsigned short* act, int lda, int ca0,

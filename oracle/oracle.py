@@ -113,6 +113,22 @@ def fbp_filter(tomo, paramf, parami, prec="f64", nthreads=None):
     return out
 
 
+def lowpass(tomo, paramf, parami, prec="f64", nthreads=None):
+    """getFilterLowPassMultiGPU restated (low_pass.cu:9-30, filters.cu:120-161):
+    the FBP filter stage with the axis offset forced to 0, cropped back to the
+    N original bins [L, L + N); filter 'none' returns the data unchanged."""
+    tomo = np.ascontiguousarray(tomo, dtype=np.float32)
+    if tomo.ndim == 2:
+        tomo = tomo[None]
+    pf = np.array(paramf, dtype=np.float32).copy()
+    pf[6] = 0.0
+    if int(parami[7]) == 0:
+        return tomo.astype(_dt(prec))
+    n, P = int(parami[0]), int(parami[4])
+    L = (P * n) // 2
+    return np.ascontiguousarray(fbp_filter(tomo, pf, parami, prec, nthreads)[:, :, L:L + n])
+
+
 def radon(phantom, angles, nrays=None, ax=1.0, ay=1.0, prec="f64", nthreads=None):
     """getRadonRTMultiGPU on a [S, Ny, Nx] phantom; no Python-side rescale."""
     phantom = np.ascontiguousarray(phantom, dtype=np.float32)

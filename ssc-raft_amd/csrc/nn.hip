@@ -19,6 +19,7 @@
 #include "raft_internal.hpp"
 #include "../../include/raft_nn.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace raft {
@@ -190,6 +191,183 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
                 pk.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
                 *reinterpret_cast<uint2*>(yp + co) = pk;
             }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution, persistent form (same arithmetic and layouts as k_conv3x3).
+// A block walks `tiles_per_block` consecutive tiles of one output-channel
+// tile and, for each, the Cin/32 input chunks.  While one (tile, chunk) item is
+// computed from LDS, the next item's halo (and its weights when the chunk
+// changes; with a single chunk they stay resident) is loaded into registers
+// and written to LDS after a barrier, so global latency overlaps the MFMAs.
+// The tap loop runs dx outer: the WR + 2 pixel fragments of one halo column
+// serve all three dy taps (3 WR MFMAs per WR + 2 LDS fragment reads).
+// ---------------------------------------------------------------------------
+template <int WR, int WC>
+struct ConvP {
+    static constexpr int TH = 4 * WR, HW = 34, HH = TH + 2, CO_T = 32 * WC;
+    static constexpr int HALO_UNITS = HH * HW * 4; // 16-byte units
+    static constexpr int W_UNITS = 9 * CO_T * 4;
+    static constexpr int HU = (HALO_UNITS + 255) / 256;
+    static constexpr int WU = (W_UNITS + 255) / 256;
+    static constexpr int LDS = (HH * HW + 9 * CO_T) * CONV_PS * 2;
+};
+
+struct ConvPArgs {
+    ConvArgs c;
+    int tiles_x, tiles_y, total_tiles, tiles_per_block;
+};
+
+template <int WR, int WC>
+__global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
+{
+    using K = ConvP<WR, WC>;
+    const ConvArgs& a = pa.c;
+    extern __shared__ uint4 convp_lds[];
+    u16* s_in = reinterpret_cast<u16*>(convp_lds);
+    u16* s_w = s_in + K::HH * K::HW * CONV_PS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int co0 = blockIdx.y * K::CO_T;
+    const int nch = a.Cin / 32;
+    const int t0 = blockIdx.x * pa.tiles_per_block;
+    const int t1 = min(t0 + pa.tiles_per_block, pa.total_tiles);
+    const int items = (t1 - t0) * nch;
+
+    float bias[WC][16];
+#pragma unroll
+    for (int c = 0; c < WC; ++c)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int co = co0 + 32 * c + 8 * (i / 4) + 4 * hi + (i % 4);
+            bias[c][i] = a.b ? a.b[co] : 0.0f;
+        }
+
+    uint4 hreg[K::HU], wreg[K::WU];
+    auto load_item = [&](int it, bool weights) {
+        const int t = t0 + it / nch, ch = it % nch;
+        const int tx0 = (t % pa.tiles_x) * 32;
+        const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
+        const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+#pragma unroll
+        for (int k = 0; k < K::HU; ++k) {
+            const int u = tid + 256 * k;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (u < K::HALO_UNITS) {
+                const int pix = u >> 2, q = u & 3;
+                const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                    v = *reinterpret_cast<const uint4*>(a.x + (img + (size_t)gy * a.W + gx) * a.ldx + a.cx0 + 32 * ch +
+                                                        8 * q);
+            }
+            hreg[k] = v;
+        }
+        if (weights) {
+#pragma unroll
+            for (int k = 0; k < K::WU; ++k) {
+                const int u = tid + 256 * k;
+                if (u < K::W_UNITS) {
+                    const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
+                    wreg[k] = *reinterpret_cast<const uint4*>(a.w + ((size_t)(co0 + co) * 9 + tap) * a.Cin + 32 * ch +
+                                                              8 * q);
+                }
+            }
+        }
+    };
+    auto store_item = [&](bool weights) {
+#pragma unroll
+        for (int k = 0; k < K::HU; ++k) {
+            const int u = tid + 256 * k;
+            if (u < K::HALO_UNITS) *reinterpret_cast<uint4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
+        }
+        if (weights) {
+#pragma unroll
+            for (int k = 0; k < K::WU; ++k) {
+                const int u = tid + 256 * k;
+                if (u < K::W_UNITS) {
+                    const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
+                    *reinterpret_cast<uint4*>(s_w + (tap * K::CO_T + co) * CONV_PS + 8 * q) = wreg[k];
+                }
+            }
+        }
+    };
+
+    f32x16 acc[WR][WC];
+#pragma unroll
+    for (int r = 0; r < WR; ++r)
+#pragma unroll
+        for (int c = 0; c < WC; ++c)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
+
+    if (items > 0) load_item(0, true);
+    for (int it = 0; it < items; ++it) {
+        const int ch = it % nch;
+        __syncthreads();
+        store_item(it == 0 || nch > 1);
+        __syncthreads();
+        if (it + 1 < items) load_item(it + 1, nch > 1);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+#pragma unroll
+            for (int kk = 0; kk < 32; kk += 16) {
+                bf16x8 bp[WR + 2];
+#pragma unroll
+                for (int rr = 0; rr < WR + 2; ++rr)
+                    bp[rr] = lds8(s_in + ((wave * WR + rr) * K::HW + l32 + dx) * CONV_PS + kk + 8 * hi);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    bf16x8 aw[WC];
+#pragma unroll
+                    for (int c = 0; c < WC; ++c)
+                        aw[c] = lds8(s_w + ((3 * dy + dx) * K::CO_T + 32 * c + l32) * CONV_PS + kk + 8 * hi);
+#pragma unroll
+                    for (int r = 0; r < WR; ++r)
+#pragma unroll
+                        for (int c = 0; c < WC; ++c)
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aw[c], bp[r + dy], acc[r][c], 0, 0, 0);
+                }
+            }
+        }
+        if (ch == nch - 1) {
+            const int t = t0 + it / nch;
+            const int tx0 = (t % pa.tiles_x) * 32;
+            const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
+            const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+            if (tx0 + l32 < a.W) {
+#pragma unroll
+                for (int r = 0; r < WR; ++r) {
+                    if (ty0 + wave * WR + r < a.H) {
+                        const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
+                        u16* yp = a.y + p * a.ldy + a.cy0;
+#pragma unroll
+                        for (int c = 0; c < WC; ++c)
+#pragma unroll
+                            for (int g = 0; g < 4; ++g) {
+                                const int co = co0 + 32 * c + 8 * g + 4 * hi;
+                                u16 o[4];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    float v = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                                    if (a.relu) v = fmaxf(v, 0.0f);
+                                    o[i] = bf16_bits(v);
+                                }
+                                uint2 pk;
+                                pk.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
+                                pk.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+                                *reinterpret_cast<uint2*>(yp + co) = pk;
+                            }
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < WR; ++r)
+#pragma unroll
+                for (int c = 0; c < WC; ++c)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
+        }
     }
 }
 
@@ -366,34 +544,72 @@ struct CView {
     int ld, c0;
 };
 
+// The elementwise kernels below move 8 channels (16 bytes) per thread; every
+// channel count, stride and offset they see is a multiple of 8.
+__device__ __forceinline__ void unpack8(uint4 v, float f[8])
+{
+    const unsigned e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        f[2 * j] = __uint_as_float(e[j] << 16);
+        f[2 * j + 1] = __uint_as_float(e[j] & 0xffff0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8])
+{
+    unsigned e[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) e[j] = (unsigned)bf16_bits(f[2 * j]) | ((unsigned)bf16_bits(f[2 * j + 1]) << 16);
+    return make_uint4(e[0], e[1], e[2], e[3]);
+}
+__device__ __forceinline__ uint4 ld8(const u16* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ __forceinline__ void st8(u16* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+
 // out = g * (act > 0)
 __global__ void k_relu_mask(CView g, CView act, View out, long long P, int C)
 {
-    const long long total = P * C;
+    const int C8 = C / 8;
+    const long long total = P * C8;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const long long p = e / C;
-        const int c = (int)(e % C);
-        const u16 gv = g.p[p * g.ld + g.c0 + c];
-        out.p[p * out.ld + out.c0 + c] = bf16_float(act.p[p * act.ld + act.c0 + c]) > 0.0f ? gv : (u16)0;
+        const long long p = e / C8;
+        const int c = 8 * (int)(e % C8);
+        const uint4 gv = ld8(g.p + p * g.ld + g.c0 + c), av = ld8(act.p + p * act.ld + act.c0 + c);
+        const unsigned ge[4] = {gv.x, gv.y, gv.z, gv.w}, ae[4] = {av.x, av.y, av.z, av.w};
+        unsigned o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned lo = __uint_as_float(ae[j] << 16) > 0.0f ? (ge[j] & 0xffffu) : 0u;
+            const unsigned hv = __uint_as_float(ae[j] & 0xffff0000u) > 0.0f ? (ge[j] & 0xffff0000u) : 0u;
+            o[j] = lo | hv;
+        }
+        st8(out.p + p * out.ld + out.c0 + c, make_uint4(o[0], o[1], o[2], o[3]));
     }
 }
 
 // 2x2 average pool: y (Ho x Wo) from x (2Ho x 2Wo)
 __global__ void k_pool2(CView x, View y, int N, int Ho, int Wo, int C)
 {
-    const long long total = (long long)N * Ho * Wo * C;
+    const int C8 = C / 8;
+    const long long total = (long long)N * Ho * Wo * C8;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(e % C);
-        const long long q = e / C;
+        const int c = 8 * (int)(e % C8);
+        const long long q = e / C8;
         const int w = (int)(q % Wo), h = (int)((q / Wo) % Ho);
         const long long n = q / ((long long)Wo * Ho);
         const long long base = (n * 2 * Ho + 2 * h) * 2 * Wo + 2 * w;
-        const float s = bf16_float(x.p[base * x.ld + x.c0 + c]) + bf16_float(x.p[(base + 1) * x.ld + x.c0 + c]) +
-                        bf16_float(x.p[(base + 2 * Wo) * x.ld + x.c0 + c]) +
-                        bf16_float(x.p[(base + 2 * Wo + 1) * x.ld + x.c0 + c]);
-        y.p[q * y.ld + y.c0 + c] = bf16_bits(0.25f * s);
+        const long long src[4] = {base, base + 1, base + 2 * Wo, base + 2 * Wo + 1};
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unpack8(ld8(x.p + src[k] * x.ld + x.c0 + c), f);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += f[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] *= 0.25f;
+        st8(y.p + q * y.ld + y.c0 + c, pack8(acc));
     }
 }
 
@@ -401,41 +617,52 @@ __global__ void k_pool2(CView x, View y, int N, int Ho, int Wo, int C)
 //   out = (gcat + pool_backward(gpool)) * (s > 0),  pool_backward(g)[2h+a][2w+b] = g[h][w] / 4
 __global__ void k_skip_grad(CView gcat, CView gpool, CView act, View out, int N, int H, int W, int C)
 {
-    const long long total = (long long)N * H * W * C;
+    const int C8 = C / 8;
+    const long long total = (long long)N * H * W * C8;
     const int Ho = H / 2, Wo = W / 2;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(e % C);
-        const long long p = e / C;
+        const int c = 8 * (int)(e % C8);
+        const long long p = e / C8;
         const int w = (int)(p % W), h = (int)((p / W) % H);
         const long long n = p / ((long long)W * H);
         const long long pp = (n * Ho + h / 2) * Wo + w / 2;
-        float v = bf16_float(gcat.p[p * gcat.ld + gcat.c0 + c]) +
-                  0.25f * bf16_float(gpool.p[pp * gpool.ld + gpool.c0 + c]);
-        if (!(bf16_float(act.p[p * act.ld + act.c0 + c]) > 0.0f)) v = 0.0f;
-        out.p[p * out.ld + out.c0 + c] = bf16_bits(v);
+        float a8[8], g8[8], q8[8];
+        unpack8(ld8(act.p + p * act.ld + act.c0 + c), a8);
+        unpack8(ld8(gcat.p + p * gcat.ld + gcat.c0 + c), g8);
+        unpack8(ld8(gpool.p + pp * gpool.ld + gpool.c0 + c), q8);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) g8[j] = a8[j] > 0.0f ? g8[j] + 0.25f * q8[j] : 0.0f;
+        st8(out.p + p * out.ld + out.c0 + c, pack8(g8));
     }
 }
 
 // nearest 2x upsampling (Hl x Wl -> 2Hl x 2Wl) and its backward (sum of the 4)
 __global__ void k_up2(CView x, View y, int N, int Hl, int Wl, int C, int backward)
 {
-    const long long total = (long long)N * Hl * Wl * C;
+    const int C8 = C / 8;
+    const long long total = (long long)N * Hl * Wl * C8;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(e % C);
-        const long long q = e / C;
+        const int c = 8 * (int)(e % C8);
+        const long long q = e / C8;
         const int w = (int)(q % Wl), h = (int)((q / Wl) % Hl);
         const long long n = q / ((long long)Wl * Hl);
         const long long base = (n * 2 * Hl + 2 * h) * 2 * Wl + 2 * w;
         const long long hp[4] = {base, base + 1, base + 2 * Wl, base + 2 * Wl + 1};
         if (!backward) {
-            const u16 v = x.p[q * x.ld + x.c0 + c];
-            for (int k = 0; k < 4; ++k) y.p[hp[k] * y.ld + y.c0 + c] = v;
+            const uint4 v = ld8(x.p + q * x.ld + x.c0 + c);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) st8(y.p + hp[k] * y.ld + y.c0 + c, v);
         } else {
-            float s = 0.0f;
-            for (int k = 0; k < 4; ++k) s += bf16_float(x.p[hp[k] * x.ld + x.c0 + c]);
-            y.p[q * y.ld + y.c0 + c] = bf16_bits(s);
+            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                unpack8(ld8(x.p + hp[k] * x.ld + x.c0 + c), f);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[j] += f[j];
+            }
+            st8(y.p + q * y.ld + y.c0 + c, pack8(acc));
         }
     }
 }
@@ -552,21 +779,28 @@ __global__ __launch_bounds__(256) void k_head(HeadArgs a)
     if (threadIdx.x == 0) a.loss[blockIdx.x + 1] = s_loss[0] + s_loss[1] + s_loss[2] + s_loss[3];
 }
 
-// dw, db from the head partials; loss[0] = sum of the per-block losses (fixed order)
-__global__ void k_head_reduce(const float* part, const int nparts, float* dw, float* db, double* loss, int accumulate)
+// dw, db from the head partials (block k < 33 reduces column k) and loss[0]
+// = sum of the per-block losses (block 33); fixed order, so deterministic.
+__global__ __launch_bounds__(256) void k_head_reduce(const float* part, const int nparts, float* dw, float* db,
+                                                     double* loss, int accumulate)
 {
-    const int k = threadIdx.x;
-    if (k < 33) {
-        float s = 0.0f;
-        for (int b = 0; b < nparts; ++b) s += part[b * 33 + k];
-        float* d = k < 32 ? dw + k : db;
-        *d = accumulate ? *d + s : s;
+    __shared__ double sd[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += 256) s += k < 33 ? (double)part[b * 33 + k] : loss[b + 1];
+    sd[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sd[threadIdx.x] += sd[threadIdx.x + w];
+        __syncthreads();
     }
-    if (k == 63) {
-        double s = 0.0;
-        for (int b = 0; b < nparts; ++b) s += loss[b + 1];
-        loss[0] = s;
+    if (threadIdx.x != 0) return;
+    if (k == 33) {
+        loss[0] = sd[0];
+        return;
     }
+    float* d = k < 32 ? dw + k : db;
+    *d = accumulate ? *d + (float)sd[0] : (float)sd[0];
 }
 
 // Adam (bias-corrected) on FP32 master weights; refreshes the bf16 copy
@@ -642,6 +876,32 @@ int launch_conv(const ConvArgs& a, hipStream_t st)
     return launched();
 }
 
+template <int WR, int WC>
+int launch_convp(const ConvArgs& a, hipStream_t st)
+{
+    using K = ConvP<WR, WC>;
+    static bool attr = false;
+    if (!attr) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, K::LDS) != hipSuccess)
+            return RAFT_EHIP;
+        attr = true;
+    }
+    ConvPArgs pa;
+    pa.c = a;
+    pa.tiles_x = (a.W + 31) / 32;
+    pa.tiles_y = (a.H + K::TH - 1) / K::TH;
+    const long long total = (long long)a.N * pa.tiles_y * pa.tiles_x;
+    if (total > (1LL << 30)) return RAFT_EINVAL;
+    pa.total_tiles = (int)total;
+    const int ny = a.Cout / K::CO_T;
+    long long per = (total * ny + 1023) / 1024;
+    pa.tiles_per_block = (int)std::max(1LL, std::min(per, 16LL));
+    const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
+    hipLaunchKernelGGL((k_conv3x3p<WR, WC>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
+    return launched();
+}
+
 struct WgradPlan {
     int nblk, per, tiles, pairs;
 };
@@ -651,7 +911,7 @@ bool wgrad_plan(int N, int H, int W, int Cin, int Cout, WgradPlan& p)
     if (N < 1 || W < 1 || H < 1 || Cin % 32 || Cout % 32 || Cin < 32 || Cout < 32) return false;
     p.tiles = N * ((H + WG_TH - 1) / WG_TH) * ((W + 31) / 32);
     p.pairs = (Cout / 32) * (Cin / 32);
-    int nblk = (1024 + p.pairs - 1) / p.pairs;
+    int nblk = (512 + p.pairs - 1) / p.pairs;
     if (nblk > p.tiles) nblk = p.tiles;
     p.per = (p.tiles + nblk - 1) / nblk;
     p.nblk = (p.tiles + p.per - 1) / p.per;
@@ -680,8 +940,20 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
         ldy % 4 || cy0 % 4 || cx0 + Cin > ldx || cy0 + Cout > ldy)
         return RAFT_EINVAL;
     ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu};
-    if (variant == 0) variant = Cout % 64 == 0 ? 1 : 2;
+    // measured on gfx950: the persistent (2,1) form wins when one 32-channel chunk
+    // is the whole input; with more chunks the one-tile (2,2) / (4,1) forms do
+    if (variant == 0) variant = Cin == 32 ? 9 : (Cout % 64 == 0 ? 1 : 2);
     switch (variant) {
+    case 6:
+        return launch_convp<4, 1>(a, S(stream));
+    case 7:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_convp<2, 2>(a, S(stream));
+    case 8:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_convp<4, 2>(a, S(stream));
+    case 9:
+        return launch_convp<2, 1>(a, S(stream));
     case 1:
         if (Cout % 64) return RAFT_EINVAL;
         return launch_conv<2, 2>(a, S(stream));
@@ -740,10 +1012,13 @@ int raft_nn_conv3x3_wgrad(const unsigned short* x, int ldx, int cx0, const unsig
     return launched();
 }
 
+static bool al8(int a, int b, int c, int d) { return a % 8 == 0 && b % 8 == 0 && c % 8 == 0 && d % 8 == 0; }
+
 int raft_nn_relu_mask(const unsigned short* g, int ldg, int cg0, const unsigned short* act, int lda, int ca0,
                       unsigned short* out, int ldo, int co0, long long P, int C, void* stream)
 {
-    hipLaunchKernelGGL(k_relu_mask, dim3(grid_for(P * C)), dim3(256), 0, S(stream), CView{g, ldg, cg0},
+    if (C < 8 || !al8(C, ldg, cg0, lda) || !al8(ca0, ldo, co0, 0)) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_relu_mask, dim3(grid_for(P * C / 8)), dim3(256), 0, S(stream), CView{g, ldg, cg0},
                        CView{act, lda, ca0}, View{out, ldo, co0}, P, C);
     return launched();
 }
@@ -751,7 +1026,8 @@ int raft_nn_relu_mask(const unsigned short* g, int ldg, int cg0, const unsigned 
 int raft_nn_pool2(const unsigned short* x, int ldx, int cx0, unsigned short* y, int ldy, int cy0, int N, int Ho,
                   int Wo, int C, void* stream)
 {
-    hipLaunchKernelGGL(k_pool2, dim3(grid_for((long long)N * Ho * Wo * C)), dim3(256), 0, S(stream),
+    if (C < 8 || !al8(C, ldx, cx0, ldy) || cy0 % 8) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_pool2, dim3(grid_for((long long)N * Ho * Wo * C / 8)), dim3(256), 0, S(stream),
                        CView{x, ldx, cx0}, View{y, ldy, cy0}, N, Ho, Wo, C);
     return launched();
 }
@@ -760,8 +1036,8 @@ int raft_nn_skip_grad(const unsigned short* gcat, int ldc, int cc0, const unsign
                       const unsigned short* act, int lda, int ca0, unsigned short* out, int ldo, int co0, int N, int H,
                       int W, int C, void* stream)
 {
-    if (H % 2 || W % 2) return RAFT_EINVAL;
-    hipLaunchKernelGGL(k_skip_grad, dim3(grid_for((long long)N * H * W * C)), dim3(256), 0, S(stream),
+    if (H % 2 || W % 2 || C < 8 || !al8(C, ldc, cc0, ldp) || !al8(cp0, lda, ca0, ldo) || co0 % 8) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_skip_grad, dim3(grid_for((long long)N * H * W * C / 8)), dim3(256), 0, S(stream),
                        CView{gcat, ldc, cc0}, CView{gpool, ldp, cp0}, CView{act, lda, ca0}, View{out, ldo, co0}, N, H,
                        W, C);
     return launched();
@@ -770,7 +1046,8 @@ int raft_nn_skip_grad(const unsigned short* gcat, int ldc, int cc0, const unsign
 int raft_nn_up2(const unsigned short* x, int ldx, int cx0, unsigned short* y, int ldy, int cy0, int N, int Hl, int Wl,
                 int C, int backward, void* stream)
 {
-    hipLaunchKernelGGL(k_up2, dim3(grid_for((long long)N * Hl * Wl * C)), dim3(256), 0, S(stream), CView{x, ldx, cx0},
+    if (C < 8 || !al8(C, ldx, cx0, ldy) || cy0 % 8) return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_up2, dim3(grid_for((long long)N * Hl * Wl * C / 8)), dim3(256), 0, S(stream), CView{x, ldx, cx0},
                        View{y, ldy, cy0}, N, Hl, Wl, C, backward);
     return launched();
 }
@@ -798,7 +1075,7 @@ int raft_nn_head(const unsigned short* h, int ldh, int ch0, const float* w, cons
     hipLaunchKernelGGL(k_head, dim3(nb), dim3(256), 0, S(stream), a);
     if (hipGetLastError() != hipSuccess) return RAFT_EHIP;
     if (!target) return RAFT_OK;
-    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, S(stream), work, nb, dw, db, loss, accumulate);
+    hipLaunchKernelGGL(k_head_reduce, dim3(34), dim3(256), 0, S(stream), work, nb, dw, db, loss, accumulate);
     return launched();
 }
 

@@ -459,6 +459,53 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
 // ---------------------------------------------------------------------------
 // Radon on device pointers
 // ---------------------------------------------------------------------------
+// Filter-only export (low_pass.cu:9-141 / filters.cu:120-161 filterFBPpad):
+// edge-replicate pad, the FBP filter with no axis offset, crop back to the N
+// original bins, / (npad * pixel); in place.  Filter 'none' leaves the data as is.
+static void lowpass_device(int dev, float* d_tomo, const float* paramf, const int* parami, hipStream_t st)
+{
+    if (!paramf) fail(RAFT_EINVAL, "null parameter array");
+    float pf[7];
+    std::memcpy(pf, paramf, sizeof(pf));
+    pf[6] = 0.0f; // getFilterLowPass builds its Filter with axis_offset = 0
+    const FbpCfg c = decode_fbp(pf, parami);
+    if (c.S == 0 || c.filter == 0) return;
+    if (!d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+    const int A = c.A, N = c.N;
+    FilterTables& F = filter_tables(D, c, st);
+    std::vector<float> ones(A, 1.0f);
+    DevBuf dwa;
+    upload(dwa, ones, st);
+    const int SB = BP_SLICES_F32;
+    const size_t per_group = (size_t)A * N * 16;
+    int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
+    B = std::min(c.S, ((B + SB - 1) / SB) * SB);
+    const size_t budget = block_budget();
+    while (B > SB && (size_t)((B + SB - 1) / SB) * per_group > budget) B = (((B + 1) / 2 + SB - 1) / SB) * SB;
+    float* q = static_cast<float*>(D.workspace(0, (size_t)((B + SB - 1) / SB) * per_group));
+    for (int s0 = 0; s0 < c.S; s0 += B) {
+        const int nb = std::min(B, c.S - s0);
+        FilterArgs fa;
+        fa.tomo = d_tomo + (size_t)s0 * A * N;
+        fa.q = q;
+        fa.H = F.H.as<float2>();
+        fa.tw = F.tw.as<float2>();
+        fa.g = F.g.as<float>();
+        fa.wa = dwa.as<float>();
+        fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = c.L; fa.W = N;
+        fa.none = 0;
+        {
+            ProfScope ps(KF_FILTER, st);
+            launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
+        }
+        launch_from_interleaved(q, d_tomo + (size_t)s0 * A * N, nb, A, N, st);
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
 static void radon_device(int dev, float* d_proj, const float* d_obj, const float* angles, int Nx, int Ny, int nrays,
                          int nangles, int nslices, float ax, float ay, hipStream_t st)
 {
@@ -507,11 +554,13 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
 // ---------------------------------------------------------------------------
 // eEM on device pointers
 // ---------------------------------------------------------------------------
+// kl_only: KL(y || A x) of the given images (one projection, x unchanged), kl[S]
 static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
-                       double* kl, long long* kl_ex, hipStream_t st)
+                       double* kl, long long* kl_ex, hipStream_t st, bool kl_only = false)
 {
     if (!parami) fail(RAFT_EINVAL, "null parameter array");
-    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = parami[8];
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = kl_only ? 1 : parami[8];
+    if (kl_only && !kl && !kl_ex) fail(RAFT_EINVAL, "no KL output");
     int blocksize = parami[9];
     if (R < 2) fail(RAFT_EINVAL, "nrays must be >= 2");
     if (A < 2) fail(RAFT_EINVAL, "nangles must be >= 2");
@@ -586,8 +635,8 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     }
 
     // backones = B(1): the same for every slice
-    launch_fill_rows(onerows, dwa.as<float>(), A, g.width, g.tlo, R, st);
-    {
+    if (!kl_only) {
+        launch_fill_rows(onerows, dwa.as<float>(), A, g.width, g.tlo, R, st);
         BpArgs b1;
         b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
         b1.cA = dcA.as<double>(); b1.cB = dcB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
@@ -612,6 +661,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
             launch_project(p, 1, s);
         }
         if (want_kl) launch_kl_reduce(part, expart, A, nb, klo, exo, K, it, graph ? counter : nullptr, s);
+        if (kl_only) return;
         BpArgs b;
         b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
         b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
@@ -626,7 +676,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // (RAFT_EM_GRAPH=0 disables it).  The graph holds kernels only: the counter it
     // advances is cleared by a kernel outside it.
     const char* genv = std::getenv("RAFT_EM_GRAPH");
-    const bool use_graph = K >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
+    const bool use_graph = !kl_only && K >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
 
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
@@ -664,7 +714,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         } else {
             for (int it = 0; it < K; ++it) iteration(it, nb, y, klo, exo, false, st);
         }
-        launch_from_interleaved(xil, x, nb, n, n, st);
+        if (!kl_only) launch_from_interleaved(xil, x, nb, n, n, st);
     }
     if (want_kl && K > 0) {
         std::vector<double> hk((size_t)S * K);
@@ -834,6 +884,49 @@ using namespace raft;
 
 extern "C" {
 
+static void lowpass_host(int* gpus, int ngpus, float* tomo, float* paramf, int* parami)
+{
+    if (!paramf) fail(RAFT_EINVAL, "null parameter array");
+    float pf[7];
+    std::memcpy(pf, paramf, sizeof(pf));
+    pf[6] = 0.0f;
+    const FbpCfg c = decode_fbp(pf, parami);
+    if (!tomo) fail(RAFT_EINVAL, "null data pointer");
+    check_gpus(gpus, ngpus);
+    if (c.filter == 0) return;
+    run_slabs(gpus, ngpus, c.S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t tin = (size_t)c.A * c.N;
+        int B = c.blocksize > 0 ? std::min(c.blocksize, cnt) : cnt;
+        const size_t need = tin * 4 * 2;
+        const size_t budget = block_budget();
+        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
+        DevBuf dt((size_t)B * tin * 4);
+        std::vector<int> pi(parami, parami + 10);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            float* h = tomo + (size_t)(first + s0) * tin;
+            RAFT_HIP(hipMemcpyAsync(dt.p, h, (size_t)nb * tin * 4, hipMemcpyHostToDevice, st.s));
+            pi[2] = nb;
+            pi[9] = 0;
+            lowpass_device(dev, dt.as<float>(), pf, pi.data(), st.s);
+            RAFT_HIP(hipMemcpyAsync(h, dt.p, (size_t)nb * tin * 4, hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+void getFilterLowPassMultiGPU(int* gpus, int ngpus, float* tomogram, float* paramf, int* parami)
+{
+    guarded([&] { lowpass_host(gpus, ngpus, tomogram, paramf, parami); });
+}
+
+int raft_lowpass_device(int device, float* d_tomo, const float* paramf, const int* parami, void* stream)
+{
+    return guarded([&] { lowpass_device(device, d_tomo, paramf, parami, static_cast<hipStream_t>(stream)); });
+}
+
 void getFBPMultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf, int* parami)
 {
     guarded([&] { fbp_host(gpus, ngpus, obj, tomogram, angles, paramf, parami); });
@@ -891,6 +984,15 @@ int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* 
 {
     return guarded([&] {
         eem_device(device, d_obj, d_tomo, angles, parami, kl, kl_excluded, static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_eem_kl_device(int device, const float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                       double* kl, long long* kl_excluded, void* stream)
+{
+    return guarded([&] {
+        eem_device(device, const_cast<float*>(d_obj), d_tomo, angles, parami, kl, kl_excluded,
+                   static_cast<hipStream_t>(stream), true);
     });
 }
 

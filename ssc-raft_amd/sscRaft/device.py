@@ -93,6 +93,37 @@ def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None):
     return obj, None, None
 
 
+def lowpass_device(tomo, paramf, parami, stream=None):
+    """Filter-only export on device data, in place on tomo [S, A, N]; returns tomo."""
+    _check_tensor(tomo, "tomo")
+    paramf = CNICE(numpy.asarray(paramf), numpy.float32)
+    parami = CNICE(numpy.asarray(parami), numpy.int32)
+    rc = libraft.raft_lowpass_device(tomo.device.index or 0, ctypes.c_void_p(tomo.data_ptr()),
+                                     paramf.ctypes.data_as(ctypes.c_void_p), parami.ctypes.data_as(ctypes.c_void_p),
+                                     _stream(tomo, stream))
+    if rc:
+        raft_check("raft_lowpass_device")
+    return tomo
+
+
+def eem_kl_device(tomo, angles, obj, stream=None):
+    """KL(y || A x) per slice for images obj [S, R, R] (not modified); returns (kl [S], excluded [S])."""
+    _check_tensor(tomo, "tomo")
+    _check_tensor(obj, "obj")
+    S, A, R = tomo.shape
+    angles = CNICE(numpy.asarray(angles))
+    parami = CNICE(numpy.array([R, A, S, obj.shape[-1], 0, 0, 0, 1, 1, 0]), numpy.int32)
+    kl = numpy.zeros(S, dtype=numpy.float64)
+    ex = numpy.zeros(S, dtype=numpy.int64)
+    rc = libraft.raft_eem_kl_device(tomo.device.index or 0, ctypes.c_void_p(obj.data_ptr()),
+                                    ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
+                                    parami.ctypes.data_as(ctypes.c_void_p), kl.ctypes.data_as(ctypes.c_void_p),
+                                    ex.ctypes.data_as(ctypes.c_void_p), _stream(tomo, stream))
+    if rc:
+        raft_check("raft_eem_kl_device")
+    return kl, ex
+
+
 def set_profiling(on):
     libraft.raft_set_profiling(1 if on else 0)
 

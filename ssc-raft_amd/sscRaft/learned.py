@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import phantom as ph
-from .device import eem_device, fbp_device, radon_device
+from .device import eem_device, eem_kl_device, fbp_device, radon_device
 from .geometries.parallel.fbp_methods import fbp_params
 from .nn import UNet, _stream, nn_call
 
@@ -85,14 +85,25 @@ def fbp_start(p_hat, ang, cfg=WarmStartConfig):
     return fbp_to_em_grid(f)
 
 
+def valid_kl(kl, ex):
+    """KL curve with invalid iterates set to +inf.
+
+    The KL skips bins whose projection is not positive (and counts them); a
+    diverging eEM iterate with NaN pixels projects to NaN bins, which the sum
+    skips, so its KL looks small.  An iterate is valid only while its count of
+    skipped bins has not grown since iteration 0 and its KL is finite."""
+    bad = ~np.isfinite(kl) | (ex > ex[:, :1])
+    return np.where(bad, np.inf, kl)
+
+
 def best_kl_eem(y, ang, iterations, chunk=512):
-    """Plain eEM from ones for `iterations`, keeping each slice's lowest-KL iterate.
+    """Plain eEM from ones for `iterations`, keeping each slice's lowest-KL valid iterate.
 
     Returns (best [S, N, N], best_kl [S], best_iter [S], kl_curve [S, iterations])."""
     import torch
 
     S, A, N = y.shape
-    best = torch.empty((S, N, N), dtype=torch.float32, device=y.device)
+    best = torch.ones((S, N, N), dtype=torch.float32, device=y.device)
     best_kl = np.full(S, np.inf)
     best_it = np.zeros(S, dtype=np.int64)
     curve = np.zeros((S, iterations))
@@ -101,10 +112,14 @@ def best_kl_eem(y, ang, iterations, chunk=512):
         yc = y[c0:c1].contiguous()
         x = torch.ones((c1 - c0, N, N), dtype=torch.float32, device=y.device)
         prev = torch.empty_like(x)
+        ex0 = None
         for it in range(iterations):
             prev.copy_(x)
-            _, kl, _ = eem_device(yc, ang, 1, obj=x)  # kl = KL of prev; x advances one iteration
-            k = kl[:, 0]
+            _, kl, ex = eem_device(yc, ang, 1, obj=x)  # kl = KL of prev; x advances one iteration
+            if ex0 is None:
+                ex0 = ex[:, 0].copy()
+            finite = torch.isfinite(prev).flatten(1).all(dim=1).cpu().numpy()
+            k = np.where(finite & np.isfinite(kl[:, 0]) & (ex[:, 0] <= ex0), kl[:, 0], np.inf)
             curve[c0:c1, it] = k
             better = np.isfinite(k) & (k < best_kl[c0:c1])
             if better.any():
@@ -117,11 +132,15 @@ def best_kl_eem(y, ang, iterations, chunk=512):
 
 def make_pairs(count, first, cfg=WarmStartConfig, log=print):
     """Training / evaluation pairs: (input [count, N, N], target [count, N, N], y, ang, info)."""
+    import torch
+
     t0 = time.time()
     p_hat, y, _, ang = sinograms(count, first, cfg)
     inp = fbp_start(p_hat, ang, cfg)
     del p_hat
     target, best_kl, best_it, _ = best_kl_eem(y, ang, cfg.em_iterations)
+    if not bool(torch.isfinite(target).all()) or not bool(torch.isfinite(inp).all()):
+        raise RuntimeError("non-finite training pair")
     log(f"pairs: {count} slices in {time.time() - t0:.1f} s; lowest-KL iteration median {np.median(best_it):.0f} "
         f"(min {best_it.min()}, max {best_it.max()})")
     return inp, target, y, ang, {"best_kl": best_kl, "best_iter": best_it}
@@ -155,7 +174,9 @@ def train(net, inp, target, epochs, batch=8, lr=1e-3, lr_end=1e-5, seed=0, log=p
 
 def warm_start(net, p_hat, y, ang, cfg=WarmStartConfig, batch=32, check=True):
     """Starting images for eEM: network output (clamped to cfg.floor), with the
-    per-slice acceptance rule against the FBP start.  Returns (x0, accepted mask, fbp start)."""
+    per-slice acceptance rule against the FBP start.
+
+    Returns (x0, accepted mask, fbp start, (KL of the network start, KL of the FBP start))."""
     import torch
 
     fb = fbp_start(p_hat, ang, cfg)
@@ -165,14 +186,14 @@ def warm_start(net, p_hat, y, ang, cfg=WarmStartConfig, batch=32, check=True):
         net.forward(fb[b0:b1], out=x0[b0:b1], floor=cfg.floor)
     xf = torch.clamp(fb, min=cfg.floor)
     if not check:
-        return x0, None, xf
-    _, kl_net, _ = eem_device(y, ang, 1, obj=x0.clone())
-    _, kl_fbp, _ = eem_device(y, ang, 1, obj=xf.clone())
-    accept = kl_net[:, 0] <= kl_fbp[:, 0]
+        return x0, None, xf, None
+    kl_net, ex_net = eem_kl_device(y, ang, x0)
+    kl_fbp, ex_fbp = eem_kl_device(y, ang, xf)
+    accept = (kl_net <= kl_fbp) & (ex_net <= ex_fbp)
     if not accept.all():
         idx = torch.from_numpy(np.nonzero(~accept)[0]).to(x0.device)
         x0.index_copy_(0, idx, xf.index_select(0, idx))
-    return x0, accept, xf
+    return x0, accept, xf, (kl_net, kl_fbp)
 
 
 def iterations_to_reach(curve, target):
@@ -197,15 +218,17 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
         return r, time.perf_counter() - t
 
     # plain eEM: KL curve over K iterations (untimed exploration), then the timed run to the lowest KL
-    _, kl_plain, _ = eem_device(y, ang, K)
-    target = np.nanmin(np.where(np.isfinite(kl_plain), kl_plain, np.nan), axis=1)
-    k_plain = np.nanargmin(np.where(np.isfinite(kl_plain), kl_plain, np.nan), axis=1)
+    _, kl_plain, ex_plain = eem_device(y, ang, K)
+    kl_plain = valid_kl(kl_plain, ex_plain)
+    target = kl_plain.min(axis=1)
+    k_plain = kl_plain.argmin(axis=1)
     Kp = int(k_plain.max())
     _, t_plain = timed(lambda: eem_device(y, ang, Kp, with_kl=False))
     # FBP start and learned start: iterations needed
-    (x0, accept, xf), t_ws = timed(lambda: warm_start(net, p_hat, y, ang, cfg))
-    _, kl_net, _ = eem_device(y, ang, K, obj=x0.clone())
-    _, kl_fbp, _ = eem_device(y, ang, K, obj=xf.clone())
+    (x0, accept, xf, kl0), t_ws = timed(lambda: warm_start(net, p_hat, y, ang, cfg))
+    _, kl_net, ex_net = eem_device(y, ang, K, obj=x0.clone())
+    _, kl_fbp, ex_fbp = eem_device(y, ang, K, obj=xf.clone())
+    kl_net, kl_fbp = valid_kl(kl_net, ex_net), valid_kl(kl_fbp, ex_fbp)
     k_net = iterations_to_reach(kl_net, target)
     k_fbp = iterations_to_reach(kl_fbp, target)
     reached = k_net >= 0
@@ -216,11 +239,16 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
            else None, "learned_reached": int(reached.sum()),
            "fbp_start_iterations_median": float(np.median(k_fbp[k_fbp >= 0])) if (k_fbp >= 0).any() else None,
            "fbp_start_reached": int((k_fbp >= 0).sum()), "accepted": int(accept.sum()),
-           "kl_start_net_median": float(np.median(kl_net[:, 0])), "kl_start_fbp_median": float(np.median(kl_fbp[:, 0])),
-           "kl_target_median": float(np.median(target))}
+           "kl_start_net_median": float(np.median(kl0[0])), "kl_start_fbp_median": float(np.median(kl0[1])),
+           "kl_target_median": float(np.median(target)),
+           "fbp_start_iterations_max": int(k_fbp.max()) if (k_fbp >= 0).all() else None}
+    miss = np.nonzero(~reached)[0]
+    res["not_reached"] = [{"slice": int(s), "target": float(target[s]), "best_kl": float(kl_net[s].min()),
+                           "best_iter": int(kl_net[s].argmin()), "kl_start": float(kl_net[s, 0]),
+                           "plain_best_iter": int(k_plain[s])} for s in miss[:8]]
     if Kl is not None:
         def learned():
-            x, _, _ = warm_start(net, p_hat, y, ang, cfg)
+            x, _, _, _ = warm_start(net, p_hat, y, ang, cfg)
             if Kl > 0:
                 eem_device(y, ang, Kl, obj=x, with_kl=False)
             return x

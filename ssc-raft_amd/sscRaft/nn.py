@@ -143,13 +143,34 @@ class UNet:
         return self.params.cpu().numpy().copy()
 
     def save(self, path, **meta):
-        np.savez(path, params=self.numpy_params(), base=self.base, adam_steps=self.step_count,
-                 **{k: np.asarray(v) for k, v in meta.items()})
+        """Weights file: 3x3 conv weights as bf16 bit patterns (the convolutions
+        only ever read their bf16 copy), biases and the head in FP32."""
+        p = self.numpy_params()
+        conv = np.zeros(self.n, dtype=bool)
+        for name, co, ci in self.layers:
+            w0, b0 = self.offsets[name][0], self.offsets[name][1]
+            conv[w0:b0] = True
+        u = p.view(np.uint32).astype(np.uint64)
+        bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        np.savez_compressed(path, conv_bf16=bits[conv], other_f32=p[~conv], n=self.n, base=self.base,
+                            adam_steps=self.step_count, **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path, device="cuda"):
         with np.load(path, allow_pickle=False) as z:
-            return cls(params=z["params"], base=int(z["base"]), device=device)
+            base = int(z["base"])
+            if "params" in z.files:
+                return cls(params=z["params"], base=base, device=device)
+            offsets, n = param_layout(base)
+            if int(z["n"]) != n:
+                raise ValueError(f"{path}: {int(z['n'])} parameters, expected {n}")
+            conv = np.zeros(n, dtype=bool)
+            for name, co, ci in layer_table(base):
+                conv[offsets[name][0]:offsets[name][1]] = True
+            p = np.zeros(n, np.float32)
+            p[conv] = (z["conv_bf16"].astype(np.uint32) << 16).view(np.float32)
+            p[~conv] = z["other_f32"]
+            return cls(params=p, base=base, device=device)
 
     # --------------------------------------------------------------- buffers
     def _buffers(self, n, h, w):

@@ -57,6 +57,12 @@ libraft.raft_radon_device.argtypes = [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f,
 libraft.raft_radon_device.restype = _i
 libraft.raft_eem_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_eem_device.restype = _i
+libraft.getFilterLowPassMultiGPU.argtypes = [_vp, _i, _vp, _vp, _vp]
+libraft.getFilterLowPassMultiGPU.restype = None
+libraft.raft_lowpass_device.argtypes = [_i, _vp, _vp, _vp, _vp]
+libraft.raft_lowpass_device.restype = _i
+libraft.raft_eem_kl_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+libraft.raft_eem_kl_device.restype = _i
 libraft.raft_fbp_plan_info.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_fbp_plan_info.restype = _i
 libraft.raft_set_profiling.argtypes = [_i]

@@ -64,8 +64,27 @@ def test_no_device_error_path():
         sscRaft.radon_RT(np.zeros((1, 16, 16), np.float32), ang, [0])
     with pytest.raises(RuntimeError):
         sscRaft.eEMRT_GPU_(np.ones((1, 8, 16), np.float32), ang, 2, [0], 0)
+    with pytest.raises(RuntimeError):
+        sscRaft.lowpass(np.ones((1, 8, 16), np.float32), dic={"gpu": [0]})
     sscRaft.libraft.raft_clear_error()
     assert sscRaft.libraft.raft_last_error() == 0
+
+
+def test_lowpass_param_packing_and_defaults_match_reference():
+    """processing/filters.py:38-80: defaults, offset forced to 0, beta/delta inversion."""
+    from sscRaft.io.io_ import SetDictionary
+    from sscRaft.processing.filters import lowpass_params
+
+    dic = SetDictionary({"gpu": [0]}, ("gpu",),
+                        ("filter", "padding", "regularization", "beta/delta", "blocksize", "energy[eV]", "z2[m]",
+                         "detectorPixel[m]"), ("ramp", 2, 0.0, 0.0, 0, 1.0, 1.0, 1.0))
+    pf, pi = lowpass_params(100, 40, 2, dic)
+    np.testing.assert_array_equal(pi, np.array([100, 40, 2, 100, 2, 0, 0, 7, 0, 0], np.int32))
+    np.testing.assert_array_equal(pf, np.array([0.0, 0.0, 1.0, 0.0, 1.0, 1.0, 0.0], np.float32))
+    dic.update({"beta/delta": 50.0, "filter": "gaussian", "regularization": 0.3, "energy[eV]": 8.0, "z2[m]": 2.0})
+    pf, pi = lowpass_params(100, 40, 2, dic)
+    np.testing.assert_array_equal(pf, np.array([1 / 50.0, 0.3, 8.0, 2.0, 1.0, 1.0, 0.0], np.float32))
+    assert pi[7] == 1
 
 
 def plan(angles, paramf, parami):

@@ -228,3 +228,56 @@ def test_eem_graph_replay_matches_direct_launches(gpu, monkeypatch):
     np.testing.assert_array_equal(ex1, ex2)
     xr, klr, _ = oracle.eem(y, ang, k)
     assert rel(x1, xr) <= 1e-5
+
+
+def test_eem_kl_only_matches_first_iteration(gpu):
+    """raft_eem_kl_device: KL of the given image, image unchanged, equal to KL[0] of an eEM run."""
+    import torch
+    from sscRaft import device as dv
+
+    n, a = 48, 40
+    img = ph.phantoms(n, 3, config=3, first=2)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=6), 0).astype(np.float32)
+    x0 = np.random.default_rng(1).uniform(0.2, 1.0, (3, n, n)).astype(np.float32)
+    yt, xt = torch.from_numpy(y).cuda(), torch.from_numpy(x0).cuda()
+    kl, ex = dv.eem_kl_device(yt, ang, xt)
+    np.testing.assert_array_equal(xt.cpu().numpy(), x0)
+    _, kl_ref, ex_ref = oracle.eem(y, ang, 1, obj=x0.astype(np.float64))
+    assert np.max(np.abs(kl - kl_ref[:, 0]) / kl_ref[:, 0]) <= 1e-5
+    np.testing.assert_array_equal(ex, ex_ref[:, 0])
+    _, kl1, _ = dv.eem_device(yt, ang, 1, obj=xt.clone())
+    np.testing.assert_array_equal(kl, kl1[:, 0])
+
+
+@pytest.mark.parametrize("filt", FILTERS)
+@pytest.mark.parametrize("n,a,pad", [(64, 48, 2), (37, 20, 1), (256, 64, 3)])
+def test_lowpass(gpu, filt, n, a, pad):
+    """getFilterLowPassMultiGPU (in place) vs the oracle: FBP filter stage, offset 0, cropped to N bins."""
+    import sscRaft
+
+    rng = np.random.default_rng(n + a + pad)
+    tomo = rng.standard_normal((3, a, n)).astype(np.float32)
+    dic = {"gpu": [0], "filter": filt, "padding": pad, "regularization": 0.6, "beta/delta": 90.0,
+           "energy[eV]": 2.0e4, "z2[m]": 0.4, "detectorPixel[m]": 0.3, "blocksize": 0}
+    from sscRaft.processing.filters import lowpass_params
+
+    pf, pi = lowpass_params(n, a, 3, sscRaft.SetDictionary(dict(dic), ("gpu",), (), ()))
+    ref = oracle.lowpass(tomo, pf, pi)
+    out = sscRaft.lowpass(tomo.copy(), dic=dic)
+    e = rel(out, ref)
+    print(f"lowpass {filt} n={n} a={a} pad={pad}: rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
+def test_lowpass_device_in_place_and_offset_ignored(gpu):
+    import torch
+    from sscRaft import device as dv
+
+    n, a = 96, 30
+    tomo = np.random.default_rng(3).standard_normal((5, a, n)).astype(np.float32)
+    pi = np.array([n, a, 5, n, 2, 0, 0, 7, 0, 0], np.int32)
+    pf = np.array([0, 0, 1, 0, 0.5, 0.5, 3.0], np.float32)
+    t = torch.from_numpy(tomo).cuda()
+    dv.lowpass_device(t, pf, pi)
+    assert rel(t.cpu().numpy(), oracle.lowpass(tomo, pf, pi)) <= FP32_TOL

@@ -415,3 +415,31 @@ def test_eem_kl_decreases_on_phantom():
     y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=3), 0)
     _, kl, _ = oracle.eem(y, ang, 8)
     assert kl[0, -1] < kl[0, 0]
+
+
+@pytest.mark.parametrize("filt", [1, 3, 5, 7])
+def test_lowpass_is_the_cropped_filter_stage_without_offset(filt):
+    """lowpass = rows [L, L+N) of the FBP filter stage with the axis offset set
+    to 0 (low_pass.cu:20 builds its Filter with axis_offset = 0.0); checked
+    against the independent NumPy restatement of the filter stage."""
+    import oracle_np
+
+    rng = np.random.default_rng(filt)
+    n, A, S, P = 37, 6, 2, 2
+    tomo = rng.standard_normal((S, A, n)).astype(np.float32)
+    pi = np.array([n, A, S, n, P, 0, 0, filt, 0, 0], np.int32)
+    pf = np.array([0.02, 0.7, 1.0, 0.0, 0.5, 0.5, 1.25], np.float32)  # offset 1.25 must be ignored
+    got = oracle.lowpass(tomo, pf, pi)
+    pf0 = pf.copy()
+    pf0[6] = 0.0
+    L = (P * n) // 2
+    ref = oracle_np.filter_rows(tomo, pf0, pi)[:, :, L:L + n]
+    np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-12)
+    assert got.shape == (S, A, n)
+
+
+def test_lowpass_none_returns_the_input():
+    tomo = np.random.default_rng(0).standard_normal((1, 4, 16)).astype(np.float32)
+    pi = np.array([16, 4, 1, 16, 2, 0, 0, 0, 0, 0], np.int32)
+    pf = np.array([0, 0, 1, 0, 1, 1, 0], np.float32)
+    np.testing.assert_array_equal(oracle.lowpass(tomo, pf, pi), tomo)
