@@ -103,6 +103,20 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* Transmission EM (replaces get_tEM_RT_MultiGPU, cuda/inc/geometries/parallel/
+ * em.hpp:13-15, em_rt.cu:12-80; bound at rafttypes.py:128-133, called from
+ * sscRaft/geometries/parallel/em_methods.py:141): obj [nslices][objsize][objsize]
+ * in/out (initial guess), count [nslices][nangles][nrays] (= exp(-data), as the
+ * Python wrapper passes it), flat [nslices][nrays] (one row per slice), angles
+ * [nangles]; parami as eEM; paramf unused.  The flat row of slice s is used
+ * for slice s (the reference indexes the flat buffer by angle). */
+void get_tEM_RT_MultiGPU(int* gpus, int ngpus, float* obj, float* count, float* flat, float* angles, float* paramf,
+                         int* parami);
+
+/* Same on one device (device pointers, stream-ordered, synchronises before returning). */
+int raft_tem_device(int device, float* d_obj, const float* d_count, const float* d_flat, const float* angles,
+                    const int* parami, void* stream);
+
 /* Filter-only export (replaces the reference's getFilterLowPassMultiGPU,
  * cuda/src/processing/filters/low_pass.cu:87-141, bound at rafttypes.py:190-195,
  * called from sscRaft/processing/filters.py:84): in place on host

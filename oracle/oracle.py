@@ -182,6 +182,29 @@ def em_backproject(q, angles, n, ones=False, prec="f64"):
     return out
 
 
+def tem(count, flat, angles, iterations, obj=None, prec="f64"):
+    """get_tEM_RT_MultiGPU restated (em_rt.cu:155-192, em_RT_kernel.cu:20-31,
+    52-65,92-184) from the pinned eEM operators:
+        backcounts = B(count);  K x { x <- x * B(flat[s] * exp(-A x)) / backcounts }
+    count [S, A, R] (already exp(-data)), flat [S, R]: slice s uses its own
+    flat row (the reference indexes the flat buffer by angle, em_RT_kernel.cu:29).
+    obj: initial guess [S, n, n] (ones by default, n = R)."""
+    count = np.asarray(count, dtype=np.float32)
+    if count.ndim == 2:
+        count = count[None]
+    S, A, R = count.shape
+    flat = np.asarray(flat, dtype=np.float32).reshape(S, R)
+    x = np.ones((S, R, R)) if obj is None else np.array(obj, dtype=np.float64, copy=True).reshape(S, -1, R)
+    n = x.shape[-1]
+    for s in range(S):
+        bc = em_backproject(count[s], angles, n, prec=prec)
+        for _ in range(iterations):
+            v = em_forward(x[s], angles, R, prec=prec)
+            t = flat[s][None, :].astype(np.float64) * np.exp(-v)
+            x[s] = x[s] * em_backproject(t, angles, n, prec=prec) / bc
+    return x
+
+
 def em_forward(x, angles, nrays, prec="f64"):
     x = np.ascontiguousarray(x, dtype=_dt(prec))
     n = x.shape[0]

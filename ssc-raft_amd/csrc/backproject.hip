@@ -34,13 +34,14 @@ struct BpKArgs {
     const uint4* q;     // [G][A][W] entries of SB values
     float* out;         // MODE 0: [S][n][n]; MODE 1: interleaved [G][n][n][4]
     const float* x;     // MODE 1: interleaved current image [G][n][n][4] (may alias out)
-    const float* norm;  // MODE 1: backones [n][n] (shared by all slices)
+    const float* norm;  // MODE 1: backones [n][n] (shared by all slices), or per slice interleaved if norm_il
     float* xt;          // MODE 1: interleaved transposed image [G][n][n][4] (or null)
     const double* cA;
     const double* cB;
     double xc, c0;
     int n, A, tlo, W, S;
     int tiles_x;
+    int norm_il;
 };
 
 // accumulate one 16-byte staged entry into SB running sums
@@ -204,12 +205,18 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
             if (MODE == 1) {
                 // eEM update on the interleaved image (SB == 4 here)
                 const size_t pix = ((size_t)g * nn + (size_t)j * a.n + i) * 4;
-                const float nv = a.norm[(size_t)j * a.n + i];
+                float4 nv;
+                if (a.norm_il) {
+                    nv = *reinterpret_cast<const float4*>(a.norm + pix);
+                } else {
+                    const float n1 = a.norm[(size_t)j * a.n + i];
+                    nv = make_float4(n1, n1, n1, n1);
+                }
                 float4 xv = *reinterpret_cast<const float4*>(a.x + pix);
-                xv.x = xv.x * tot[q][p][0] / nv;
-                xv.y = xv.y * tot[q][p][1] / nv;
-                xv.z = xv.z * tot[q][p][2] / nv;
-                xv.w = xv.w * tot[q][p][3] / nv;
+                xv.x = xv.x * tot[q][p][0] / nv.x;
+                xv.y = xv.y * tot[q][p][1] / nv.y;
+                xv.z = xv.z * tot[q][p][2] / nv.z;
+                xv.w = xv.w * tot[q][p][3] / nv.w;
                 *reinterpret_cast<float4*>(a.out + pix) = xv;
                 if (a.xt) *reinterpret_cast<float4*>(a.xt + ((size_t)g * nn + (size_t)i * a.n + j) * 4) = xv;
             } else {
@@ -233,6 +240,27 @@ __global__ void k_fill_rows(float4* q, const float* wa, int A, int W, int tlo, i
         const int T = tlo + w;
         const float v = (T >= 0 && T < nbins) ? wa[ang] : 0.0f;
         q[e] = make_float4(v, v, v, v);
+    }
+}
+
+// rows from planar data [S][A][nbins]: q[g][a][w][s] = data[4g+s][a][tlo+w] * wa[a]
+__global__ void k_rows_from_planar(float4* q, const float* data, const float* wa, int S, int A, int W, int tlo,
+                                   int nbins)
+{
+    const int G = (S + 3) / 4;
+    const size_t total = (size_t)G * A * W;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int w = (int)(e % W);
+        const int ang = (int)((e / W) % A);
+        const int g = (int)(e / ((size_t)W * A));
+        const int T = tlo + w;
+        float v[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int sl = 4 * g + s;
+            v[s] = (T >= 0 && T < nbins && sl < S) ? data[((size_t)sl * A + ang) * nbins + T] * wa[ang] : 0.0f;
+        }
+        q[e] = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -300,7 +328,7 @@ void launch_backproject(const BpArgs& b, int store, hipStream_t st)
     if (b.S <= 0 || b.n <= 0) return;
     BpKArgs k;
     k.q = static_cast<const uint4*>(b.q);
-    k.out = b.out; k.x = b.x; k.norm = b.norm; k.xt = b.xt;
+    k.out = b.out; k.x = b.x; k.norm = b.norm; k.xt = b.xt; k.norm_il = b.norm_il;
     k.cA = b.cA; k.cB = b.cB; k.xc = b.xc; k.c0 = b.c0;
     k.n = b.n; k.A = b.A; k.tlo = b.tlo; k.W = b.W; k.S = b.S;
     k.tiles_x = (b.n + BP_TILE_X - 1) / BP_TILE_X;
@@ -318,6 +346,17 @@ void launch_backproject(const BpArgs& b, int store, hipStream_t st)
     } else {
         hipLaunchKernelGGL((k_backproject<float, 0>), grid, block, 0, st, k);
     }
+    RAFT_HIP(hipGetLastError());
+}
+
+void launch_rows_from_planar(float* q, const float* data, const float* wa, int S, int A, int W, int tlo, int nbins,
+                             hipStream_t st)
+{
+    const size_t total = (size_t)((S + 3) / 4) * A * W;
+    if (!total) return;
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_rows_from_planar, dim3(blocks), dim3(256), 0, st, reinterpret_cast<float4*>(q), data, wa, S,
+                       A, W, tlo, nbins);
     RAFT_HIP(hipGetLastError());
 }
 

@@ -203,6 +203,18 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
                     const int sl = g * 4 + s;
                     if (sl < a.S) a.out[((size_t)sl * a.nangles + ang) * a.nrays + i] = v[s];
                 }
+            } else if (EM == 2) {
+                // transmission EM (em_RT_kernel.cu:20-31 with the slice's own flat
+                // row, y = flat [S][nrays]): flat * exp(-v), times dtheta
+                const float wv = a.wa[ang];
+                float rr[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int sl = g * 4 + s;
+                    rr[s] = (sl < a.S) ? a.y[(size_t)sl * a.nrays + i] * expf(-v[s]) * wv : 0.0f;
+                }
+                reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
+                    make_float4(rr[0], rr[1], rr[2], rr[3]);
             } else {
                 const float wv = a.wa[ang];
                 float rr[4];
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
         }
         __syncthreads(); // keep the waves (angles) of the workgroup on the same ray block
     }
-    if (!EM || !a.kl_part || !have) return;
+    if (EM != 1 || !a.kl_part || !have) return;
     // deterministic per-(slice, angle) KL partial: wave reduction in fixed order
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -295,7 +307,8 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     const int G = (p.S + 3) / 4;
     if (G > 65535) fail(RAFT_EUNSUPPORTED, "projector grid too large");
     dim3 grid((p.g.nangles + PROJ_ANGLES - 1) / PROJ_ANGLES, G);
-    if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
+    if (em_mode == 2) hipLaunchKernelGGL((k_project<2>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
+    else if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
     else hipLaunchKernelGGL((k_project<0>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
     RAFT_HIP(hipGetLastError());
 }

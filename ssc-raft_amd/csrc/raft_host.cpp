@@ -729,6 +729,115 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     RAFT_HIP(hipStreamSynchronize(st));
 }
 
+// Transmission EM (em_rt.cu:155-192, em_RT_kernel.cu:20-31,52-65,92-184):
+//   backcounts = B(count);  K x { x <- x * B(flat * exp(-A x)) / backcounts }
+// with the eEM projector / backprojector geometry.  The flat field is the
+// slice's own row flat[s][nrays] (the reference indexes it by angle,
+// em_RT_kernel.cu:29, reading rows it never copied).
+static void tem_device(int dev, float* d_obj, const float* d_count, const float* d_flat, const float* angles,
+                       const int* parami, hipStream_t st)
+{
+    if (!parami) fail(RAFT_EINVAL, "null parameter array");
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = parami[8];
+    const int blocksize = parami[9];
+    if (R < 2) fail(RAFT_EINVAL, "nrays must be >= 2");
+    if (A < 2) fail(RAFT_EINVAL, "nangles must be >= 2");
+    if (n < 2) fail(RAFT_EINVAL, "objsize must be >= 2");
+    if (S < 0 || K < 0) fail(RAFT_EINVAL, "nslices and iterations must be >= 0");
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    const double rr = (double)(R - 1) / (double)(n - 1);
+    if ((double)n * (double)n * 16.0 >= 2147483648.0) fail(RAFT_EUNSUPPORTED, "image too large for 32-bit offsets");
+    if (rr * std::hypot((double)(BP_TILE_X - 1), (double)(BP_TILE_Y - 1)) + 5.0 > (double)BP_SLOT)
+        fail(RAFT_EUNSUPPORTED, "nrays larger than objsize beyond the backprojection tile window");
+    if (S == 0) return;
+    if (!d_obj || !d_count || !d_flat) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+
+    std::vector<double> cv(A), sv(A);
+    std::vector<float> wa(A);
+    BpGeom g;
+    g.n = n;
+    g.nangles = A;
+    g.nbins = R;
+    g.xc = (double)(n - 1) / 2.0;
+    g.c0 = (double)(R - 1) / 2.0;
+    g.cA.resize(A);
+    g.cB.resize(A);
+    for (int a = 0; a < A; ++a) {
+        cv[a] = std::cos((double)angles[a]);
+        sv[a] = std::sin((double)angles[a]);
+        g.cA[a] = cv[a] * rr;
+        g.cB[a] = sv[a] * rr;
+        wa[a] = (float)angle_step(angles, A, a);
+    }
+    bp_window(g);
+    if (g.tlo > 0 || g.tlo + g.width < R) {
+        const int lo = std::min(g.tlo, 0), hi = std::max(g.tlo + g.width, R);
+        g.tlo = lo;
+        g.width = hi - lo;
+    }
+    DevBuf dc, ds, dcA, dcB, dwa;
+    upload(dc, cv, st);
+    upload(ds, sv, st);
+    upload(dcA, g.cA, st);
+    upload(dcB, g.cB, st);
+    upload(dwa, wa, st);
+    RayGeom rg = ray_geom_em(R, A, n);
+
+    const size_t img = (size_t)n * n;
+    const size_t per_slice = img * 16 + (size_t)A * g.width * 4;
+    int B = blocksize > 0 ? std::min(blocksize, S) : S;
+    B = ((B + 3) / 4) * 4;
+    const size_t budget = block_budget();
+    while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
+    const int Gb = B / 4;
+    float* rows = static_cast<float*>(D.workspace(0, (size_t)Gb * A * g.width * 16));
+    float* xil = static_cast<float*>(D.workspace(1, (size_t)Gb * img * 16));
+    float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
+    float* normil = static_cast<float*>(D.workspace(3, (size_t)Gb * img * 16));
+    float* tmp = static_cast<float*>(D.workspace(5, (size_t)B * img * 4));
+
+    for (int s0 = 0; s0 < S; s0 += B) {
+        const int nb = std::min(B, S - s0);
+        float* x = d_obj + (size_t)s0 * img;
+        // backcounts = B(count), per slice, into the interleaved layout of the iterate
+        launch_rows_from_planar(rows, d_count + (size_t)s0 * A * R, dwa.as<float>(), nb, A, g.width, g.tlo, R, st);
+        BpArgs b0;
+        b0.q = rows; b0.out = tmp; b0.x = nullptr; b0.norm = nullptr; b0.xt = nullptr;
+        b0.cA = dcA.as<double>(); b0.cB = dcB.as<double>(); b0.xc = g.xc; b0.c0 = g.c0;
+        b0.n = n; b0.A = A; b0.S = nb; b0.tlo = g.tlo; b0.W = g.width; b0.mode = 0;
+        launch_backproject(b0, RAFT_STORE_F32, st);
+        launch_to_interleaved(tmp, normil, nb, n, n, false, st);
+        // the rows' margins (bins outside [0, R)) are zero from here on: the
+        // projector rewrites only [0, R)
+        launch_to_interleaved(x, xil, nb, n, n, false, st);
+        launch_to_interleaved(x, xT, nb, n, n, true, st);
+        for (int it = 0; it < K; ++it) {
+            ProjArgs p;
+            p.img = xil; p.imgT = xT; p.out = rows; p.y = d_flat + (size_t)s0 * R; p.wa = dwa.as<float>();
+            p.kl_part = nullptr; p.ex_part = nullptr;
+            p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
+            p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
+            {
+                ProfScope ps(KF_PROJ, st);
+                launch_project(p, 2, st);
+            }
+            BpArgs b;
+            b.q = rows; b.out = xil; b.x = xil; b.norm = normil; b.norm_il = 1; b.xt = xT;
+            b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
+            b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
+            {
+                ProfScope ps(KF_EMBP, st);
+                launch_backproject(b, RAFT_STORE_F32, st);
+            }
+        }
+        launch_from_interleaved(xil, x, nb, n, n, st);
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
 // ---------------------------------------------------------------------------
 // host-pointer wrappers: slab split over gpus[], one host thread per device
 // ---------------------------------------------------------------------------
@@ -883,6 +992,55 @@ static void eem_host(int* gpus, int ngpus, float* obj, float* tomo, float* angle
 using namespace raft;
 
 extern "C" {
+
+static void tem_host(int* gpus, int ngpus, float* obj, float* count, float* flat, float* angles, int* parami)
+{
+    if (!parami) fail(RAFT_EINVAL, "null parameter array");
+    if (!obj || !count || !flat || !angles) fail(RAFT_EINVAL, "null data pointer");
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = parami[8];
+    if (R < 2 || A < 2 || n < 2 || S < 0 || K < 0) fail(RAFT_EINVAL, "invalid tEM parameters");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t tin = (size_t)A * R, img = (size_t)n * n;
+        int B = parami[9] > 0 ? std::min(parami[9], cnt) : cnt;
+        const size_t budget = block_budget();
+        const size_t need = (tin + R + 5 * img) * 4 + (size_t)A * (size_t)(1.5 * R + 64) * 4;
+        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
+        DevBuf dt((size_t)B * tin * 4), df((size_t)B * R * 4), dx((size_t)B * img * 4);
+        std::vector<int> pi(parami, parami + 10);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            RAFT_HIP(hipMemcpyAsync(dt.p, count + (size_t)(first + s0) * tin, (size_t)nb * tin * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            RAFT_HIP(hipMemcpyAsync(df.p, flat + (size_t)(first + s0) * R, (size_t)nb * R * 4, hipMemcpyHostToDevice,
+                                    st.s));
+            RAFT_HIP(hipMemcpyAsync(dx.p, obj + (size_t)(first + s0) * img, (size_t)nb * img * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            pi[2] = nb;
+            pi[9] = 0;
+            tem_device(dev, dx.as<float>(), dt.as<float>(), df.as<float>(), angles, pi.data(), st.s);
+            RAFT_HIP(hipMemcpyAsync(obj + (size_t)(first + s0) * img, dx.p, (size_t)nb * img * 4,
+                                    hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+void get_tEM_RT_MultiGPU(int* gpus, int ngpus, float* obj, float* count, float* flat, float* angles, float* paramf,
+                         int* parami)
+{
+    (void)paramf;
+    guarded([&] { tem_host(gpus, ngpus, obj, count, flat, angles, parami); });
+}
+
+int raft_tem_device(int device, float* d_obj, const float* d_count, const float* d_flat, const float* angles,
+                    const int* parami, void* stream)
+{
+    return guarded(
+        [&] { tem_device(device, d_obj, d_count, d_flat, angles, parami, static_cast<hipStream_t>(stream)); });
+}
 
 static void lowpass_host(int* gpus, int ngpus, float* tomo, float* paramf, int* parami)
 {

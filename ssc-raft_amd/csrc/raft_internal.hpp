@@ -128,13 +128,14 @@ struct BpArgs {
     const void* q;      // [G][A][W][SB] staged rows (float SB = 4, or half SB = 8)
     float* out;         // mode 0: planar [S][n][n]; mode 1: interleaved [G][n][n][4]
     const float* x;     // mode 1: interleaved current image (may alias out)
-    const float* norm;  // mode 1: backones [n][n], shared by all slices
+    const float* norm;  // mode 1: backones [n][n] shared by all slices, or per slice interleaved (norm_il)
     float* xt;          // mode 1: interleaved transposed copy of the updated image (or null)
     const double* cA;
     const double* cB;
     double xc, c0;
     int n, A, S, tlo, W;
-    int mode;           // 0 = store the sums (FBP, backones), 1 = eEM update x * bp / norm
+    int mode;           // 0 = store the sums (FBP, backones), 1 = EM update x * bp / norm
+    int norm_il = 0;    // mode 1: norm is per slice, interleaved [G][n][n][4] (tEM)
 };
 void launch_backproject(const BpArgs& a, int store, hipStream_t st);
 
@@ -142,7 +143,7 @@ struct ProjArgs {
     const float* img;   // interleaved [G][Ny][Nx][4]
     const float* imgT;  // interleaved transpose [G][Nx][Ny][4] (null: no transposed path)
     float* out;         // Radon: planar [S][A][nrays]; eEM: interleaved ratio rows [G][A][W][4] at (i - tlo)
-    const float* y;     // eEM: measured sinogram, planar [S][A][nrays] (null for Radon)
+    const float* y;     // eEM: measured sinogram, planar [S][A][nrays] (null for Radon); tEM: flat [S][nrays]
     const float* wa;    // eEM: per-angle dtheta
     double* kl_part;    // eEM: per-(slice, angle) KL partial sums [S][A] (null = skip)
     unsigned long long* ex_part; // eEM: per-(slice, angle) excluded counts
@@ -152,6 +153,7 @@ struct ProjArgs {
     int S;
     int tlo, W;         // eEM ratio-row layout
 };
+// em_mode 0 = Radon, 1 = eEM ratio (+ KL), 2 = tEM flat * exp(-v)
 void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
 
 // planar [S][rows][cols] -> interleaved [G][rows][cols][4] (or its transpose [G][cols][rows][4])
@@ -160,6 +162,9 @@ void launch_to_interleaved(const float* in, float* out, int S, int rows, int col
 void launch_from_interleaved(const float* in, float* out, int S, int rows, int cols, hipStream_t st);
 // rows for B(1): interleaved [A][W][4], value wa[a] on valid bins, 0 elsewhere
 void launch_fill_rows(float* q, const float* wa, int A, int W, int tlo, int nbins, hipStream_t st);
+// rows from planar data: q[g][a][w][s] = data[4g+s][a][tlo+w] * wa[a] (0 outside [0, nbins) and past S)
+void launch_rows_from_planar(float* q, const float* data, const float* wa, int S, int A, int W, int tlo, int nbins,
+                             hipStream_t st);
 // kl[s*K + it] for it = *iter_ptr (if iter_ptr) or iter
 void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,
                       unsigned long long* ex, int K, int iter, const int* iter_ptr, hipStream_t st);

@@ -36,7 +36,8 @@ def fbp(tomogram, angles=None, obj=None, dic=None, nstreams=0, **kwargs):
 def em(data, flat=None, angles=None, obj=None, dic=None, **kwargs):
     """Expectation maximisation for parallel-beam data [slices, angles, rays].
 
-    dic keys (defaults): 'gpu' (required), 'method' (required: 'eEMRT'),
+    dic keys (defaults): 'gpu' (required), 'method' (required: 'eEMRT' or 'tEMRT'),
+    'flat' (tEMRT: [slices, rays], ones when absent),
     'angles[rad]', 'iterations' (10), 'detectorPixel[m]' (0.0), 'padding' (2),
     'beamgeometry' ('parallel'), 'interpolation' ('bilinear'), 'blocksize' (0)."""
     required = ("gpu",)
@@ -56,7 +57,7 @@ def em(data, flat=None, angles=None, obj=None, dic=None, **kwargs):
         flat = dic.get("flat", None)
         if flat is None:
             logger.warning("No flat provided.")
-            flat = numpy.ones((data.shape[0], data.shape[-1]))
+            flat = numpy.ones((1 if len(data.shape) == 2 else data.shape[0], data.shape[-1]))
     if method == "eEMRT":
         return eEMRT_GPU_(data, angles, iterations, gpus, blocksize, obj=obj)
     if method == "tEMRT":

@@ -1,6 +1,7 @@
 """eEMRT_GPU_: emission EM with ray-tracing operators, as
 sscRaft/geometries/parallel/em_methods.py:3-70, plus eEMRT_GPU_KL (same call,
-returning the per-iteration FP64 KL objective as well)."""
+returning the per-iteration FP64 KL objective as well), and tEMRT_GPU_
+(transmission EM, em_methods.py:72-151)."""
 from ...rafttypes import CNICE, libraft, raft_check, numpy, ctypes
 
 __all__ = ["eEMRT_GPU_", "eEMRT_GPU_KL", "tEMRT_GPU_"]
@@ -51,5 +52,31 @@ def eEMRT_GPU_KL(tomo, angles, iterations, gpus, blocksize, obj=None):
 
 
 def tEMRT_GPU_(counts, flat, angles, iterations, gpus, blocksize, obj=None):
-    """Transmission EM is not part of this build yet."""
-    raise NotImplementedError("tEMRT (transmission EM) is not implemented in this build")
+    """Transmission EM of data [slices, angles, rays] with flat [slices, 1, rays]
+    (or [slices, rays]), as em_methods.py:72-151: the C function receives
+    exp(-counts) and a flat row per slice.  Unlike the reference (em_methods.py:109,
+    numpy.exp(-counts, counts)), the caller's `counts` array is not overwritten.
+    `obj` (if given) is the initial guess, updated in place when float32 C-contiguous."""
+    nslices = 1 if len(counts.shape) == 2 else counts.shape[0]
+    nflats = 1 if len(counts.shape) == 2 else numpy.asarray(flat).shape[0]
+    nangles, nrays = counts.shape[-2], counts.shape[-1]
+    objsize = nrays
+    gpus = numpy.ascontiguousarray(numpy.array(gpus).astype(numpy.intc))
+    count = CNICE(numpy.exp(-numpy.asarray(counts, dtype=numpy.float32)))
+    flat = CNICE(numpy.asarray(flat, dtype=numpy.float32))
+    if flat.size != nslices * nrays:
+        raise ValueError(f"flat must hold one row of {nrays} values per slice ({nslices} slices), got {flat.shape}")
+    if obj is not None:
+        obj = CNICE(obj)
+    else:
+        obj = CNICE(numpy.ones([nslices, objsize, objsize], dtype=numpy.float32))
+    angles = CNICE(numpy.array(angles))
+    parami = CNICE(numpy.array([nrays, nangles, nslices, objsize, 0, 0, 0, nflats, iterations, blocksize]),
+                   numpy.int32)
+    paramf = CNICE(numpy.array([0]), numpy.float32)
+    libraft.get_tEM_RT_MultiGPU(gpus.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(len(gpus)),
+                                obj.ctypes.data_as(ctypes.c_void_p), count.ctypes.data_as(ctypes.c_void_p),
+                                flat.ctypes.data_as(ctypes.c_void_p), angles.ctypes.data_as(ctypes.c_void_p),
+                                paramf.ctypes.data_as(ctypes.c_void_p), parami.ctypes.data_as(ctypes.c_void_p))
+    raft_check("get_tEM_RT_MultiGPU")
+    return obj

@@ -66,6 +66,8 @@ def test_no_device_error_path():
         sscRaft.eEMRT_GPU_(np.ones((1, 8, 16), np.float32), ang, 2, [0], 0)
     with pytest.raises(RuntimeError):
         sscRaft.lowpass(np.ones((1, 8, 16), np.float32), dic={"gpu": [0]})
+    with pytest.raises(RuntimeError):
+        sscRaft.tEMRT_GPU_(np.ones((1, 8, 16), np.float32), np.ones((1, 1, 16), np.float32), ang, 2, [0], 0)
     sscRaft.libraft.raft_clear_error()
     assert sscRaft.libraft.raft_last_error() == 0
 
@@ -189,4 +191,6 @@ def test_fbp_front_end_validation():
     with pytest.raises(KeyError):
         sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0]})  # 'method' is required by the reference too
     with pytest.raises(NotImplementedError):
-        sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0], "method": "tEMRT"})
+        sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0], "method": "tEMFQ"})
+    with pytest.raises(ValueError, match="Invalid EM method"):
+        sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0], "method": "xEM"})

@@ -281,3 +281,46 @@ def test_lowpass_device_in_place_and_offset_ignored(gpu):
     t = torch.from_numpy(tomo).cuda()
     dv.lowpass_device(t, pf, pi)
     assert rel(t.cpu().numpy(), oracle.lowpass(tomo, pf, pi)) <= FP32_TOL
+
+
+@pytest.mark.parametrize("n,a,k", [(32, 30, 1), (48, 40, 4), (33, 20, 3)])
+def test_tem(gpu, n, a, k):
+    """get_tEM_RT_MultiGPU vs the oracle (slice-own flat rows)."""
+    import sscRaft
+
+    rng = np.random.default_rng(n + k)
+    img = ph.phantoms(n, 3, config=3, first=n + 1) * 0.5
+    ang = ph.uniform_angles(a)
+    data = ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n)  # -ln(counts / I0)
+    flat = rng.uniform(0.9, 1.1, (3, 1, n)).astype(np.float32)
+    keep = data.copy()
+    x = sscRaft.tEMRT_GPU_(data, flat, ang, k, [0], 0)
+    np.testing.assert_array_equal(data, keep)  # the caller's array is not overwritten
+    xr = oracle.tem(np.exp(-data.astype(np.float64)).astype(np.float32), flat.reshape(3, n), ang, k)
+    e = rel(x, xr)
+    print(f"tem n={n} a={a} K={k}: rel-L2 {e:.2e}")
+    assert e <= 1e-5 * max(1, k)
+
+
+def test_tem_dispatch_and_device_entry(gpu):
+    import torch
+    import sscRaft
+    from sscRaft.rafttypes import libraft, ctypes
+
+    n, a, k = 40, 36, 3
+    img = ph.phantoms(n, 2, config=3, first=7) * 0.5
+    ang = ph.uniform_angles(a)
+    data = ph.poisson_log(oracle.radon(img, ang), 1e4, seed=3)
+    x1 = sscRaft.em(data, angles=ang, dic={"gpu": [0], "method": "tEMRT", "iterations": k})  # flat = ones
+    count = np.exp(-data.astype(np.float64)).astype(np.float32)
+    xr = oracle.tem(count, np.ones((2, n), np.float32), ang, k)
+    assert rel(x1, xr) <= 1e-5 * k
+    xd = torch.ones((2, n, n), device="cuda")
+    cd = torch.from_numpy(count).cuda()
+    fd = torch.ones((2, n), device="cuda")
+    pi = np.array([n, a, 2, n, 0, 0, 0, 2, k, 0], np.int32)
+    rc = libraft.raft_tem_device(0, ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(cd.data_ptr()),
+                                 ctypes.c_void_p(fd.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p),
+                                 pi.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    np.testing.assert_array_equal(xd.cpu().numpy(), x1)

@@ -443,3 +443,28 @@ def test_lowpass_none_returns_the_input():
     pi = np.array([16, 4, 1, 16, 2, 0, 0, 0, 0, 0], np.int32)
     pf = np.array([0, 0, 1, 0, 1, 1, 0], np.float32)
     np.testing.assert_array_equal(oracle.lowpass(tomo, pf, pi), tomo)
+
+
+def test_tem_fixed_point_and_numpy_restatement():
+    """tEM: data generated from x* (count = flat exp(-A x*)) leaves x* fixed;
+    iterates match a NumPy composition of the NumPy eEM operators."""
+    import oracle_np
+
+    rng = np.random.default_rng(4)
+    n, A = 24, 18
+    ang = np.linspace(0, np.pi, A, endpoint=False).astype(np.float32)
+    xs = rng.uniform(0.2, 1.0, (n, n))
+    flat = rng.uniform(0.8, 1.2, (1, n)).astype(np.float32)
+    count = (flat[0][None, :] * np.exp(-oracle.em_forward(xs, ang, n))).astype(np.float32)
+    # float32 data: the fixed point holds to float32 rounding of count
+    x = oracle.tem(count, flat, ang, 3, obj=xs[None])
+    np.testing.assert_allclose(x[0], xs, rtol=2e-5)
+    # generic data: compare with a NumPy composition
+    count2 = rng.uniform(0.3, 1.0, (1, A, n)).astype(np.float32)
+    x2 = oracle.tem(count2, flat, ang, 4)
+    bc = oracle_np.em_backproject(count2[0], ang, n)
+    xr = np.ones((n, n))
+    for _ in range(4):
+        v = oracle_np.em_forward(xr, ang, n)
+        xr = xr * oracle_np.em_backproject(flat[0][None, :].astype(np.float64) * np.exp(-v), ang, n) / bc
+    np.testing.assert_allclose(x2[0], xr, rtol=1e-10)
