@@ -103,6 +103,20 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* Flat/dark background correction (replaces getBackgroundCorrectionMultiGPU,
+ * cuda/src/processing/background_correction.cu:144-178, called from
+ * sscRaft/processing/background_correction.py:105-108): in place on host
+ * frames[nslices][nangles][nrays], flat[nslices][numflats][nrays],
+ * dark[nslices][nrays]:  v = max(D - dark, 0.5) / max(F - dark, 0.5), F the
+ * first flat or, with two or more flats, the first two blended by angle/nangles;
+ * then v = -log(v) when is_log. */
+void getBackgroundCorrectionMultiGPU(int* gpus, int ngpus, float* frames, float* flat, float* dark, int nrays,
+                                     int nangles, int nslices, int numflats, int is_log, int blocksize);
+
+/* Same on one device, in place on d_frames (stream-ordered, asynchronous). */
+int raft_background_device(int device, float* d_frames, const float* d_flat, const float* d_dark, int nrays,
+                           int nangles, int nslices, int numflats, int is_log, void* stream);
+
 /* Transmission EM (replaces get_tEM_RT_MultiGPU, cuda/inc/geometries/parallel/
  * em.hpp:13-15, em_rt.cu:12-80; bound at rafttypes.py:128-133, called from
  * sscRaft/geometries/parallel/em_methods.py:141): obj [nslices][objsize][objsize]

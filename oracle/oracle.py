@@ -182,6 +182,23 @@ def em_backproject(q, angles, n, ones=False, prec="f64"):
     return out
 
 
+def background(frames, flat, dark, is_log=False):
+    """getBackgroundCorrectionMultiGPU restated (background_correction.cu:13-45,
+    opt.cu:204-214) in float64 from the float32 inputs: frames [S, A, R],
+    flat [S, nflats, R], dark [S, R]."""
+    D = np.asarray(frames, np.float32).astype(np.float64)
+    S, A, R = D.shape
+    F = np.asarray(flat, np.float32).astype(np.float64).reshape(S, -1, R)
+    dk = np.asarray(dark, np.float32).astype(np.float64).reshape(S, 1, R)
+    if F.shape[1] > 1:
+        interp = (np.arange(A, dtype=np.float32) / np.float32(A)).astype(np.float64)[None, :, None]
+        ft = F[:, :1, :] * (1.0 - interp) + interp * F[:, 1:2, :]
+    else:
+        ft = F[:, :1, :]
+    v = np.maximum(D - dk, 0.5) / np.maximum(ft - dk, 0.5)
+    return -np.log(v) if is_log else v
+
+
 def tem(count, flat, angles, iterations, obj=None, prec="f64"):
     """get_tEM_RT_MultiGPU restated (em_rt.cu:155-192, em_RT_kernel.cu:20-31,
     52-65,92-184) from the pinned eEM operators:

@@ -993,6 +993,57 @@ using namespace raft;
 
 extern "C" {
 
+static void background_host(int* gpus, int ngpus, float* frames, float* flat, float* dark, int R, int A, int S,
+                            int nflats, int is_log, int blocksize)
+{
+    if (R < 1 || A < 1 || S < 0 || nflats < 1) fail(RAFT_EINVAL, "invalid sizes");
+    if (S == 0) return;
+    if (!frames || !flat || !dark) fail(RAFT_EINVAL, "null data pointer");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t per = ((size_t)A * R + (size_t)R * nflats + R) * 4;
+        int B = blocksize > 0 ? std::min(blocksize, cnt) : cnt;
+        const size_t budget = block_budget();
+        while (B > 1 && (size_t)B * per > budget) B = (B + 1) / 2;
+        DevBuf dd((size_t)B * A * R * 4), df((size_t)B * R * nflats * 4), dk((size_t)B * R * 4);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            const size_t z = (size_t)(first + s0);
+            float* h = frames + z * A * R;
+            RAFT_HIP(hipMemcpyAsync(dd.p, h, (size_t)nb * A * R * 4, hipMemcpyHostToDevice, st.s));
+            RAFT_HIP(hipMemcpyAsync(df.p, flat + z * R * nflats, (size_t)nb * R * nflats * 4, hipMemcpyHostToDevice,
+                                    st.s));
+            RAFT_HIP(hipMemcpyAsync(dk.p, dark + z * R, (size_t)nb * R * 4, hipMemcpyHostToDevice, st.s));
+            launch_background(dd.as<float>(), df.as<float>(), dk.as<float>(), R, A, nb, nflats, is_log, st.s);
+            RAFT_HIP(hipMemcpyAsync(h, dd.p, (size_t)nb * A * R * 4, hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+void getBackgroundCorrectionMultiGPU(int* gpus, int ngpus, float* frames, float* flat, float* dark, int nrays,
+                                     int nangles, int nslices, int numflats, int is_log, int blocksize)
+{
+    guarded([&] {
+        background_host(gpus, ngpus, frames, flat, dark, nrays, nangles, nslices, numflats, is_log, blocksize);
+    });
+}
+
+int raft_background_device(int device, float* d_frames, const float* d_flat, const float* d_dark, int nrays,
+                           int nangles, int nslices, int numflats, int is_log, void* stream)
+{
+    return guarded([&] {
+        if (nrays < 1 || nangles < 1 || nslices < 0 || numflats < 1) fail(RAFT_EINVAL, "invalid sizes");
+        if (nslices == 0) return;
+        if (!d_frames || !d_flat || !d_dark) fail(RAFT_EINVAL, "null data pointer");
+        bind_device(device);
+        launch_background(d_frames, d_flat, d_dark, nrays, nangles, nslices, numflats, is_log,
+                          static_cast<hipStream_t>(stream));
+    });
+}
+
 static void tem_host(int* gpus, int ngpus, float* obj, float* count, float* flat, float* angles, int* parami)
 {
     if (!parami) fail(RAFT_EINVAL, "null parameter array");

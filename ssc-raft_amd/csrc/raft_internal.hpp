@@ -171,6 +171,10 @@ void launch_kl_reduce(const double* part, const unsigned long long* expart, int 
 // device iteration counter: op 0 clears, op 1 increments (a kernel, so it may be graph-captured)
 void launch_counter(int* c, int op, hipStream_t st);
 
+// flat/dark background correction (+ optional -log), in place on [S][A][R]
+void launch_background(float* data, const float* flat, const float* dark, int R, int A, int S, int nflats, int is_log,
+                       hipStream_t st);
+
 // profiling hooks (events around kernel families)
 enum KernelFamily { KF_FILTER = 0, KF_BP = 1, KF_PROJ = 2, KF_EMBP = 3, KF_COUNT = 4 };
 struct ProfScope {

@@ -1,1 +1,2 @@
+from .background_correction import *
 from .filters import *

@@ -57,6 +57,10 @@ libraft.raft_radon_device.argtypes = [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f,
 libraft.raft_radon_device.restype = _i
 libraft.raft_eem_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_eem_device.restype = _i
+libraft.getBackgroundCorrectionMultiGPU.argtypes = [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i]
+libraft.getBackgroundCorrectionMultiGPU.restype = None
+libraft.raft_background_device.argtypes = [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]
+libraft.raft_background_device.restype = _i
 libraft.get_tEM_RT_MultiGPU.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
 libraft.get_tEM_RT_MultiGPU.restype = None
 libraft.raft_tem_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp]

@@ -67,6 +67,12 @@ def test_no_device_error_path():
     with pytest.raises(RuntimeError):
         sscRaft.lowpass(np.ones((1, 8, 16), np.float32), dic={"gpu": [0]})
     with pytest.raises(RuntimeError):
+        sscRaft.correct_background(np.ones((1, 8, 16), np.float32), np.ones((1, 1, 16), np.float32),
+                                   np.zeros((1, 16), np.float32))
+    with pytest.raises(ValueError, match="Flat dimension"):
+        sscRaft.correct_background(np.ones((2, 8, 16), np.float32), np.ones((1, 1, 16), np.float32),
+                                   np.zeros((2, 16), np.float32))
+    with pytest.raises(RuntimeError):
         sscRaft.tEMRT_GPU_(np.ones((1, 8, 16), np.float32), np.ones((1, 1, 16), np.float32), ang, 2, [0], 0)
     sscRaft.libraft.raft_clear_error()
     assert sscRaft.libraft.raft_last_error() == 0

@@ -468,3 +468,18 @@ def test_tem_fixed_point_and_numpy_restatement():
         v = oracle_np.em_forward(xr, ang, n)
         xr = xr * oracle_np.em_backproject(flat[0][None, :].astype(np.float64) * np.exp(-v), ang, n) / bc
     np.testing.assert_allclose(x2[0], xr, rtol=1e-10)
+
+
+def test_background_hand_cases():
+    """Closed-form values: dark subtraction, the 0.5 floors, the two-flat blend
+    by angle / nangles (angle 0 = first flat), -log."""
+    frames = np.array([[[10.0, 3.0, 0.2], [10.0, 3.0, 0.2]]], np.float32)  # S=1, A=2, R=3
+    dark = np.array([[1.0, 1.0, 0.0]], np.float32)
+    flat1 = np.array([[[5.0, 1.2, 2.0]]], np.float32)
+    v = oracle.background(frames, flat1, dark)
+    np.testing.assert_allclose(v[0, 0], [9.0 / 4.0, 2.0 / 0.5, 0.5 / 2.0])
+    flat2 = np.array([[[5.0, 1.2, 2.0], [9.0, 1.2, 4.0]]], np.float32)
+    v2 = oracle.background(frames, flat2, dark)
+    np.testing.assert_allclose(v2[0, 0], v[0, 0])                         # interp = 0
+    np.testing.assert_allclose(v2[0, 1, 0], 9.0 / (0.5 * 4.0 + 0.5 * 8.0))  # interp = 1/2
+    np.testing.assert_allclose(oracle.background(frames, flat2, dark, is_log=True), -np.log(v2))
