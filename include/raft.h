@@ -141,6 +141,24 @@ void getFilterLowPassMultiGPU(int* gpus, int ngpus, float* tomogram, float* para
 /* Same on one device, in place on d_tomo (stream-ordered, synchronises before returning). */
 int raft_lowpass_device(int device, float* d_tomo, const float* paramf, const int* parami, void* stream);
 
+/* Batched uneven scans in one graph launch (no counterpart in the reference,
+ * which reconstructs one volume of equal-sized slices per call).  For scan i:
+ *   FBP of d_tomo[i] [nangles[i]][nrays[i]] -> d_fbp[i] [nrays[i]]^2, with
+ *     paramf[7 i .. 7 i + 6] (getFBPMultiGPU layout), padding `pad`, filter
+ *     `filter`, filtered rows stored as `store` (RAFT_STORE_F32 / _F16);
+ *   `iterations` eEM iterations on d_y[i] [nangles[i]][nrays[i]], d_em[i] the
+ *     in/out iterate [nrays[i]]^2 (objsize = nrays).
+ * Pass d_fbp = NULL to skip FBP, d_em = NULL (or iterations 0) to skip eEM.
+ * Scans with equal geometry are batched together; every batch is a branch of
+ * one captured hipGraph, launched once on `stream`.  All pointers in the
+ * arrays are device pointers; the arrays themselves are host arrays.
+ * *graph_ms (if non-null) receives the device time of the graph launch.
+ * Synchronises the stream before returning. */
+int raft_batch_device(int device, int nscans, const int* nrays, const int* nangles, const float* const* angles,
+                      const float* const* d_tomo, const float* const* d_y, float* const* d_fbp, float* const* d_em,
+                      const float* paramf, int pad, int filter, int store, int iterations, void* stream,
+                      float* graph_ms);
+
 /* KL(y || A x) per slice of the eEM objective for the given images (one
  * projection with the eEM projector; d_obj is not modified): kl/kl_excluded
  * are HOST arrays [nslices]; iterations in parami are ignored.  No

@@ -48,3 +48,26 @@ void launch_background(float* data, const float* flat, const float* dark, int R,
 }
 
 } // namespace raft
+
+namespace raft {
+
+namespace {
+
+__global__ void k_fill(float* p, long long n, float v)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+} // namespace
+
+// p[0 .. n) = v, as a kernel (usable inside captured graphs, unlike memset nodes)
+void launch_fill(float* p, long long n, float v, hipStream_t st)
+{
+    if (n <= 0) return;
+    const long long blocks = std::min<long long>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    RAFT_HIP(hipGetLastError());
+}
+
+} // namespace raft

@@ -839,6 +839,253 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
 }
 
 // ---------------------------------------------------------------------------
+// Batched uneven scans: FBP and eEM of many independent scans of different
+// sizes in ONE graph launch.  Scans with the same geometry (size, angles, FBP
+// parameters) form a bucket; each bucket is gathered into contiguous slices,
+// reconstructed with the same kernels as fbp_device / eem_device, and
+// scattered back.  All buckets are captured into one hipGraph on forked
+// streams (independent branches), then launched once.
+// ---------------------------------------------------------------------------
+struct BatchBucket {
+    std::vector<int> scans;
+    int N = 0, A = 0;
+    std::vector<float> angles, paramf;
+    // FBP
+    FbpCfg cf;
+    BpGeom gf;
+    FilterTables* F = nullptr;
+    DevBuf fcA, fcB, fwa, frows, tomo, fout;
+    // eEM
+    BpGeom ge;
+    RayGeom rg;
+    DevBuf dc, ds, ecA, ecB, ewa, erows, xil, xT, ones, y, x;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+};
+
+static void batch_device(int dev, int nscans, const int* nrays, const int* nangles, const float* const* angles,
+                         const float* const* d_tomo, const float* const* d_y, float* const* d_fbp, float* const* d_em,
+                         const float* paramf, int pad, int filter, int store, int K, hipStream_t st,
+                         float* graph_ms)
+{
+    if (nscans < 0 || !nrays || !nangles || !angles || !paramf) fail(RAFT_EINVAL, "null batch description");
+    if (store != RAFT_STORE_F32 && store != RAFT_STORE_F16) fail(RAFT_EINVAL, "store must be F32 or F16");
+    if (K < 0) fail(RAFT_EINVAL, "iterations must be >= 0");
+    const bool do_fbp = d_fbp != nullptr && d_tomo != nullptr;
+    const bool do_em = d_em != nullptr && d_y != nullptr && K > 0;
+    if (nscans == 0 || (!do_fbp && !do_em)) return;
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+
+    // buckets: same size, angles and FBP parameters
+    std::map<std::string, std::unique_ptr<BatchBucket>> bmap;
+    std::vector<BatchBucket*> buckets;
+    for (int i = 0; i < nscans; ++i) {
+        const int N = nrays[i], A = nangles[i];
+        if (N < 2 || A < 2) fail(RAFT_EINVAL, "each scan needs nrays >= 2 and nangles >= 2");
+        if (!angles[i]) fail(RAFT_EINVAL, "null angles");
+        if ((do_fbp && (!d_tomo[i] || !d_fbp[i])) || (do_em && (!d_y[i] || !d_em[i])))
+            fail(RAFT_EINVAL, "null scan pointer");
+        std::string key(reinterpret_cast<const char*>(&N), sizeof N);
+        key.append(reinterpret_cast<const char*>(&A), sizeof A);
+        key.append(reinterpret_cast<const char*>(angles[i]), sizeof(float) * A);
+        key.append(reinterpret_cast<const char*>(paramf + 7 * (size_t)i), sizeof(float) * 7);
+        auto& b = bmap[key];
+        if (!b) {
+            b.reset(new BatchBucket());
+            b->N = N;
+            b->A = A;
+            b->angles.assign(angles[i], angles[i] + A);
+            b->paramf.assign(paramf + 7 * (size_t)i, paramf + 7 * (size_t)i + 7);
+            buckets.push_back(b.get());
+        }
+        b->scans.push_back(i);
+    }
+
+    // set-up outside the capture: tables, uploads, buffers
+    for (BatchBucket* b : buckets) {
+        const int N = b->N, A = b->A, S = (int)b->scans.size();
+        const size_t img = (size_t)N * N, sino = (size_t)A * N;
+        if (do_fbp) {
+            const int pi[10] = {N, A, S, N, pad, 0, 0, filter, 0, 0};
+            b->cf = decode_fbp(b->paramf.data(), pi);
+            std::vector<float> wa;
+            fbp_geometry(b->cf, b->angles.data(), b->gf, wa);
+            b->F = &filter_tables(D, b->cf, st);
+            upload(b->fcA, b->gf.cA, st);
+            upload(b->fcB, b->gf.cB, st);
+            upload(b->fwa, wa, st);
+            const int SB = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
+            b->frows.alloc((size_t)((S + SB - 1) / SB) * A * b->gf.width * 16);
+            b->tomo.alloc((size_t)S * sino * 4);
+            b->fout.alloc((size_t)S * img * 4);
+        }
+        if (do_em) {
+            const double rr = 1.0;
+            std::vector<double> cv(A), sv(A);
+            std::vector<float> wa(A);
+            BpGeom& g = b->ge;
+            g.n = N;
+            g.nangles = A;
+            g.nbins = N;
+            g.xc = (double)(N - 1) / 2.0;
+            g.c0 = (double)(N - 1) / 2.0;
+            g.cA.resize(A);
+            g.cB.resize(A);
+            for (int a = 0; a < A; ++a) {
+                cv[a] = std::cos((double)b->angles[a]);
+                sv[a] = std::sin((double)b->angles[a]);
+                g.cA[a] = cv[a] * rr;
+                g.cB[a] = sv[a] * rr;
+                wa[a] = (float)angle_step(b->angles.data(), A, a);
+            }
+            bp_window(g);
+            if (g.tlo > 0 || g.tlo + g.width < N) {
+                const int lo = std::min(g.tlo, 0), hi = std::max(g.tlo + g.width, N);
+                g.tlo = lo;
+                g.width = hi - lo;
+            }
+            if ((double)N * (double)N * 16.0 >= 2147483648.0) fail(RAFT_EUNSUPPORTED, "image too large");
+            upload(b->dc, cv, st);
+            upload(b->ds, sv, st);
+            upload(b->ecA, g.cA, st);
+            upload(b->ecB, g.cB, st);
+            upload(b->ewa, wa, st);
+            b->rg = ray_geom_em(N, A, N);
+            const int G = (S + 3) / 4;
+            b->erows.alloc((size_t)G * A * g.width * 16);
+            b->xil.alloc((size_t)G * img * 16);
+            b->xT.alloc((size_t)G * img * 16);
+            b->ones.alloc(img * 4 + (size_t)A * g.width * 16);
+            b->y.alloc((size_t)S * sino * 4);
+            b->x.alloc((size_t)S * img * 4);
+        }
+        RAFT_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+        RAFT_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+
+    auto enqueue = [&](BatchBucket* b) {
+        hipStream_t s = b->s;
+        const int N = b->N, A = b->A, S = (int)b->scans.size();
+        const size_t img = (size_t)N * N, sino = (size_t)A * N;
+        if (do_fbp) {
+            for (int k = 0; k < S; ++k)
+                RAFT_HIP(hipMemcpyAsync(b->tomo.as<float>() + k * sino, d_tomo[b->scans[k]], sino * 4,
+                                        hipMemcpyDeviceToDevice, s));
+            FilterArgs fa;
+            fa.tomo = b->tomo.as<float>();
+            fa.q = b->frows.p;
+            fa.H = b->F->H.as<float2>();
+            fa.tw = b->F->tw.as<float2>();
+            fa.g = b->F->g.as<float>();
+            fa.wa = b->fwa.as<float>();
+            fa.N = N; fa.A = A; fa.S = S; fa.npad = b->cf.npad; fa.L = b->cf.L; fa.tlo = b->gf.tlo; fa.W = b->gf.width;
+            fa.none = (b->cf.filter == 0);
+            launch_fbp_filter(fa, b->F->plan, store, s);
+            BpArgs ba;
+            ba.q = b->frows.p; ba.out = b->fout.as<float>(); ba.x = nullptr; ba.norm = nullptr; ba.xt = nullptr;
+            ba.cA = b->fcA.as<double>(); ba.cB = b->fcB.as<double>(); ba.xc = b->gf.xc; ba.c0 = b->gf.c0;
+            ba.n = N; ba.A = A; ba.S = S; ba.tlo = b->gf.tlo; ba.W = b->gf.width; ba.mode = 0;
+            launch_backproject(ba, store, s);
+            for (int k = 0; k < S; ++k)
+                RAFT_HIP(hipMemcpyAsync(d_fbp[b->scans[k]], b->fout.as<float>() + k * img, img * 4,
+                                        hipMemcpyDeviceToDevice, s));
+        }
+        if (do_em) {
+            const BpGeom& g = b->ge;
+            for (int k = 0; k < S; ++k) {
+                RAFT_HIP(hipMemcpyAsync(b->y.as<float>() + k * sino, d_y[b->scans[k]], sino * 4,
+                                        hipMemcpyDeviceToDevice, s));
+                RAFT_HIP(hipMemcpyAsync(b->x.as<float>() + k * img, d_em[b->scans[k]], img * 4,
+                                        hipMemcpyDeviceToDevice, s));
+            }
+            float* ones = b->ones.as<float>();
+            float* onerows = ones + img;
+            launch_fill_rows(onerows, b->ewa.as<float>(), A, g.width, g.tlo, N, s);
+            BpArgs b1;
+            b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
+            b1.cA = b->ecA.as<double>(); b1.cB = b->ecB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
+            b1.n = N; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 0;
+            launch_backproject(b1, RAFT_STORE_F32, s);
+            const int G = (S + 3) / 4;
+            launch_fill(b->erows.as<float>(), (long long)G * A * g.width * 4, 0.0f, s);
+            launch_to_interleaved(b->x.as<float>(), b->xil.as<float>(), S, N, N, false, s);
+            launch_to_interleaved(b->x.as<float>(), b->xT.as<float>(), S, N, N, true, s);
+            for (int it = 0; it < K; ++it) {
+                ProjArgs p;
+                p.img = b->xil.as<float>(); p.imgT = b->xT.as<float>(); p.out = b->erows.as<float>();
+                p.y = b->y.as<float>(); p.wa = b->ewa.as<float>(); p.kl_part = nullptr; p.ex_part = nullptr;
+                p.cosv = b->dc.as<double>(); p.sinv = b->ds.as<double>();
+                p.g = b->rg; p.S = S; p.tlo = g.tlo; p.W = g.width;
+                launch_project(p, 1, s);
+                BpArgs bu;
+                bu.q = b->erows.as<float>(); bu.out = b->xil.as<float>(); bu.x = b->xil.as<float>(); bu.norm = ones;
+                bu.xt = b->xT.as<float>();
+                bu.cA = b->ecA.as<double>(); bu.cB = b->ecB.as<double>(); bu.xc = g.xc; bu.c0 = g.c0;
+                bu.n = N; bu.A = A; bu.S = S; bu.tlo = g.tlo; bu.W = g.width; bu.mode = 1;
+                launch_backproject(bu, RAFT_STORE_F32, s);
+            }
+            launch_from_interleaved(b->xil.as<float>(), b->x.as<float>(), S, N, N, s);
+            for (int k = 0; k < S; ++k)
+                RAFT_HIP(hipMemcpyAsync(d_em[b->scans[k]], b->x.as<float>() + k * img, img * 4,
+                                        hipMemcpyDeviceToDevice, s));
+        }
+    };
+
+    // capture every bucket as a branch of one graph
+    hipStream_t cs = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&] {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (fork) (void)hipEventDestroy(fork);
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+        for (BatchBucket* b : buckets) {
+            if (b->done) (void)hipEventDestroy(b->done);
+            if (b->s) (void)hipStreamDestroy(b->s);
+        }
+        if (cs) (void)hipStreamDestroy(cs);
+    };
+    try {
+        RAFT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        RAFT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        RAFT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            RAFT_HIP(hipEventRecord(fork, cs));
+            for (BatchBucket* b : buckets) {
+                RAFT_HIP(hipStreamWaitEvent(b->s, fork, 0));
+                enqueue(b);
+                RAFT_HIP(hipEventRecord(b->done, b->s));
+                RAFT_HIP(hipStreamWaitEvent(cs, b->done, 0));
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(cs, &graph);
+            throw;
+        }
+        RAFT_HIP(hipStreamEndCapture(cs, &graph));
+        RAFT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        if (graph_ms) {
+            RAFT_HIP(hipEventCreate(&t0));
+            RAFT_HIP(hipEventCreate(&t1));
+            RAFT_HIP(hipEventRecord(t0, st));
+        }
+        RAFT_HIP(hipGraphLaunch(exec, st));
+        if (graph_ms) RAFT_HIP(hipEventRecord(t1, st));
+        RAFT_HIP(hipStreamSynchronize(st));
+        if (graph_ms) RAFT_HIP(hipEventElapsedTime(graph_ms, t0, t1));
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+// ---------------------------------------------------------------------------
 // host-pointer wrappers: slab split over gpus[], one host thread per device
 // ---------------------------------------------------------------------------
 static void check_gpus(const int* gpus, int ngpus)
@@ -1193,6 +1440,17 @@ int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* 
 {
     return guarded([&] {
         eem_device(device, d_obj, d_tomo, angles, parami, kl, kl_excluded, static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_batch_device(int device, int nscans, const int* nrays, const int* nangles, const float* const* angles,
+                      const float* const* d_tomo, const float* const* d_y, float* const* d_fbp, float* const* d_em,
+                      const float* paramf, int pad, int filter, int store, int iterations, void* stream,
+                      float* graph_ms)
+{
+    return guarded([&] {
+        batch_device(device, nscans, nrays, nangles, angles, d_tomo, d_y, d_fbp, d_em, paramf, pad, filter, store,
+                     iterations, static_cast<hipStream_t>(stream), graph_ms);
     });
 }
 

@@ -175,6 +175,9 @@ void launch_counter(int* c, int op, hipStream_t st);
 void launch_background(float* data, const float* flat, const float* dark, int R, int A, int S, int nflats, int is_log,
                        hipStream_t st);
 
+// p[0 .. n) = v (a kernel, so it may be captured into a graph)
+void launch_fill(float* p, long long n, float v, hipStream_t st);
+
 // profiling hooks (events around kernel families)
 enum KernelFamily { KF_FILTER = 0, KF_BP = 1, KF_PROJ = 2, KF_EMBP = 3, KF_COUNT = 4 };
 struct ProfScope {

@@ -93,6 +93,39 @@ def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None):
     return obj, None, None
 
 
+def batch_device(tomos, angles, paramf, pad=2, filter_type=7, store="f16", fbp_out=None, ys=None, em=None,
+                 iterations=0, stream=None):
+    """Batched uneven scans in one graph launch (raft_batch_device).
+
+    tomos: list of CUDA float32 [A_i, N_i] sinograms (FBP input); angles: list of
+    float32 arrays; paramf: [n, 7] FBP float parameters per scan; fbp_out: list of
+    [N_i, N_i] outputs (None: skip FBP); ys / em: eEM data and in/out iterates
+    (None: skip eEM).  Returns the device time (ms) of the graph launch."""
+    n = len(tomos) if tomos is not None else len(ys)
+    nr = numpy.array([(tomos[i] if tomos is not None else ys[i]).shape[-1] for i in range(n)], numpy.int32)
+    na = numpy.array([(tomos[i] if tomos is not None else ys[i]).shape[-2] for i in range(n)], numpy.int32)
+    angs = [CNICE(numpy.asarray(a)) for a in angles]
+    pa = (ctypes.c_void_p * n)(*[a.ctypes.data for a in angs])
+
+    def ptrs(lst):
+        if lst is None:
+            return None
+        for t in lst:
+            _check_tensor(t, "scan buffer")
+        return (ctypes.c_void_p * n)(*[t.data_ptr() for t in lst])
+
+    pf = CNICE(numpy.asarray(paramf, numpy.float32).reshape(n, 7))
+    ms = ctypes.c_float(0.0)
+    ref = tomos[0] if tomos is not None else ys[0]
+    rc = libraft.raft_batch_device(ref.device.index or 0, n, nr.ctypes.data_as(ctypes.c_void_p),
+                                   na.ctypes.data_as(ctypes.c_void_p), pa, ptrs(tomos), ptrs(ys), ptrs(fbp_out),
+                                   ptrs(em), pf.ctypes.data_as(ctypes.c_void_p), pad, filter_type,
+                                   1 if store == "f16" else 0, iterations, _stream(ref, stream), ctypes.byref(ms))
+    if rc:
+        raft_check("raft_batch_device")
+    return ms.value
+
+
 def lowpass_device(tomo, paramf, parami, stream=None):
     """Filter-only export on device data, in place on tomo [S, A, N]; returns tomo."""
     _check_tensor(tomo, "tomo")

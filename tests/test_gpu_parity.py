@@ -346,3 +346,34 @@ def test_background_correction(gpu, nflats, is_log):
     out2 = sscRaft.correct_projections(np.swapaxes(frames, 0, 1).copy(), np.swapaxes(flat, 0, 1).copy(),
                                        dark[None].copy(), {"gpu": [0], "uselog": is_log})
     np.testing.assert_array_equal(out2, out)
+
+
+def test_batch_uneven_scans_match_single_calls(gpu):
+    """raft_batch_device (one graph, buckets by size) gives exactly what the
+    single-volume entry points give for each scan: FBP (FP16 rows) and eEM."""
+    import torch
+    from sscRaft import device as dv
+
+    sizes = [32, 48, 32, 64, 48, 32]
+    K = 6
+    tomos, ys, angs, pfs, fouts, ems = [], [], [], [], [], []
+    for k, n in enumerate(sizes):
+        img = ph.phantoms(n, 1, config=6, first=k)
+        ang = ph.uniform_angles(n)
+        p = ph.poisson_log(oracle.radon(img, ang), 10.0 ** (3 + 2 * k / len(sizes)), seed=k)[0]
+        tomos.append(torch.from_numpy(p).cuda())
+        ys.append(torch.from_numpy(np.maximum(p, 0)).cuda())
+        angs.append(ang)
+        pfs.append([0, 0, 1, 0, 2.0 / n, 2.0 / n, 0])
+        fouts.append(torch.zeros((n, n), device="cuda"))
+        ems.append(torch.ones((n, n), device="cuda"))
+    ms = dv.batch_device(tomos, angs, pfs, pad=2, filter_type=7, store="f16", fbp_out=fouts, ys=ys, em=ems,
+                         iterations=K)
+    print(f"batch of {len(sizes)} scans: graph {ms:.3f} ms")
+    for k, n in enumerate(sizes):
+        pf = np.array(pfs[k], np.float32)
+        pi = np.array([n, n, 1, n, 2, 0, 0, 7, 0, 0], np.int32)
+        ref = dv.fbp_device(tomos[k][None].contiguous(), angs[k], pf, pi, store="f16")
+        np.testing.assert_array_equal(fouts[k].cpu().numpy(), ref[0].cpu().numpy())
+        x, _, _ = dv.eem_device(ys[k][None].contiguous(), angs[k], K, with_kl=False)
+        np.testing.assert_array_equal(ems[k].cpu().numpy(), x[0].cpu().numpy())
