@@ -17,7 +17,11 @@ Also reported on the same JSON line:
                 uniform start, 200 iterations, FP64 KL per iteration);
   em.learned_init  time-to-KL of the U-Net warm start vs plain eEM on held-out
                 512^2 slices (128 angles, I0 = 1e4), with the trained weights
-                in ssc-raft_amd/sscRaft/weights/ (null if they are absent).
+                in ssc-raft_amd/sscRaft/weights/ (null if they are absent);
+  c1            one 512^2 slice (512 angles): GPU call time and the FP32 host
+                restatement's time;
+  batch         256 uneven scans (N in {256..1024}): FBP + 200 eEM iterations
+                of all of them in one captured graph launch.
 Parity of the timed paths against the FP64 oracle is the job of tests/ (the
 2048^2 FBP case is tests/test_gpu_parity.py::test_fbp_2048); the oracle is
 used here only for the CPU baseline.
@@ -186,6 +190,84 @@ def em_bench(args, torch, dv, ph):
     return r
 
 
+def c1_bench(args, torch, dv, ph):
+    """One 512^2 slice, 512 angles, pad 2, ramp: the CPU-runnable case, timed on
+    the GPU (one call, data resident) and with the FP32 restatement on the host."""
+    from sscRaft.geometries.parallel.fbp_methods import fbp_params
+
+    N = A = 512
+    ang = ph.uniform_angles(A)
+    img = ph.phantoms(N, 1, config=1, device=torch.device("cuda:0"))
+    y = ph.poisson_log_torch(dv.radon_device(img, ang), 1.0e4, seed=1000)
+    pf, pi = fbp_params(N, A, 1, fbp_dic(N))
+    out = torch.empty((1, N, N), dtype=torch.float32, device=y.device)
+    for _ in range(3):
+        dv.fbp_device(y, ang, pf, pi, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        dv.fbp_device(y, ang, pf, pi, out=out)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / 10
+    r = {"config": "1 slice 512^2, 512 angles, pad 2, ramp, I0=1e4", "gpu_seconds": gpu_s}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        ys = y.cpu().numpy()
+        cores = host_cores()
+        t0 = time.perf_counter()
+        oracle.fbp(ys, ang, pf, pi, prec="f32", nthreads=cores)
+        r["cpu_seconds"] = time.perf_counter() - t0
+        r["cpu_cores"] = cores
+    log(f"C1: GPU {gpu_s * 1e3:.2f} ms per slice" + (f", CPU {r['cpu_seconds']:.3f} s on {r['cpu_cores']} threads"
+                                                     if "cpu_seconds" in r else ""))
+    return r
+
+
+def batch_bench(args, torch, dv, ph):
+    """256 independent scans, N from {256, 384, 512, 768, 1024} (angles = bins = N),
+    I0 log-uniform in [1e3, 1e5]; FBP (pad 2, ramp, FP16 rows) + 200 eEM
+    iterations of every scan in one captured graph launch."""
+    import numpy as np
+
+    rng = np.random.default_rng(4000)
+    sizes = rng.choice([256, 384, 512, 768, 1024], size=args.batch_scans)
+    i0s = 10.0 ** rng.uniform(3.0, 5.0, size=args.batch_scans)
+    dev = torch.device("cuda:0")
+    tomos, ys, angs, pfs, fouts, ems = [], [], [], [], [], []
+    t0 = time.perf_counter()
+    for k, n in enumerate(sizes):
+        n = int(n)
+        ang = ph.uniform_angles(n)
+        img = ph.phantoms(n, 1, config=4, first=k, device=dev)
+        p = ph.poisson_log_torch(dv.radon_device(img, ang), float(i0s[k]), seed=4000 + k)[0].contiguous()
+        tomos.append(p)
+        ys.append(torch.clamp(p, min=0.0).contiguous())
+        angs.append(ang)
+        pfs.append([0, 0, 1, 0, 2.0 / n, 2.0 / n, 0])
+        fouts.append(torch.empty((n, n), dtype=torch.float32, device=dev))
+        ems.append(torch.ones((n, n), dtype=torch.float32, device=dev))
+    torch.cuda.synchronize()
+    log(f"C4 inputs: {len(sizes)} scans generated in {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = dv.batch_device(tomos, angs, pfs, pad=2, filter_type=7, store="f16", fbp_out=fouts, ys=ys, em=ems,
+                         iterations=args.em_iters)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n3 = float(np.sum(sizes.astype(np.float64) ** 3))
+    r = {"config": f"{len(sizes)} scans, N in {{256,384,512,768,1024}}, angles = bins = N, I0 log-uniform "
+                   f"[1e3, 1e5], FBP pad 2 ramp FP16 rows + {args.em_iters} eEM iterations, one graph launch",
+         "scans": int(len(sizes)), "sizes_count": {int(v): int((sizes == v).sum()) for v in np.unique(sizes)},
+         "graph_seconds": ms / 1e3, "wall_seconds": wall, "scans_per_s": len(sizes) / wall,
+         "em_samples_per_s": 2.0 * n3 * args.em_iters / (ms / 1e3),
+         "finite": bool(all(bool(torch.isfinite(f).all()) for f in fouts))}
+    log(f"C4: {len(sizes)} scans in {wall:.2f} s (graph {ms / 1e3:.2f} s), eEM "
+        f"{r['em_samples_per_s']:.3e} samples/s")
+    return r
+
+
 def learned_bench(args):
     """Time-to-KL of the U-Net warm start vs plain eEM (sscRaft.learned.evaluate)."""
     from sscRaft import learned
@@ -209,12 +291,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--learned-slices", type=int, default=512)
+    ap.add_argument("--batch-scans", type=int, default=256)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-em", action="store_true")
     ap.add_argument("--em-iters", type=int, default=200)
-    ap.add_argument("--only", choices=["all", "fbp", "em"], default="all",
+    ap.add_argument("--only", choices=["all", "fbp", "em", "batch"], default="all",
                     help="profiling runs: time only one workload (the JSON line keeps the same fields)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of the 64 x 2048 x 2048 sinogram (known byte count) so "
@@ -234,6 +318,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.only == "batch":
+        print(json.dumps({"metric": METRIC, "batch": batch_bench(args, torch, dv, ph)}), flush=True)
+        return
     if args.only == "em":
         em = em_bench(args, torch, dv, ph)
         if not args.no_learned:
@@ -244,6 +331,8 @@ def main():
     em = None if (args.no_em or args.only == "fbp") else em_bench(args, torch, dv, ph)
     if em is not None and not args.no_learned:
         em["learned_init"] = learned_bench(args)
+    c1 = None if args.only == "fbp" else c1_bench(args, torch, dv, ph)
+    batch = None if (args.no_batch or args.only == "fbp") else batch_bench(args, torch, dv, ph)
     r32 = res["f32"]
     line = {
         "metric": METRIC,
@@ -267,6 +356,8 @@ def main():
                       "filter_ms": res["f16"]["filter_ms"]},
         "filter_ms": r32["filter_ms"],
         "em": em,
+        "c1": c1,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

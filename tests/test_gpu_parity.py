@@ -312,7 +312,7 @@ def test_tem_dispatch_and_device_entry(gpu):
     ang = ph.uniform_angles(a)
     data = ph.poisson_log(oracle.radon(img, ang), 1e4, seed=3)
     x1 = sscRaft.em(data, angles=ang, dic={"gpu": [0], "method": "tEMRT", "iterations": k})  # flat = ones
-    count = np.exp(-data.astype(np.float64)).astype(np.float32)
+    count = np.exp(-data.astype(np.float32))  # what tEMRT_GPU_ passes to the C function
     xr = oracle.tem(count, np.ones((2, n), np.float32), ang, k)
     assert rel(x1, xr) <= 1e-5 * k
     xd = torch.ones((2, n, n), device="cuda")
@@ -339,7 +339,8 @@ def test_background_correction(gpu, nflats, is_log):
     frames[0, 0, :5] = 0.0  # below dark: the 0.5 floor
     ref = oracle.background(frames, flat, dark, is_log)
     out = sscRaft.correct_background(frames.copy(), flat, dark, [0], is_log)
-    e = float(np.max(np.abs(out - ref) / np.maximum(np.abs(ref), 1e-3)))
+    # relative error, but absolute near 0 (-log of values close to 1)
+    e = float(np.max(np.abs(out - ref) / np.maximum(np.abs(ref), 1.0 if is_log else 1e-3)))
     print(f"background nflats={nflats} log={is_log}: max rel err {e:.2e}")
     assert e <= 2e-6
     # detector-order front end: frames [angles, slices, rays], flat [nflats, slices, rays], dark [1, slices, rays]
@@ -349,8 +350,9 @@ def test_background_correction(gpu, nflats, is_log):
 
 
 def test_batch_uneven_scans_match_single_calls(gpu):
-    """raft_batch_device (one graph, buckets by size) gives exactly what the
-    single-volume entry points give for each scan: FBP (FP16 rows) and eEM."""
+    """raft_batch_device (one graph, buckets by size) gives what the
+    single-volume entry points give for each scan: eEM bit for bit, FBP (FP16
+    rows) to rounding."""
     import torch
     from sscRaft import device as dv
 
@@ -374,6 +376,8 @@ def test_batch_uneven_scans_match_single_calls(gpu):
         pf = np.array(pfs[k], np.float32)
         pi = np.array([n, n, 1, n, 2, 0, 0, 7, 0, 0], np.int32)
         ref = dv.fbp_device(tomos[k][None].contiguous(), angs[k], pf, pi, store="f16")
-        np.testing.assert_array_equal(fouts[k].cpu().numpy(), ref[0].cpu().numpy())
+        # the filter transforms two rows as one complex FFT; in a batch a row's
+        # partner is another scan's row instead of a zero row, so rounding differs
+        assert rel(fouts[k].cpu().numpy(), ref[0].cpu().numpy()) <= 1e-4  # FP16 rows: one f16 rounding may flip
         x, _, _ = dv.eem_device(ys[k][None].contiguous(), angs[k], K, with_kl=False)
         np.testing.assert_array_equal(ems[k].cpu().numpy(), x[0].cpu().numpy())
