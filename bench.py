@@ -21,7 +21,9 @@ Also reported on the same JSON line:
   c1            one 512^2 slice (512 angles): GPU call time and the FP32 host
                 restatement's time;
   batch         256 uneven scans (N in {256..1024}): FBP + 200 eEM iterations
-                of all of them in one captured graph launch.
+                of all of them in one captured graph launch;
+  unet          the warm start U-Net's forward / training throughput against
+                dense BF16 MFMA (2.5 PFLOP/s).
 Parity of the timed paths against the FP64 oracle is the job of tests/ (the
 2048^2 FBP case is tests/test_gpu_parity.py::test_fbp_2048); the oracle is
 used here only for the CPU baseline.
@@ -268,6 +270,46 @@ def batch_bench(args, torch, dv, ph):
     return r
 
 
+def unet_bench(args, torch):
+    """U-Net forward and training step on 512^2 images (random weights and data):
+    FLOP of the 3x3 convolutions / time, against dense BF16 MFMA."""
+    from sscRaft.nn import UNet
+
+    net = UNet(seed=0)
+    r = {}
+    for B in (8, 32):
+        x = torch.rand((B, 512, 512), device="cuda")
+        out = torch.empty_like(x)
+        for _ in range(3):
+            net.forward(x, out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            net.forward(x, out=out)
+        torch.cuda.synchronize()
+        tf = (time.perf_counter() - t0) / 10
+        r[f"forward_b{B}_ms"] = tf * 1e3
+        r[f"forward_b{B}_tflops"] = net.flops_forward(B, 512, 512) / tf / 1e12
+    x = torch.rand((8, 512, 512), device="cuda")
+    t = torch.rand((8, 512, 512), device="cuda")
+    for _ in range(2):
+        net.train_step(x, t, lr=1e-5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        net.train_step(x, t, lr=1e-5)
+    torch.cuda.synchronize()
+    tt = (time.perf_counter() - t0) / 5
+    r["train_step_b8_ms"] = tt * 1e3
+    r["train_step_b8_tflops"] = 3 * net.flops_forward(8, 512, 512) / tt / 1e12
+    r["roofline"] = {"bound": "mfma", "achieved": r["forward_b32_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": r["forward_b32_tflops"] / 2500.0,
+                     "note": "forward pass, batch 32, FLOP = 2*9*Cin*Cout per output pixel over the 14 layers"}
+    log(f"U-Net 512^2: forward {r['forward_b32_tflops']:.0f} TFLOP/s (batch 32), training step "
+        f"{r['train_step_b8_tflops']:.0f} TFLOP/s (batch 8)")
+    return r
+
+
 def learned_bench(args):
     """Time-to-KL of the U-Net warm start vs plain eEM (sscRaft.learned.evaluate)."""
     from sscRaft import learned
@@ -332,6 +374,7 @@ def main():
     if em is not None and not args.no_learned:
         em["learned_init"] = learned_bench(args)
     c1 = None if args.only == "fbp" else c1_bench(args, torch, dv, ph)
+    unet = None if args.only == "fbp" else unet_bench(args, torch)
     batch = None if (args.no_batch or args.only == "fbp") else batch_bench(args, torch, dv, ph)
     r32 = res["f32"]
     line = {
@@ -357,6 +400,7 @@ def main():
         "filter_ms": r32["filter_ms"],
         "em": em,
         "c1": c1,
+        "unet": unet,
         "batch": batch,
     }
     print(json.dumps(line), flush=True)

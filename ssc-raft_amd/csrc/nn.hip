@@ -219,7 +219,7 @@ struct ConvPArgs {
     int tiles_x, tiles_y, total_tiles, tiles_per_block;
 };
 
-template <int WR, int WC>
+template <int WR, int WC, bool UNROLL_DX>
 __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
 {
     using K = ConvP<WR, WC>;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
         store_item(it == 0 || nch > 1);
         __syncthreads();
         if (it + 1 < items) load_item(it + 1, nch > 1);
-#pragma unroll
+#pragma unroll(UNROLL_DX ? 3 : 1)
         for (int dx = 0; dx < 3; ++dx) {
 #pragma unroll
             for (int kk = 0; kk < 32; kk += 16) {
@@ -876,13 +876,13 @@ int launch_conv(const ConvArgs& a, hipStream_t st)
     return launched();
 }
 
-template <int WR, int WC>
+template <int WR, int WC, bool UNROLL_DX = true>
 int launch_convp(const ConvArgs& a, hipStream_t st)
 {
     using K = ConvP<WR, WC>;
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC>),
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, K::LDS) != hipSuccess)
             return RAFT_EHIP;
         attr = true;
@@ -898,7 +898,7 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     long long per = (total * ny + 1023) / 1024;
     pa.tiles_per_block = (int)std::max(1LL, std::min(per, 16LL));
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
-    hipLaunchKernelGGL((k_conv3x3p<WR, WC>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
+    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
     return launched();
 }
 
@@ -940,9 +940,10 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
         ldy % 4 || cy0 % 4 || cx0 + Cin > ldx || cy0 + Cout > ldy)
         return RAFT_EINVAL;
     ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu};
-    // measured on gfx950: the persistent (2,1) form wins when one 32-channel chunk
-    // is the whole input; with more chunks the one-tile (2,2) / (4,1) forms do
-    if (variant == 0) variant = Cin == 32 ? 9 : (Cout % 64 == 0 ? 1 : 2);
+    // measured on gfx950 (batch 8, 512^2 .. 64^2 layers): the persistent 16x32
+    // form is fastest or within 2% everywhere except the wide-input 64-channel
+    // decoder layer, where the one-tile 8x64 form wins
+    if (variant == 0) variant = (Cout % 64 == 0 && Cin >= 192 && Cout < 128) ? 1 : 6;
     switch (variant) {
     case 6:
         return launch_convp<4, 1>(a, S(stream));
@@ -954,6 +955,16 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
         return launch_convp<4, 2>(a, S(stream));
     case 9:
         return launch_convp<2, 1>(a, S(stream));
+    case 10:
+        return launch_convp<4, 1, false>(a, S(stream));
+    case 11:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_convp<2, 2, false>(a, S(stream));
+    case 12:
+        if (Cout % 64) return RAFT_EINVAL;
+        return launch_convp<4, 2, false>(a, S(stream));
+    case 13:
+        return launch_convp<2, 1, false>(a, S(stream));
     case 1:
         if (Cout % 64) return RAFT_EINVAL;
         return launch_conv<2, 2>(a, S(stream));
