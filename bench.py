@@ -277,6 +277,9 @@ def unet_bench(args, torch):
 
     net = UNet(seed=0)
     r = {}
+    tuned = net.autotune(32, 512, 512)
+    net.autotune(8, 512, 512, backward=True)
+    r["tuned_variants_b32"] = {f"{k[3]}->{k[4]}@{k[1]}": v[0] for k, v in tuned.items()}
     for B in (8, 32):
         x = torch.rand((B, 512, 512), device="cuda")
         out = torch.empty_like(x)

