@@ -103,6 +103,21 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* FBP by the Backprojection Slice Theorem (replaces getBSTMultiGPU,
+ * cuda/src/geometries/parallel/reconstruction_methods/bst.cu:674-736, bound in
+ * rafttypes.py, called from sscRaft/geometries/parallel/fbp_methods.py:200):
+ * obj [nslices][nrays][nrays] out, tomogram [nslices][nangles][nrays]; paramf /
+ * parami as getFBPMultiGPU (the axis offset is used as an integer shift, as the
+ * reference's Python front end passes int(offset)).  Angles are assumed uniform
+ * on [0, pi) (the array is not read, as in the reference); nstreams is ignored.
+ * Padded size n = nrays (1 + pad) must factor into 2, 3, 5 and be <= 16384. */
+void getBSTMultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf, int* parami,
+                    int nstreams);
+
+/* Same on one device (device pointers, stream-ordered, synchronises before returning). */
+int raft_bst_device(int device, float* d_obj, const float* d_tomo, const float* paramf, const int* parami,
+                    void* stream);
+
 /* Flat/dark background correction (replaces getBackgroundCorrectionMultiGPU,
  * cuda/src/processing/background_correction.cu:144-178, called from
  * sscRaft/processing/background_correction.py:105-108): in place on host

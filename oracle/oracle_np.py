@@ -200,3 +200,61 @@ def eem(tomo, angles, iterations, obj=None):
             r = np.maximum(r, 0.0)
             x[s] = x[s] * em_backproject(r, angles, R) / back1
     return x, kls
+
+
+def bst(tomo, paramf, parami):
+    """getBSTMultiGPU restated in float64 (bst.cu:22-140,251-312,536-736,
+    filters.cu:295-330,369-390, padding.cu:245-268) for tomo [S, A, N]:
+    pad (edge replicate, L = pad N / 2) -> optional filter C2C with
+    apply(min(k, n-k)/n) exp(-2 pi i c k / n), c = int(offset), unnormalised
+    inverse, real part -> mirrored half-rows (weights 4, 2 at index 0, upper
+    half zero) -> FFT -> sigma_k = 2 / (n max(min(k, n-k), 1/2)) -> polar to
+    Cartesian gather (the reference's weights tfrac = iarc - angle) with the
+    checkerboard sign at the shifted index -> unnormalised 2-D inverse FFT ->
+    Re / (4 n pixel), cropped at L.  Angles are assumed uniform on [0, pi)."""
+    c = decode_fbp(paramf, parami)
+    tomo = np.asarray(tomo, dtype=np.float32).reshape(c["nslices"], c["nangles"], c["nrays"])
+    S, A, N = tomo.shape
+    n, L = c["npad"], c["L"]
+    u = pad_rows(tomo, n, L)  # [S, A, n] float64
+    if c["ftype"] != 0:
+        k = np.arange(n)
+        w = np.minimum(k, n - k) / n
+        H = filter_apply(c["ftype"], w, c["reg"], c["pag"]) * np.exp(-2j * np.pi * int(c["offset"]) * k / n)
+        u = np.real(np.fft.ifft(np.fft.fft(u, axis=-1) * H, axis=-1) * n)
+    half = n // 2
+    idx = np.arange(half)
+    wts = np.where(idx == 0, 2.0, 4.0)
+    P = np.zeros((S, 2 * A, n), dtype=np.complex128)
+    P[:, :A, :half] = u[:, :, half - 1 - idx] * wts
+    P[:, A:, :half] = u[:, :, half + idx] * wts
+    P = np.fft.fft(P, axis=-1)
+    kk = np.arange(n)
+    P *= 2.0 / (n * np.maximum(np.minimum(kk, n - kk), 0.5))
+    s = n
+    A2 = 2 * A
+    tx = np.arange(s)[None, :]
+    ty = np.arange(s)[:, None]
+    posx, posy = tx - s // 2, ty - s // 2
+    rho = n * np.hypot(posx, posy) / s
+    angle = A2 * (0.5 * np.arctan2(posy, posx) / np.pi + 0.5)
+    irho = np.floor(rho).astype(np.int64)
+    iarc = np.floor(angle).astype(np.int64)
+    ok = irho < n // 2 - 1
+    pfrac = rho - irho
+    tfrac = iarc - angle
+    ia = iarc % A2
+    ua = (ia + 1) % A2
+    ir = np.where(ok, irho, 0)
+    out = np.zeros((S, N, N))
+    sign = np.where((tx % 2) == (ty % 2), 1.0, -1.0)
+    for sl in range(S):
+        Q = P[sl]
+        i0 = Q[ia, ir] * (1 - pfrac) + Q[ia, ir + 1] * pfrac
+        i1 = Q[ua, ir] * (1 - pfrac) + Q[ua, ir + 1] * pfrac
+        val = np.where(ok, i0 * tfrac + i1 * (1 - tfrac), 0.0) * sign
+        C = np.zeros((s, s), dtype=np.complex128)
+        C[(ty + s // 2) % s, (tx + s // 2) % s] = val
+        img = np.real(np.fft.ifft2(C)) * (s * s) / (4.0 * n * c["pixel"])
+        out[sl] = img[L:L + N, L:L + N]
+    return out

@@ -354,6 +354,164 @@ void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t
     }
 }
 
+// ---------------------------------------------------------------------------
+// FBP by the Backprojection Slice Theorem (bst.cu:22-140, 251-312): mirrored
+// half-rows of the filtered, padded projections -> C2C FFT -> sigma weights ->
+// polar-to-Cartesian gather on the frequency grid -> 2-D inverse FFT.
+// ---------------------------------------------------------------------------
+struct BstPolarK {
+    const float* q; // filtered padded rows, interleaved [G][A][n][4]
+    float2* polar;  // [S][2A][n]
+    const float2* tw;
+    int n, A, S;
+};
+
+// row r of slice sl: r < A -> q[sl][r][n/2 - 1 - idx], r >= A -> q[sl][r - A][n/2 + idx],
+// idx < n/2, weight 4 (2 at idx 0), zeros for idx >= n/2; forward FFT; times
+// sigma_k = 2 / (n max(min(k, n - k), 1/2))
+template <int THREADS, int E>
+__global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
+{
+    extern __shared__ float2 z[];
+    float2* tws = z + lds_elems(a.n);
+    const int ntw = 128 + a.n / 128 + 1;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
+    const Tw tw{tws, tws + 128};
+    const int sl = blockIdx.x / (2 * a.A), r = blockIdx.x % (2 * a.A);
+    const int ang = r % a.A, right = r / a.A;
+    const int center = a.n / 2;
+    const float* row = a.q + ((size_t)(sl / 4) * a.A + ang) * a.n * 4 + (sl % 4);
+    for (int idx = threadIdx.x; idx < a.n; idx += THREADS) {
+        float v = 0.0f;
+        if (idx < a.n / 2) {
+            const int m = right ? center + idx : center - 1 - idx;
+            v = row[(size_t)m * 4] * (idx == 0 ? 2.0f : 4.0f);
+        }
+        z[pix(idx)] = make_float2(v, 0.0f);
+    }
+    __syncthreads();
+    fft_run<THREADS, E>(z, tw, p);
+    float2* out = a.polar + ((size_t)sl * 2 * a.A + r) * a.n;
+    for (int k = threadIdx.x; k < a.n; k += THREADS) {
+        const float sig = 2.0f / ((float)a.n * fmaxf(fminf((float)k, (float)(a.n - k)), 0.5f));
+        const float2 v = z[pix(k)];
+        out[k] = make_float2(v.x * sig, v.y * sig);
+    }
+}
+
+// unnormalised inverse C2C FFT of `rows` rows of length n, in place
+template <int THREADS, int E>
+__global__ __launch_bounds__(THREADS) void k_ifft_rows(float2* data, const float2* twg, FftK p)
+{
+    extern __shared__ float2 z[];
+    const int n = p.n;
+    float2* tws = z + lds_elems(n);
+    const int ntw = 128 + n / 128 + 1;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = twg[k];
+    const Tw tw{tws, tws + 128};
+    float2* row = data + (size_t)blockIdx.x * n;
+    for (int k = threadIdx.x; k < n; k += THREADS) {
+        const float2 v = row[k];
+        z[pix(k)] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    fft_run<THREADS, E>(z, tw, p);
+    for (int k = threadIdx.x; k < n; k += THREADS) {
+        const float2 v = z[pix(k)];
+        row[k] = make_float2(v.x, -v.y);
+    }
+}
+
+// polar (2A rows x n radii) -> Cartesian frequency grid s x s (bst.cu:63-99),
+// index arithmetic in FP64; the reference's weights (tfrac, 1 - tfrac) with
+// tfrac = iarc - angle are kept as they are
+__global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int s, int S)
+{
+    const long long total = (long long)S * s * s;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int tx = (int)(e % s), ty = (int)((e / s) % s);
+        const long long sl = e / ((long long)s * s);
+        const int posx = tx - s / 2, posy = ty - s / 2;
+        const double rho = (double)n * hypot((double)posx, (double)posy) / (double)s;
+        const double angle = (double)A2 * (0.5 * atan2((double)posy, (double)posx) / M_PI + 0.5);
+        const long long irho = (long long)rho;
+        int iarc = (int)angle;
+        float2 val = make_float2(0.0f, 0.0f);
+        if (irho < n / 2 - 1) {
+            const float pfrac = (float)(rho - (double)irho);
+            const float tfrac = (float)((double)iarc - angle);
+            iarc = iarc % A2;
+            const int uarc = (iarc + 1) % A2;
+            const float2* P = polar + sl * (long long)A2 * n;
+            const float2 a0 = P[(size_t)iarc * n + irho], a1 = P[(size_t)iarc * n + irho + 1];
+            const float2 b0 = P[(size_t)uarc * n + irho], b1 = P[(size_t)uarc * n + irho + 1];
+            const float2 i0 = make_float2(a0.x * (1.0f - pfrac) + a1.x * pfrac, a0.y * (1.0f - pfrac) + a1.y * pfrac);
+            const float2 i1 = make_float2(b0.x * (1.0f - pfrac) + b1.x * pfrac, b0.y * (1.0f - pfrac) + b1.y * pfrac);
+            val = make_float2(i0.x * tfrac + i1.x * (1.0f - tfrac), i0.y * tfrac + i1.y * (1.0f - tfrac));
+        }
+        const float sg = ((tx & 1) == (ty & 1)) ? 1.0f : -1.0f; // 4 (tx%2 - 1/2)(ty%2 - 1/2)
+        cart[(sl * s + (ty + s / 2) % s) * (long long)s + (tx + s / 2) % s] = make_float2(val.x * sg, val.y * sg);
+    }
+}
+
+// out[sl][x][y] = in[sl][y][x] for s x s complex planes (32 x 32 LDS tiles)
+__global__ __launch_bounds__(256) void k_transpose_c(const float2* in, float2* out, int s)
+{
+    __shared__ float2 t[32][33];
+    const int sl = blockIdx.z;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const float2* src = in + (size_t)sl * s * s;
+    float2* dst = out + (size_t)sl * s * s;
+    for (int k = threadIdx.x; k < 32 * 32; k += 256) {
+        const int dy = k / 32, dx = k % 32;
+        if (y0 + dy < s && x0 + dx < s) t[dy][dx] = src[(size_t)(y0 + dy) * s + x0 + dx];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 32 * 32; k += 256) {
+        const int dx = k / 32, dy = k % 32;
+        if (y0 + dy < s && x0 + dx < s) dst[(size_t)(x0 + dx) * s + y0 + dy] = t[dy][dx];
+    }
+}
+
+// obj[sl][j][i] = Re(R[sl][i + L][j + L]) / scale, R the transposed 2-D inverse FFT
+__global__ void k_bst_extract(const float2* R, float* obj, int s, int N, int L, float scale, int S)
+{
+    const long long total = (long long)S * N * N;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % N), j = (int)((e / N) % N);
+        const long long sl = e / ((long long)N * N);
+        obj[e] = R[(sl * s + i + L) * (long long)s + j + L].x / scale;
+    }
+}
+
+template <int THREADS, int E>
+void launch_bst_polar_t(const BstPolarK& k, const FftK& p, size_t lds, hipStream_t st)
+{
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_polar<THREADS, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_bst_polar<THREADS, E>), dim3((unsigned)((long long)k.S * 2 * k.A)), dim3(THREADS), lds, st,
+                       k, p);
+}
+
+template <int THREADS, int E>
+void launch_ifft_t(float2* data, const float2* tw, const FftK& p, long long rows, size_t lds, hipStream_t st)
+{
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ifft_rows<THREADS, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_ifft_rows<THREADS, E>), dim3((unsigned)rows), dim3(THREADS), lds, st, data, tw, p);
+}
+
+FftK fftk_of(const FftPlan& plan)
+{
+    FftK p;
+    p.n = plan.n;
+    p.nstages = plan.nstages;
+    for (int i = 0; i < 24; ++i) p.radix[i] = plan.radix[i];
+    return p;
+}
+
 } // namespace
 
 FftPlan fft_plan(int n)
@@ -410,6 +568,57 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
             hipLaunchKernelGGL((k_filter_direct<float>), dim3((int)blocks), dim3(256), lds, st, k);
         }
     }
+    RAFT_HIP(hipGetLastError());
+}
+
+void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
+{
+    if (a.S <= 0) return;
+    if (!plan.ok) fail(RAFT_EUNSUPPORTED, "BST needs a padded size n = N(1+pad) <= 16384 with factors 2, 3, 5");
+    const int n = plan.n, s = n;
+    const FftK p = fftk_of(plan);
+    const size_t lds = sizeof(float2) * (lds_elems(n) + (size_t)(128 + n / 128 + 1));
+    if ((long long)a.S * 2 * a.A > 0x7fffffffLL || (long long)a.S * s > 0x7fffffffLL)
+        fail(RAFT_EUNSUPPORTED, "BST block too large for one launch");
+    BstPolarK k{a.q, a.polar, a.tw, n, a.A, a.S};
+    if (n > 8192) launch_bst_polar_t<1024, 16>(k, p, lds, st);
+    else {
+        switch (plan.threads) {
+        case 64: launch_bst_polar_t<64, 8>(k, p, lds, st); break;
+        case 128: launch_bst_polar_t<128, 8>(k, p, lds, st); break;
+        case 256: launch_bst_polar_t<256, 8>(k, p, lds, st); break;
+        case 512: launch_bst_polar_t<512, 8>(k, p, lds, st); break;
+        default: launch_bst_polar_t<1024, 8>(k, p, lds, st); break;
+        }
+    }
+    RAFT_HIP(hipGetLastError());
+    const long long cells = (long long)a.S * s * s;
+    hipLaunchKernelGGL(k_bst_p2c, dim3((unsigned)std::min<long long>((cells + 255) / 256, 65536)), dim3(256), 0, st,
+                       a.polar, a.cart, n, 2 * a.A, s, a.S);
+    RAFT_HIP(hipGetLastError());
+    for (int pass = 0; pass < 2; ++pass) {
+        float2* d = pass == 0 ? a.cart : a.cartT;
+        const long long rows = (long long)a.S * s;
+        if (n > 8192) launch_ifft_t<1024, 16>(d, a.tw, p, rows, lds, st);
+        else {
+            switch (plan.threads) {
+            case 64: launch_ifft_t<64, 8>(d, a.tw, p, rows, lds, st); break;
+            case 128: launch_ifft_t<128, 8>(d, a.tw, p, rows, lds, st); break;
+            case 256: launch_ifft_t<256, 8>(d, a.tw, p, rows, lds, st); break;
+            case 512: launch_ifft_t<512, 8>(d, a.tw, p, rows, lds, st); break;
+            default: launch_ifft_t<1024, 8>(d, a.tw, p, rows, lds, st); break;
+            }
+        }
+        RAFT_HIP(hipGetLastError());
+        if (pass == 0) {
+            dim3 g((s + 31) / 32, (s + 31) / 32, a.S);
+            hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s);
+            RAFT_HIP(hipGetLastError());
+        }
+    }
+    const long long outn = (long long)a.S * a.N * a.N;
+    hipLaunchKernelGGL(k_bst_extract, dim3((unsigned)std::min<long long>((outn + 255) / 256, 65536)), dim3(256), 0,
+                       st, a.cartT, a.obj, s, a.N, a.L, a.scale, a.S);
     RAFT_HIP(hipGetLastError());
 }
 

@@ -459,6 +459,70 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
 // ---------------------------------------------------------------------------
 // Radon on device pointers
 // ---------------------------------------------------------------------------
+// FBP by the Backprojection Slice Theorem (bst.cu:251-312, 536-736): same
+// parameter layout as getFBPMultiGPU; the padded rows (edge replicate, pad
+// L = pad N / 2) are filtered in frequency by apply(min(k, n-k)/n) times
+// exp(-2 pi i c k / n), c = (int) axis offset (filters.cu:295-307, 369-390,
+// unnormalised inverse), then launch_bst; the image is cropped from the
+// n x n plane at offset L and divided by 4 n pixel.  The reference assumes
+// angles uniform on [0, pi) (the Python front end passes linspace).
+static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* paramf, const int* parami,
+                       hipStream_t st)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (c.S == 0) return;
+    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+    const int n = c.npad, A = c.A, N = c.N;
+    FilterTables& F = filter_tables(D, c, st);
+    if (!F.plan.ok) fail(RAFT_EUNSUPPORTED, "BST needs a padded size n = N(1+pad) <= 16384 with factors 2, 3, 5");
+    std::vector<float2> H(n);
+    const int center = (int)c.offset;
+    for (int k = 0; k < n; ++k) {
+        const double w = (double)std::min(k, n - k) / (double)n;
+        const double amp = filter_apply(c.filter, w, c.reg, c.paganin);
+        const double ph = -2.0 * M_PI * (double)center * (double)k / (double)n;
+        H[k] = make_float2((float)(amp * std::cos(ph)), (float)(amp * std::sin(ph)));
+    }
+    DevBuf dH, dwa;
+    const std::vector<float> ones(A, 1.0f);
+    upload(dH, H, st);
+    upload(dwa, ones, st);
+    const size_t s2 = (size_t)n * n;
+    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * n * 8 + 2 * s2 * 8;
+    int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
+    B = ((B + 3) / 4) * 4;
+    const size_t budget = block_budget();
+    while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
+    float* q = static_cast<float*>(D.workspace(0, (size_t)B * A * n * 4));
+    float2* polar = static_cast<float2*>(D.workspace(1, (size_t)B * 2 * A * n * 8));
+    float2* cart = static_cast<float2*>(D.workspace(2, (size_t)B * s2 * 8));
+    float2* cartT = static_cast<float2*>(D.workspace(3, (size_t)B * s2 * 8));
+    for (int s0 = 0; s0 < c.S; s0 += B) {
+        const int nb = std::min(B, c.S - s0);
+        FilterArgs fa;
+        fa.tomo = d_tomo + (size_t)s0 * A * N;
+        fa.q = q;
+        fa.H = dH.as<float2>();
+        fa.tw = F.tw.as<float2>();
+        fa.g = nullptr;
+        fa.wa = dwa.as<float>();
+        fa.N = N; fa.A = A; fa.S = nb; fa.npad = n; fa.L = c.L; fa.tlo = 0; fa.W = n;
+        fa.none = (c.filter == 0);
+        launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
+        BstArgs ba;
+        ba.q = q; ba.polar = polar; ba.cart = cart; ba.cartT = cartT;
+        ba.obj = d_obj + (size_t)s0 * N * N;
+        ba.tw = F.tw.as<float2>();
+        ba.N = N; ba.A = A; ba.S = nb; ba.L = c.L;
+        ba.scale = (float)((double)n * c.pixel * 4.0);
+        launch_bst(ba, F.plan, st);
+    }
+    RAFT_HIP(hipStreamSynchronize(st));
+}
+
 // Filter-only export (low_pass.cu:9-141 / filters.cu:120-161 filterFBPpad):
 // edge-replicate pad, the FBP filter with no axis offset, crop back to the N
 // original bins, / (npad * pixel); in place.  Filter 'none' leaves the data as is.
@@ -1338,6 +1402,50 @@ int raft_tem_device(int device, float* d_obj, const float* d_count, const float*
 {
     return guarded(
         [&] { tem_device(device, d_obj, d_count, d_flat, angles, parami, static_cast<hipStream_t>(stream)); });
+}
+
+static void bst_host(int* gpus, int ngpus, float* obj, float* tomo, float* paramf, int* parami)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (!obj || !tomo) fail(RAFT_EINVAL, "null data pointer");
+    check_gpus(gpus, ngpus);
+    run_slabs(gpus, ngpus, c.S, [&](int dev, int first, int cnt) {
+        bind_device(dev);
+        Stream st;
+        const size_t tin = (size_t)c.A * c.N, oout = (size_t)c.N * c.N;
+        const size_t n = (size_t)c.npad;
+        int B = c.blocksize > 0 ? std::min(c.blocksize, cnt) : cnt;
+        const size_t need = (tin + oout) * 4 + (size_t)c.A * n * 20 + n * n * 16;
+        const size_t budget = block_budget();
+        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
+        DevBuf dt((size_t)B * tin * 4), dob((size_t)B * oout * 4);
+        std::vector<int> pi(parami, parami + 10);
+        for (int s0 = 0; s0 < cnt; s0 += B) {
+            const int nb = std::min(B, cnt - s0);
+            RAFT_HIP(hipMemcpyAsync(dt.p, tomo + (size_t)(first + s0) * tin, (size_t)nb * tin * 4,
+                                    hipMemcpyHostToDevice, st.s));
+            pi[2] = nb;
+            pi[9] = 0;
+            bst_device(dev, dob.as<float>(), dt.as<float>(), paramf, pi.data(), st.s);
+            RAFT_HIP(hipMemcpyAsync(obj + (size_t)(first + s0) * oout, dob.p, (size_t)nb * oout * 4,
+                                    hipMemcpyDeviceToHost, st.s));
+        }
+        RAFT_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+void getBSTMultiGPU(int* gpus, int ngpus, float* obj, float* tomogram, float* angles, float* paramf, int* parami,
+                    int nstreams)
+{
+    (void)angles;
+    (void)nstreams;
+    guarded([&] { bst_host(gpus, ngpus, obj, tomogram, paramf, parami); });
+}
+
+int raft_bst_device(int device, float* d_obj, const float* d_tomo, const float* paramf, const int* parami,
+                    void* stream)
+{
+    return guarded([&] { bst_device(device, d_obj, d_tomo, paramf, parami, static_cast<hipStream_t>(stream)); });
 }
 
 static void lowpass_host(int* gpus, int ngpus, float* tomo, float* paramf, int* parami)

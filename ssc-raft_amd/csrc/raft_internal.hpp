@@ -124,6 +124,21 @@ struct FilterArgs {
 };
 void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
 
+// BST after the row filter: q = filtered padded rows (interleaved [G][A][n][4],
+// window [0, n)), polar [S][2A][n], cart / cartT [S][n][n] complex work planes,
+// obj [S][N][N]; tw the twiddle tables of n
+struct BstArgs {
+    const float* q;
+    float2* polar;
+    float2* cart;
+    float2* cartT;
+    float* obj;
+    const float2* tw;
+    int N, A, S, L;
+    float scale;        // 4 n pixel
+};
+void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st);
+
 struct BpArgs {
     const void* q;      // [G][A][W][SB] staged rows (float SB = 4, or half SB = 8)
     float* out;         // mode 0: planar [S][n][n]; mode 1: interleaved [G][n][n][4]

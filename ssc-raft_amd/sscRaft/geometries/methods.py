@@ -2,7 +2,7 @@
 from ..io import SetDictionary
 from ..rafttypes import logger, numpy
 from .parallel.em_methods import eEMRT_GPU_, tEMRT_GPU_
-from .parallel.fbp_methods import fbpGPU
+from .parallel.fbp_methods import bstGPU, fbpGPU
 
 __all__ = ["fbp", "em"]
 
@@ -13,7 +13,7 @@ def fbp(tomogram, angles=None, obj=None, dic=None, nstreams=0, **kwargs):
     dic keys (defaults): 'gpu' (required), 'angles[rad]', 'filter' ('ramp'),
     'rotation axis offset' (0.0), 'padding' (2), 'regularization' (0.0),
     'beta/delta' (0.0), 'blocksize' (0), 'energy[eV]' (1.0), 'z2[m]' (1.0),
-    'method' ('RT'), 'detectorPixel[m]' (1.0)."""
+    'method' ('RT' or 'BST'), 'detectorPixel[m]' (1.0)."""
     required = ("gpu",)
     optional = ("filter", "rotation axis offset", "padding", "regularization", "beta/delta", "blocksize",
                 "energy[eV]", "z2[m]", "method", "detectorPixel[m]")
@@ -29,7 +29,8 @@ def fbp(tomogram, angles=None, obj=None, dic=None, nstreams=0, **kwargs):
             raise ValueError("Missing angles list!!")
         return fbpGPU(tomogram, angles, gpus, dic, obj=obj)
     if method == "BST":
-        raise NotImplementedError("FBP method 'BST' is not implemented in this build")
+        angles = numpy.linspace(0.0, numpy.pi, tomogram.shape[-2], endpoint=False)
+        return bstGPU(tomogram, angles, gpus, dic, obj=obj, nstreams=nstreams)
     raise ValueError(f"Invalid FBP method: {method}")
 
 

@@ -2,7 +2,7 @@
 sscRaft/geometries/parallel/fbp_methods.py:4-103."""
 from ...rafttypes import CNICE, FilterNumber, libraft, logger, raft_check, numpy, ctypes
 
-__all__ = ["fbpGPU", "fbp_params"]
+__all__ = ["fbpGPU", "fbp_params", "bstGPU"]
 
 
 def fbp_params(nrays, nangles, nslices, dic):
@@ -51,4 +51,29 @@ def fbpGPU(tomogram, angles, gpus, dic, obj=None):
                            angles.ctypes.data_as(ctypes.c_void_p), paramf.ctypes.data_as(ctypes.c_void_p),
                            parami.ctypes.data_as(ctypes.c_void_p))
     raft_check("getFBPMultiGPU")
+    return obj
+
+
+def bstGPU(tomogram, angles, gpus, dic, obj=None, nstreams=0):
+    """FBP by the Backprojection Slice Theorem, as fbp_methods.py:105-205: same
+    parameter packing as fbpGPU except that the axis offset is passed as an
+    integer (int(dic['rotation axis offset'])); calls getBSTMultiGPU.  The
+    reconstruction assumes angles uniform on [0, pi)."""
+    gpus = numpy.ascontiguousarray(numpy.array(gpus).astype(numpy.intc))
+    nrays = tomogram.shape[-1]
+    nangles = tomogram.shape[-2]
+    nslices = 1 if len(tomogram.shape) == 2 else tomogram.shape[0]
+    d = dict(dic)
+    d["rotation axis offset"] = int(d["rotation axis offset"])
+    paramf, parami = fbp_params(nrays, nangles, nslices, d)
+    tomogram = CNICE(tomogram)
+    if obj is None:
+        obj = numpy.zeros([nslices, nrays, nrays], dtype=numpy.float32)
+    obj = CNICE(obj)
+    angles = CNICE(numpy.array(angles))
+    libraft.getBSTMultiGPU(gpus.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(len(gpus)),
+                           obj.ctypes.data_as(ctypes.c_void_p), tomogram.ctypes.data_as(ctypes.c_void_p),
+                           angles.ctypes.data_as(ctypes.c_void_p), paramf.ctypes.data_as(ctypes.c_void_p),
+                           parami.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(nstreams))
+    raft_check("getBSTMultiGPU")
     return obj

@@ -192,8 +192,10 @@ def test_fbp_front_end_validation():
         sscRaft.fbp(y, dic={"gpu": [0]})
     with pytest.raises(ValueError, match="gpu"):
         sscRaft.fbp(y, angles=np.zeros(8), dic={})
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(RuntimeError, match="no HIP device"):
         sscRaft.fbp(y, angles=np.zeros(8), dic={"gpu": [0], "method": "BST"})
+    with pytest.raises(ValueError, match="Invalid FBP method"):
+        sscRaft.fbp(y, angles=np.zeros(8), dic={"gpu": [0], "method": "XX"})
     with pytest.raises(KeyError):
         sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0]})  # 'method' is required by the reference too
     with pytest.raises(NotImplementedError):

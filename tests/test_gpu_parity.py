@@ -381,3 +381,23 @@ def test_batch_uneven_scans_match_single_calls(gpu):
         assert rel(fouts[k].cpu().numpy(), ref[0].cpu().numpy()) <= 1e-4  # FP16 rows: one f16 rounding may flip
         x, _, _ = dv.eem_device(ys[k][None].contiguous(), angs[k], K, with_kl=False)
         np.testing.assert_array_equal(ems[k].cpu().numpy(), x[0].cpu().numpy())
+
+
+@pytest.mark.parametrize("n,a,pad,filt,offset", [(64, 96, 2, "ramp", 0.0), (96, 64, 2, "hann", 2.0),
+                                                 (100, 90, 1, "gaussian", 0.0), (64, 48, 2, "none", 0.0),
+                                                 (128, 128, 2, "ramp", -3.0)])
+def test_bst(gpu, n, a, pad, filt, offset):
+    """getBSTMultiGPU vs the float64 restatement (oracle_np.bst)."""
+    import oracle_np
+    import sscRaft
+
+    y, ang = noisy_sino(n, a, 2, seed=n + a)
+    dic = fbp_dic(ang, pad=pad, filt=filt, pixel=2.0 / n, offset=offset, reg=0.5)
+    dic["method"] = "BST"
+    out = sscRaft.fbp(y, dic=dic)
+    pf, pi = oracle.fbp_params(n, a, 2, pad=pad, filter_type=sscRaft.FilterNumber(filt), pixel=2.0 / n,
+                               offset=float(int(offset)), reg=0.5)
+    ref = oracle_np.bst(y, pf, pi)
+    e = rel(out, ref)
+    print(f"bst n={n} a={a} pad={pad} {filt} offset={offset}: rel-L2 {e:.2e}")
+    assert e <= 2e-5
