@@ -13,6 +13,11 @@ Also reported on the same JSON line:
   cpu_baseline  the FP32 restatement in oracle/ timed on this host's cores on
                 a bounded sample (whole slices of the same workload);
   fp16_rows     the same FBP with FP16-stored filtered rows;
+  host_pointers_pcie  the same FBP through the host-pointer entry point
+                (getFBPMultiGPU), PCIe copies included;
+  radon         Radon-RT of the 64 phantoms (2048^2 -> 2048 x 2048): whole
+                call and projector kernel, gathered-sample roofline, FP32 port
+                on the host, error vs the FP64 oracle;
   em            plain eEM on 16 slices of 1024^2 (1024 angles, I0 = 1e4,
                 uniform start, 200 iterations, FP64 KL per iteration);
   em.learned_init  time-to-KL of the U-Net warm start vs plain eEM on held-out
@@ -25,8 +30,10 @@ Also reported on the same JSON line:
   unet          the warm start U-Net's forward / training throughput against
                 dense BF16 MFMA (2.5 PFLOP/s).
 Parity of the timed paths against the FP64 oracle is the job of tests/ (the
-2048^2 FBP case is tests/test_gpu_parity.py::test_fbp_2048); the oracle is
-used here only for the CPU baseline.
+2048^2 FBP case is tests/test_gpu_parity.py::test_fbp_2048); here the oracle
+is used only in the CPU baseline leg: the FP32 port is timed, and the FP64
+oracle checks the first 4 slices of the timed GPU outputs (FBP with FP32 and
+FP16 rows, Radon).
 """
 import argparse
 import json
@@ -41,6 +48,7 @@ sys.path.insert(0, os.path.join(ROOT, "ssc-raft_amd"))
 
 METRIC = "FBP slices/s @2048²; EM time-to-KL, learned-init vs plain, 1 GPU"
 HBM_PEAK = 8.0e12  # B/s, MI355X
+ERR_SLICES = 4  # slices of each workload checked against the FP64 oracle
 
 
 def log(*a):
@@ -75,24 +83,115 @@ def fbp_dic(N):
             "blocksize": 0, "energy[eV]": 1.0, "z2[m]": 0.0, "detectorPixel[m]": 2.0 / N, "padding": 2}
 
 
-def cpu_baseline(args, y, ang, N, A):
-    """The FP32 C restatement (oracle/) on this host's cores, on whole slices."""
+def _err(a, ref):
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return dict(rel_l2=float(np.linalg.norm(a - ref) / max(np.linalg.norm(ref), 1e-300)),
+                max_abs=float(np.max(np.abs(a - ref))), max_ref=float(np.max(np.abs(ref))))
+
+
+def cpu_baseline(args, host, ang, N, A):
+    """The FP32 restatement in oracle/ on this host's cores, on whole slices;
+    then the FP64 oracle on the first ERR_SLICES slices as the checker of the
+    GPU outputs kept in `host` (FBP with FP32 and FP16 rows, Radon)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     from sscRaft.geometries.parallel.fbp_methods import fbp_params
 
-    ys = y[: args.cpu_slices].cpu().numpy()
-    pf1, pi1 = fbp_params(N, A, args.cpu_slices, fbp_dic(N))
+    ns = min(args.cpu_slices, host["y"].shape[0])
+    ys = np.ascontiguousarray(host["y"][:ns])
+    pf1, pi1 = fbp_params(N, A, ns, fbp_dic(N))
     cores = host_cores()
     t0 = time.perf_counter()
-    oracle.fbp(ys, ang, pf1, pi1, prec="f32", nthreads=cores)
+    port = oracle.fbp(ys, ang, pf1, pi1, prec="f32", nthreads=cores)
     dt = time.perf_counter() - t0
-    cpu = dict(value=args.cpu_slices / dt, unit="slices/s", cores=cores, kind="port",
-               sample=f"{args.cpu_slices} of the 64 slices (2048^2, 2048 angles, pad 2, ramp), "
-                      f"FP32 C restatement with FP64 bin indices, {dt:.2f} s")
+    cpu = dict(value=ns / dt, unit="slices/s", cores=cores, kind="port",
+               sample=f"{ns} of the 64 slices (2048^2, 2048 angles, pad 2, ramp), "
+                      f"FP32 C restatement with FP64 bin indices, {dt:.2f} s",
+               host_cpu=host_cpu_model())
     log(f"CPU baseline: {cpu['value']:.3f} slices/s on {cores} threads")
-    return cpu
+    ne = host["fbp_f32"].shape[0]
+    pfe, pie = fbp_params(N, A, ne, fbp_dic(N))
+    t0 = time.perf_counter()
+    ref = oracle.fbp(np.ascontiguousarray(host["y"][:ne]), ang, pfe, pie, prec="f64", nthreads=cores)
+    cpu["errors_vs_fp64_oracle"] = dict(
+        slices=ne, gpu_f32_rows=_err(host["fbp_f32"], ref), gpu_f16_rows=_err(host["fbp_f16"], ref),
+        cpu_port=_err(port, ref[:ns]))
+    log(f"FP64 oracle on {ne} slices: {time.perf_counter() - t0:.1f} s; GPU rel-L2 "
+        f"{cpu['errors_vs_fp64_oracle']['gpu_f32_rows']['rel_l2']:.2e} (FP32 rows), "
+        f"{cpu['errors_vs_fp64_oracle']['gpu_f16_rows']['rel_l2']:.2e} (FP16 rows)")
+    # Radon: FP32 port on one slice (timed), FP64 oracle on ERR_SLICES slices (checker)
+    img = host["img"]
+    t0 = time.perf_counter()
+    oracle.radon(np.ascontiguousarray(img[:1]), ang, prec="f32", nthreads=cores)
+    rdt = time.perf_counter() - t0
+    rref = oracle.radon(np.ascontiguousarray(img), ang, prec="f64", nthreads=cores)
+    radon = dict(value=1.0 / rdt, unit="slices/s", cores=cores, kind="port",
+                 sample="1 of the 64 phantoms (2048^2 -> 2048 angles x 2048 bins), FP32 C restatement",
+                 errors_vs_fp64_oracle=dict(slices=img.shape[0], gpu=_err(host["radon"], rref)))
+    log(f"Radon CPU: {radon['value']:.3f} slices/s; GPU rel-L2 vs FP64 {radon['errors_vs_fp64_oracle']['gpu']['rel_l2']:.2e}")
+    return cpu, radon
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip() + f" ({os.cpu_count()} logical CPUs)"
+    except OSError:
+        pass
+    return None
+
+
+def radon_bench(args, torch, dv, img, ang, out):
+    """Radon-RT of the 64 phantoms (planar in HBM) into `out`: whole call and
+    the projector kernel (HIP events on the launch stream)."""
+    S, N, _ = img.shape
+    A = len(ang)
+    for _ in range(args.warmup):
+        dv.radon_device(img, ang, out=out)
+    torch.cuda.synchronize()
+    dv.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dv.radon_device(img, ang, out=out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    k_ms = dv.last_kernel_ms(dv.KF_PROJ)
+    dv.set_profiling(False)
+    samples = float(S) * A * N * N
+    bytes_alg = samples * 4 + float(S) * N * N * 4 + float(S) * A * N * 4
+    ks = k_ms * 1e-3
+    r = dict(config=f"Radon-RT {S} x 2048^2 -> {A} angles x {N} bins", slices_per_s=S * args.steps / dt,
+             ms_per_step=1e3 * dt / args.steps, kernel="k_project<0>", kernel_ms=k_ms,
+             samples_per_s=samples / ks,
+             roofline=dict(bound="hbm", achieved=bytes_alg / ks / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                           frac=bytes_alg / ks / HBM_PEAK, bytes_per_launch=bytes_alg, samples_per_launch=samples,
+                           valu_frac=(2.0 * samples / ks) / 157.3e12))
+    log(f"Radon: {r['slices_per_s']:.1f} slices/s, projector {k_ms:.3f} ms/launch")
+    return r
+
+
+def pcie_bench(args, y, ang, N):
+    """The host-pointer entry point (getFBPMultiGPU through sscRaft.fbp) on the
+    same 64 slices: host -> device copy, FBP, device -> host copy."""
+    import sscRaft
+
+    yh = y.cpu().numpy()
+    dic = dict(fbp_dic(N), **{"gpu": [0], "angles[rad]": ang, "method": "RT"})
+    sscRaft.fbp(yh, dic=dic)
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sscRaft.fbp(yh, dic=dic)
+    dt = (time.perf_counter() - t0) / reps
+    r = dict(slices_per_s=yh.shape[0] / dt, seconds_per_call=dt,
+             bytes_moved=float(yh.nbytes) + float(yh.shape[0]) * N * N * 4,
+             note="sscRaft.fbp on host numpy arrays (pageable memory): includes both PCIe copies")
+    log(f"FBP via host pointers: {r['slices_per_s']:.1f} slices/s ({dt:.3f} s per call)")
+    return r
 
 
 def fbp_bench(args, torch, dv, ph):
@@ -104,9 +203,12 @@ def fbp_bench(args, torch, dv, ph):
     img = ph.phantoms(N, S, config=2, device=dev)
     p = dv.radon_device(img, ang)
     y = ph.poisson_log_torch(p, 1.0e4, seed=2000)
-    del img, p
     torch.cuda.synchronize()
     log(f"inputs: {S} x {A} x {N} sinogram generated on the GPU in {time.perf_counter() - t0:.1f} s")
+    radon = radon_bench(args, torch, dv, img, ang, p)
+    ne = min(ERR_SLICES, S)
+    host = dict(y=y[:max(ne, args.cpu_slices)].cpu().numpy(), img=img[:ne].cpu().numpy(), radon=p[:ne].cpu().numpy())
+    del img, p
     if args.copy_calibration:
         # elementwise copy kernel: reads and writes exactly 4 * S * A * N bytes
         for _ in range(3):
@@ -136,9 +238,13 @@ def fbp_bench(args, torch, dv, ph):
         dv.set_profiling(False)
         res[store] = dict(seconds=dt, ms_per_step=1e3 * dt / args.steps, slices_per_s=S * args.steps / dt,
                           bp_ms=bp_ms, filter_ms=filt_ms)
+        host["fbp_" + store] = out[:ne].cpu().numpy()
         log(f"FBP {store}: {res[store]['slices_per_s']:.1f} slices/s, step {res[store]['ms_per_step']:.2f} ms, "
             f"backprojection {bp_ms:.3f} ms/launch, filter {filt_ms:.3f} ms/launch")
-    cpu = None if args.no_cpu else cpu_baseline(args, y, ang, N, A)
+    cpu, radon_cpu = (None, None) if args.no_cpu else cpu_baseline(args, host, ang, N, A)
+    if radon is not None:
+        radon["cpu"] = radon_cpu
+    pcie = None if args.no_pcie else pcie_bench(args, y, ang, N)
     # roofline of the backprojection kernel (FP32 rows)
     samples = float(S) * A * N * N
     bytes_alg = samples * 4 + float(S) * A * width * 4 + float(S) * N * N * 4
@@ -149,7 +255,7 @@ def fbp_bench(args, torch, dv, ph):
                 kernel="k_backproject<float,0>", bytes_per_launch=bytes_alg, samples_per_launch=samples,
                 launch_ms=res["f32"]["bp_ms"], valu_frac=(2.0 * samples / bp_s) / 157.3e12,
                 staged_row_width=width, fft_path=bool(fft_path))
-    return res, roof, cpu
+    return res, roof, cpu, radon, pcie
 
 
 def em_bench(args, torch, dv, ph):
@@ -339,8 +445,9 @@ def main():
     ap.add_argument("--batch-scans", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=1)
+    ap.add_argument("--cpu-slices", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-em", action="store_true")
     ap.add_argument("--em-iters", type=int, default=200)
     ap.add_argument("--only", choices=["all", "fbp", "em", "batch"], default="all",
@@ -372,7 +479,7 @@ def main():
             em["learned_init"] = learned_bench(args)
         print(json.dumps({"metric": METRIC, "em": em}), flush=True)
         return
-    res, roof, cpu = fbp_bench(args, torch, dv, ph)
+    res, roof, cpu, radon, pcie = fbp_bench(args, torch, dv, ph)
     em = None if (args.no_em or args.only == "fbp") else em_bench(args, torch, dv, ph)
     if em is not None and not args.no_learned:
         em["learned_init"] = learned_bench(args)
@@ -401,6 +508,8 @@ def main():
         "fp16_rows": {"value": res["f16"]["slices_per_s"], "unit": "slices/s", "bp_ms": res["f16"]["bp_ms"],
                       "filter_ms": res["f16"]["filter_ms"]},
         "filter_ms": r32["filter_ms"],
+        "host_pointers_pcie": pcie,
+        "radon": radon,
         "em": em,
         "c1": c1,
         "unet": unet,

@@ -7,36 +7,66 @@
 // interleaved transpose instead, so the fast-varying direction across rays is
 // always the contiguous one.
 //
-// Work split: one workgroup per (block of PROJ_ANGLES consecutive angles,
-// slice group), one wave per angle.  The waves walk the same blocks of R
-// consecutive rays in lockstep (a barrier per block), so the 8 nearby angles
-// read nearly the same image region at the same time and the reuse is served
-// by L1/L2.  Every ray is split over Kp = 64/R lanes that take its samples
-// k0+kp, k0+kp+Kp, ...; at each step a wave therefore reads an R x Kp patch of
-// the rotated sampling grid, which maps to a compact patch of the image (R is
-// 64 for angles within ~2 degrees of an axis and 8 near 45 degrees).  The Kp
-// partial sums of a ray are combined with lane shuffles.
+// Two forms share the sampling and the epilogue:
 //
+// Staged form (k_project_lds, dense angle sets).  One workgroup = 8 waves =
+// 8 consecutive angles, one ray chunk, one slice group.  The waves walk the
+// same block of PL_RAYS rays in steps of PL_K samples; at every step the
+// workgroup copies the union of the 8 patches' bounding boxes (rays
+// [ib, ib + 32) x samples [ks, ks + 32), mapped into the image) from HBM/L2
+// into LDS with row-contiguous 16-byte loads -- zeros outside the image --
+// and each wave gathers its 1024 samples from LDS.  Because out-of-image
+// samples read staged zeros, the inner loop has no range test.  The gathers
+// of a direct form are limited by the L1 tag rate (a wave's 64 gathers touch
+// ~38 distinct 64-byte lines); staged, a step costs ~17 lines per staged row
+// and the gathers become LDS reads.  Rows are staged at a pitch = 2 (mod 8)
+// 16-byte entries, so rays stepping along a row, down a column or along a
+// diagonal fall into distinct LDS banks.  Near-vertical chunks stage from
+// the transposed image, so that neighbouring rays always step along a row.
+// The host admits a launch to this form only when every chunk's union box
+// fits PL_CAP entries at every patch position (proj_plan); sparse angle sets
+// use the direct form.
+//
+// Direct form (k_project): one workgroup per (block of PROJ_ANGLES
+// consecutive angles, slice group), one wave per angle.  The waves walk the
+// same blocks of R consecutive rays in lockstep (a barrier per block), so the
+// 8 nearby angles read nearly the same image region at the same time and the
+// reuse is served by L1/L2.  Every ray is split over Kp = 64/R lanes that
+// take its samples k0+kp, k0+kp+Kp, ...; at each step a wave therefore reads
+// an R x Kp patch of the rotated sampling grid (R is 64 for angles within ~2
+// degrees of an axis and 8 near 45 degrees).  The Kp partial sums of a ray
+// are combined with lane shuffles.
+
 // Sample coordinates are IEEE-double affine functions of the sample index,
 //     ux = fma(k - ck, -sin*ry, fma(i - ci, cos, ucx)),  X = (int)ux
 //     uy = fma(k - ck,  cos,    fma(i - ci, sin*rx, ucy)), Y = (int)uy
 // (bit-identical to the CPU oracle).  The range of k whose sample lies in the
-// image is computed exactly per ray: both coordinates are monotone in k, so
-// the valid set is an interval; a generous real-valued estimate is shrunk and
-// extended by evaluating the same expressions at its ends, and the inner loop
-// has no range test.
+// image is computed exactly per ray in the direct form: both coordinates are
+// monotone in k, so the valid set is an interval; a generous real-valued
+// estimate is shrunk and extended by evaluating the same expressions at its
+// ends, and the inner loop has no range test.
 //
 // eEM mode fuses kernel_radonWithDivision (em_RT_kernel.cu:188-245): it forms
 // v = dt * sum, the clamped smoothed ratio r = y SIGN(v)/sqrt(e^2+v^2)
 // (configs.hpp:31-32, e = 1e-4), stores r * dtheta_a into the interleaved
 // staged-row layout of the backprojector, and accumulates the FP64 KL term
-// v - y + y ln(y/v) (bins with v <= 0 or y < 0 are counted, not summed).
+// v - y + y ln(y/v) (bins with v <= 0 or y < 0 are counted, not summed),
+// one partial per (slice, angle, ray chunk) in a fixed order.
 // Radon mode restates Radon_RT_version_sscRadon (radon_rt.cu:131-174).
 #include "raft_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
 
 namespace raft {
 
 namespace {
+
+constexpr int PL_RAYS = 32;   // rays per patch (staged form)
+constexpr int PL_K = 32;      // samples per ray per step (staged form)
+constexpr int PL_BATCH = 64;  // steps whose boxes are computed together
+constexpr int PL_CAP = 4480;  // staged 16-byte entries per workgroup (70 KB: two workgroups per CU)
 
 struct ProjK {
     const float4* img;
@@ -52,6 +82,8 @@ struct ProjK {
     double ci, ck, ucx, ucy, ry, rx;
     float scale;
     int tlo, W;
+    int RC;          // ray chunks per angle chunk (staged form; 1 for the direct form)
+    int kl_stride;   // KL partials per slice = nangles * RC
 };
 
 __device__ __forceinline__ bool in_image(double kr, double bxk, double uxi, double cs, double uyi, int Nx, int Ny)
@@ -98,6 +130,70 @@ __device__ __forceinline__ float ratio(float yv, float v)
     const float sg = (v > 0.0f) ? 1.0f : ((v < 0.0f) ? -1.0f : 0.0f);
     const float r = yv * (sg / sqrtf(e * e + v * v));
     return r < 0.0f ? 0.0f : r;
+}
+
+// Output of one ray for the 4 slices of group g: Radon line sums, eEM ratio
+// row (+ KL terms), or tEM flat * exp(-v) row.
+template <int EM>
+__device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, float4 acc, double* klsum,
+                                        unsigned long long* exc)
+{
+    const float v[4] = {acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale};
+    if (!EM) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int sl = g * 4 + s;
+            if (sl < a.S) a.out[((size_t)sl * a.nangles + ang) * a.nrays + i] = v[s];
+        }
+    } else if (EM == 2) {
+        // transmission EM (em_RT_kernel.cu:20-31 with the slice's own flat
+        // row, y = flat [S][nrays]): flat * exp(-v), times dtheta
+        const float wv = a.wa[ang];
+        float rr[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int sl = g * 4 + s;
+            rr[s] = (sl < a.S) ? a.y[(size_t)sl * a.nrays + i] * expf(-v[s]) * wv : 0.0f;
+        }
+        reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
+            make_float4(rr[0], rr[1], rr[2], rr[3]);
+    } else {
+        const float wv = a.wa[ang];
+        float rr[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int sl = g * 4 + s;
+            const float yv = (sl < a.S) ? a.y[((size_t)sl * a.nangles + ang) * a.nrays + i] : 0.0f;
+            rr[s] = ratio(yv, v[s]) * wv;
+            if (a.kl_part && sl < a.S) {
+                const double vd = (double)v[s], yd = (double)yv;
+                if (vd > 0.0 && yd >= 0.0) klsum[s] += vd - yd + (yd > 0.0 ? yd * log(yd / vd) : 0.0);
+                else exc[s] += 1;
+            }
+        }
+        reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
+            make_float4(rr[0], rr[1], rr[2], rr[3]);
+    }
+}
+
+// deterministic KL partial of one wave (one angle): wave reduction in fixed order
+__device__ __forceinline__ void kl_flush(const ProjK& a, int g, int slot, int lane, const double* klsum,
+                                         const unsigned long long* exc)
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        double d = klsum[s];
+        unsigned long long e = exc[s];
+        for (int off = 32; off > 0; off >>= 1) {
+            d += __shfl_xor(d, off);
+            e += __shfl_xor(e, off);
+        }
+        const int sl = g * 4 + s;
+        if (lane == 0 && sl < a.S) {
+            a.kl_part[(size_t)sl * a.kl_stride + slot] = d;
+            a.ex_part[(size_t)sl * a.kl_stride + slot] = e;
+        }
+    }
 }
 
 template <int EM>
@@ -195,62 +291,209 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
             acc.z += __shfl_xor(acc.z, off);
             acc.w += __shfl_xor(acc.w, off);
         }
-        if (have && kp == 0 && i < a.nrays) {
-            const float v[4] = {acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale};
-            if (!EM) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int sl = g * 4 + s;
-                    if (sl < a.S) a.out[((size_t)sl * a.nangles + ang) * a.nrays + i] = v[s];
-                }
-            } else if (EM == 2) {
-                // transmission EM (em_RT_kernel.cu:20-31 with the slice's own flat
-                // row, y = flat [S][nrays]): flat * exp(-v), times dtheta
-                const float wv = a.wa[ang];
-                float rr[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int sl = g * 4 + s;
-                    rr[s] = (sl < a.S) ? a.y[(size_t)sl * a.nrays + i] * expf(-v[s]) * wv : 0.0f;
-                }
-                reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
-                    make_float4(rr[0], rr[1], rr[2], rr[3]);
-            } else {
-                const float wv = a.wa[ang];
-                float rr[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int sl = g * 4 + s;
-                    const float yv = (sl < a.S) ? a.y[((size_t)sl * a.nangles + ang) * a.nrays + i] : 0.0f;
-                    rr[s] = ratio(yv, v[s]) * wv;
-                    if (a.kl_part && sl < a.S) {
-                        const double vd = (double)v[s], yd = (double)yv;
-                        if (vd > 0.0 && yd >= 0.0) klsum[s] += vd - yd + (yd > 0.0 ? yd * log(yd / vd) : 0.0);
-                        else exc[s] += 1;
-                    }
-                }
-                reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
-                    make_float4(rr[0], rr[1], rr[2], rr[3]);
-            }
-        }
+        if (have && kp == 0 && i < a.nrays) ray_out<EM>(a, g, ang, i, acc, klsum, exc);
         __syncthreads(); // keep the waves (angles) of the workgroup on the same ray block
     }
-    if (EM != 1 || !a.kl_part || !have) return;
-    // deterministic per-(slice, angle) KL partial: wave reduction in fixed order
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        double d = klsum[s];
-        unsigned long long e = exc[s];
+    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang, lane, klsum, exc);
+}
+
+// ---- staged form ----
+// Bounding box of one wave's patch (rays [ib, ib + PL_RAYS) x samples
+// [ks, ks + PL_K)) in staged (row, col) coordinates, half-open.  Each
+// coordinate is affine in (ir, kr), so its extremes over the patch are
+// separable sums; the margins cover the rounding of the sample expressions
+// and truncation toward zero.
+__device__ __forceinline__ int4 patch_box(double ir0, double kr0, double rU, double rI, double rK, double cU, double cI,
+                                          double cK)
+{
+    const double ir1 = ir0 + (double)(PL_RAYS - 1), kr1 = kr0 + (double)(PL_K - 1);
+    const double rlo = rU + fmin(ir0 * rI, ir1 * rI) + fmin(kr0 * rK, kr1 * rK);
+    const double rhi = rU + fmax(ir0 * rI, ir1 * rI) + fmax(kr0 * rK, kr1 * rK);
+    const double clo = cU + fmin(ir0 * cI, ir1 * cI) + fmin(kr0 * cK, kr1 * cK);
+    const double chi = cU + fmax(ir0 * cI, ir1 * cI) + fmax(kr0 * cK, kr1 * cK);
+    return make_int4((int)floor(rlo) - 1, (int)floor(rhi) + 3, (int)floor(clo) - 1, (int)floor(chi) + 3);
+}
+
+template <int EM>
+__global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
+{
+    __shared__ float4 tile[PL_CAP];
+    __shared__ int4 s_ubox[PL_BATCH];
+    __shared__ int s_kr[PROJ_ANGLES][2];
+    // per-wave boxes of a batch of steps, computed while no step is staged
+    int4* s_wbox = reinterpret_cast<int4*>(tile);
+    static_assert(PROJ_ANGLES * PL_BATCH * sizeof(int4) <= sizeof(float4) * PL_CAP, "box scratch exceeds the tile");
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int a0 = blockIdx.x * PROJ_ANGLES;
+    const int ang = a0 + wave;
+    const bool have = ang < a.nangles;
+    const int rc = blockIdx.y;
+    const int g = blockIdx.z;
+    // staged orientation, chosen once per workgroup from its middle angle:
+    // rows = Y, cols = X (image [Y][X]), or rows = X, cols = Y (transpose)
+    const int amid = min(a0 + PROJ_ANGLES / 2, a.nangles - 1);
+    const bool steep = fabs(a.sinv[amid]) > fabs(a.cosv[amid]) && a.imgT != nullptr;
+    const int nRow = steep ? a.Nx : a.Ny, nCol = steep ? a.Ny : a.Nx;
+    const float4* __restrict__ src = (steep ? a.imgT : a.img) + (size_t)g * ((size_t)a.Nx * a.Ny);
+
+    const double cs = have ? a.cosv[ang] : 1.0, sn = have ? a.sinv[ang] : 0.0;
+    const double bxk = -sn * a.ry, byi = sn * a.rx;
+    // X = (int)fma(kr, bxk, fma(ir, cs, ucx)), Y = (int)fma(kr, cs, fma(ir, byi, ucy)),
+    // renamed to row / col: row = (int)fma(kr, rK, fma(ir, rI, rU)), col likewise
+    const double rI = steep ? cs : byi, rK = steep ? bxk : cs, rU = steep ? a.ucx : a.ucy;
+    const double cI = steep ? byi : cs, cK = steep ? cs : bxk, cU = steep ? a.ucy : a.ucx;
+
+    const int r = lane & (PL_RAYS - 1), kp = lane >> 5; // 32 rays x 2 sample phases
+    const int nrb = (a.nrays + PL_RAYS - 1) / PL_RAYS;
+
+    double klsum[4] = {0.0, 0.0, 0.0, 0.0};
+    unsigned long long exc[4] = {0, 0, 0, 0};
+
+    for (int rb = rc; rb < nrb; rb += a.RC) {
+        const int ib = rb * PL_RAYS;
+        const int i = ib + r;
+        const bool ray_ok = have && i < a.nrays;
+        const double ir = (double)i - a.ci;
+        const double rowi = fma(ir, rI, rU), coli = fma(ir, cI, cU);
+        // a superset of the k with in-image samples (exact bounds are not
+        // needed: out-of-image samples read zeros), clamped to [0, nk)
+        int kl = 0x7fffffff, kh = -1;
+        if (ray_ok) {
+            double klo = 0.0, khi = (double)a.nk;
+            k_interval(rK, rowi, -1.0, (double)nRow, a.ck, klo, khi);
+            k_interval(cK, coli, -1.0, (double)nCol, a.ck, klo, khi);
+            if (khi >= klo) {
+                kl = (int)fmax(floor(klo) - 1.0, 0.0);
+                kh = (int)fmin(ceil(khi) + 2.0, (double)a.nk);
+            }
+        }
         for (int off = 32; off > 0; off >>= 1) {
-            d += __shfl_xor(d, off);
-            e += __shfl_xor(e, off);
+            kl = min(kl, __shfl_xor(kl, off));
+            kh = max(kh, __shfl_xor(kh, off));
         }
-        const int sl = g * 4 + s;
-        if (lane == 0 && sl < a.S) {
-            a.kl_part[(size_t)sl * a.nangles + ang] = d;
-            a.ex_part[(size_t)sl * a.nangles + ang] = e;
+        __syncthreads(); // the previous ray block is done with s_kr, s_ubox and the tile
+        if (lane == 0) {
+            s_kr[wave][0] = kl;
+            s_kr[wave][1] = kh;
         }
+        __syncthreads();
+        int Kmin = 0x7fffffff, Kmax = -1;
+#pragma unroll
+        for (int w = 0; w < PROJ_ANGLES; ++w) {
+            Kmin = min(Kmin, s_kr[w][0]);
+            Kmax = max(Kmax, s_kr[w][1]);
+        }
+        const int wkl = s_kr[wave][0], wkh = s_kr[wave][1];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+
+        for (int kb = Kmin; kb < Kmax; kb += PL_K * PL_BATCH) {
+            const int nst = min(PL_BATCH, (Kmax - kb + PL_K - 1) / PL_K);
+            // boxes of this batch of steps: lane = step of its own wave, then
+            // the union over the waves, one thread per step
+            {
+                const int ks = kb + lane * PL_K;
+                int4 bx = make_int4(0, 0, 0, 0);
+                if (have && lane < nst && ks < wkh && ks + PL_K > wkl)
+                    bx = patch_box((double)ib - a.ci, (double)ks - a.ck, rU, rI, rK, cU, cI, cK);
+                // the tile (aliased by s_wbox) is free: the last step ended with a barrier
+                s_wbox[wave * PL_BATCH + lane] = bx;
+                __syncthreads();
+                if (threadIdx.x < nst) {
+                    int4 u = make_int4(0x7fffffff, -0x7fffffff, 0x7fffffff, -0x7fffffff);
+                    bool any = false;
+#pragma unroll
+                    for (int w = 0; w < PROJ_ANGLES; ++w) {
+                        const int4 b = s_wbox[w * PL_BATCH + threadIdx.x];
+                        if (b.y > b.x) {
+                            u.x = min(u.x, b.x);
+                            u.y = max(u.y, b.y);
+                            u.z = min(u.z, b.z);
+                            u.w = max(u.w, b.w);
+                            any = true;
+                        }
+                    }
+                    s_ubox[threadIdx.x] = any ? u : make_int4(0, 0, 0, 0);
+                }
+                __syncthreads();
+            }
+            for (int st = 0; st < nst; ++st) {
+                const int ks = kb + st * PL_K;
+                const int4 u = s_ubox[st];
+                int nr = u.y - u.x;
+                const int w = u.w - u.z;
+                if (nr <= 0) continue; // no wave has samples here (uniform)
+                const int pitch = w + ((2 - w) & 7);
+                // the launch was admitted only if every union fits; the clamp
+                // keeps every LDS access in bounds regardless
+                nr = min(nr, PL_CAP / pitch);
+                // stage rows [u.x, u.x + nr) x cols [u.z, u.w): one wave per row,
+                // four rows' loads in flight before their LDS writes
+                {
+                    const int C = u.z + lane;
+                    const bool cin = lane < w && (unsigned)C < (unsigned)nCol;
+                    for (int rr0 = wave; rr0 < nr; rr0 += 4 * PROJ_ANGLES) {
+                        float4 v[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int rr = rr0 + q * PROJ_ANGLES;
+                            const int R = u.x + rr;
+                            v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                            if (cin && rr < nr && (unsigned)R < (unsigned)nRow) v[q] = src[(size_t)R * nCol + C];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int rr = rr0 + q * PROJ_ANGLES;
+                            if (lane < w && rr < nr) tile[rr * pitch + lane] = v[q];
+                        }
+                    }
+                    // columns past the first 64 (boxes wider than a wave)
+                    for (int rr = wave; rr < nr && w > 64; rr += PROJ_ANGLES) {
+                        const int R = u.x + rr;
+                        for (int cc = 64 + lane; cc < w; cc += 64) {
+                            const int C2 = u.z + cc;
+                            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                            if ((unsigned)R < (unsigned)nRow && (unsigned)C2 < (unsigned)nCol) v = src[(size_t)R * nCol + C2];
+                            tile[rr * pitch + cc] = v;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (have && ks < wkh && ks + PL_K > wkl) {
+                    const int off = u.x * pitch + u.z;
+                    const unsigned cap = (unsigned)(PL_CAP - 1);
+                    double kr = (double)(ks + kp) - a.ck;
+                    if (ks + PL_K <= a.nk) {
+#pragma unroll 8
+                        for (int j = 0; j < PL_K / 2; ++j) {
+                            const int R = (int)fma(kr, rK, rowi);
+                            const int C = (int)fma(kr, cK, coli);
+                            const float4 v = tile[min((unsigned)(R * pitch + C - off), cap)];
+                            acc = add4(acc, v);
+                            kr += 2.0;
+                        }
+                    } else {
+                        const int jmax = (a.nk - ks - kp + 1) >> 1; // samples ks + kp + 2j < nk
+                        for (int j = 0; j < jmax; ++j) {
+                            const int R = (int)fma(kr, rK, rowi);
+                            const int C = (int)fma(kr, cK, coli);
+                            const float4 v = tile[min((unsigned)(R * pitch + C - off), cap)];
+                            acc = add4(acc, v);
+                            kr += 2.0;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // combine the two sample phases of each ray (lanes r and r + 32)
+        acc.x += __shfl_xor(acc.x, 32);
+        acc.y += __shfl_xor(acc.y, 32);
+        acc.z += __shfl_xor(acc.z, 32);
+        acc.w += __shfl_xor(acc.w, 32);
+        if (ray_ok && kp == 0) ray_out<EM>(a, g, ang, i, acc, klsum, exc);
     }
+    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc);
 }
 
 // kl[s*K + iter] = sum of the slice's partials, fixed order
@@ -304,13 +547,75 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     k.ci = p.g.ci; k.ck = p.g.ck; k.ucx = p.g.ucx; k.ucy = p.g.ucy; k.ry = p.g.ry; k.rx = p.g.rx;
     k.scale = p.g.scale;
     k.tlo = p.tlo; k.W = p.W;
+    k.RC = p.lds_rc > 0 ? p.lds_rc : 1;
+    k.kl_stride = p.g.nangles * k.RC;
     const int G = (p.S + 3) / 4;
     if (G > 65535) fail(RAFT_EUNSUPPORTED, "projector grid too large");
-    dim3 grid((p.g.nangles + PROJ_ANGLES - 1) / PROJ_ANGLES, G);
-    if (em_mode == 2) hipLaunchKernelGGL((k_project<2>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
-    else if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
-    else hipLaunchKernelGGL((k_project<0>), grid, dim3(64 * PROJ_ANGLES), 0, st, k);
+    const int nab = (p.g.nangles + PROJ_ANGLES - 1) / PROJ_ANGLES;
+    const dim3 block(64 * PROJ_ANGLES);
+    if (p.lds_rc > 0) {
+        const dim3 grid(nab, k.RC, G);
+        if (em_mode == 2) hipLaunchKernelGGL((k_project_lds<2>), grid, block, 0, st, k);
+        else if (em_mode) hipLaunchKernelGGL((k_project_lds<1>), grid, block, 0, st, k);
+        else hipLaunchKernelGGL((k_project_lds<0>), grid, block, 0, st, k);
+    } else {
+        const dim3 grid(nab, G);
+        if (em_mode == 2) hipLaunchKernelGGL((k_project<2>), grid, block, 0, st, k);
+        else if (em_mode) hipLaunchKernelGGL((k_project<1>), grid, block, 0, st, k);
+        else hipLaunchKernelGGL((k_project<0>), grid, block, 0, st, k);
+    }
     RAFT_HIP(hipGetLastError());
+}
+
+int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool have_T)
+{
+    const char* env = std::getenv("RAFT_PROJ_LDS");
+    if (env && env[0] == '0') return 0;
+    const int A = g.nangles;
+    if (A <= 0 || g.nrays <= 0 || S <= 0) return 0;
+    // worst union box of a chunk: the patch extent plus the spread of the
+    // patch position over the chunk's angles; the spread (max - min of affine
+    // functions) is convex in the position, so its maximum is at a corner of
+    // the (ray, sample) domain
+    const double irs[2] = {-g.ci, (double)(g.nrays - 1) - g.ci};
+    const double krs[2] = {-g.ck, (double)(g.nk - 1) - g.ck};
+    for (int a0 = 0; a0 < A; a0 += PROJ_ANGLES) {
+        const int a1 = std::min(A, a0 + PROJ_ANGLES);
+        const int am = std::min(a0 + PROJ_ANGLES / 2, A - 1);
+        const bool steep = have_T && std::fabs(s[am]) > std::fabs(c[am]);
+        double ext_r = 0.0, ext_c = 0.0, spr_r = 0.0, spr_c = 0.0;
+        for (int a = a0; a < a1; ++a) {
+            const double bxk = -s[a] * g.ry, byi = s[a] * g.rx;
+            const double rI = steep ? c[a] : byi, rK = steep ? bxk : c[a];
+            const double cI = steep ? byi : c[a], cK = steep ? c[a] : bxk;
+            ext_r = std::max(ext_r, (PL_RAYS - 1) * std::fabs(rI) + (PL_K - 1) * std::fabs(rK));
+            ext_c = std::max(ext_c, (PL_RAYS - 1) * std::fabs(cI) + (PL_K - 1) * std::fabs(cK));
+        }
+        for (double ir : irs)
+            for (double kr : krs) {
+                double rmin = 1e300, rmax = -1e300, cmin = 1e300, cmax = -1e300;
+                for (int a = a0; a < a1; ++a) {
+                    const double bxk = -s[a] * g.ry, byi = s[a] * g.rx;
+                    const double fr = steep ? ir * c[a] + kr * bxk : ir * byi + kr * c[a];
+                    const double fc = steep ? ir * byi + kr * c[a] : ir * c[a] + kr * bxk;
+                    rmin = std::min(rmin, fr); rmax = std::max(rmax, fr);
+                    cmin = std::min(cmin, fc); cmax = std::max(cmax, fc);
+                }
+                spr_r = std::max(spr_r, rmax - rmin);
+                spr_c = std::max(spr_c, cmax - cmin);
+            }
+        // kernel box: [floor(lo) - 1, floor(hi) + 3) -> at most floor(hi - lo) + 5
+        const double rows = std::floor(spr_r + ext_r) + 5.0, cols = std::floor(spr_c + ext_c) + 5.0;
+        if (!(rows < 4096.0 && cols < 4096.0)) return 0;
+        const int ic = (int)cols;
+        const long long pitch = ic + ((2 - ic) & 7);
+        if ((long long)rows * pitch > PL_CAP) return 0;
+    }
+    // ray chunks: enough workgroups to fill the device several times
+    const long long nrb = (g.nrays + PL_RAYS - 1) / PL_RAYS;
+    const long long wgs = (long long)((A + PROJ_ANGLES - 1) / PROJ_ANGLES) * ((S + 3) / 4);
+    const long long rc = std::max(1LL, std::min(nrb, (2048 + wgs - 1) / wgs));
+    return (int)rc;
 }
 
 void launch_kl_reduce(const double* part, const unsigned long long* expart, int nb, int S, double* kl,

@@ -599,6 +599,7 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
     while (B > 4 && (size_t)B * img * 8 > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
     float* il = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
     float* ilT = static_cast<float*>(D.workspace(2, (size_t)B * img * 4));
+    const int prc = proj_plan(g, cv.data(), sv.data(), B, true);
     for (int s0 = 0; s0 < nslices; s0 += B) {
         const int nb = std::min(B, nslices - s0);
         const float* src = d_obj + (size_t)s0 * img;
@@ -608,7 +609,7 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
         p.img = il; p.imgT = ilT; p.out = d_proj + (size_t)s0 * nangles * nrays;
         p.y = nullptr; p.wa = nullptr; p.kl_part = nullptr; p.ex_part = nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
-        p.g = g; p.S = nb; p.tlo = 0; p.W = nrays;
+        p.g = g; p.S = nb; p.tlo = 0; p.W = nrays; p.lds_rc = prc;
         ProfScope ps(KF_PROJ, st);
         launch_project(p, 0, st);
     }
@@ -687,8 +688,10 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
     float* ones = static_cast<float*>(D.workspace(3, img * 4 + (size_t)A * g.width * 16));
     float* onerows = ones + img;
-    double* part = static_cast<double*>(D.workspace(4, (size_t)B * A * 16));
-    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * A);
+    const int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
+    const int kslots = proj_kl_slots(A, prc);
+    double* part = static_cast<double*>(D.workspace(4, (size_t)B * kslots * 16));
+    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * kslots);
     const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
     double* dkl = nullptr;
     unsigned long long* dex = nullptr;
@@ -719,12 +722,12 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         p.kl_part = want_kl ? part : nullptr;
         p.ex_part = want_kl ? expart : nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
-        p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
+        p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width; p.lds_rc = prc;
         {
             ProfScope ps(KF_PROJ, s);
             launch_project(p, 1, s);
         }
-        if (want_kl) launch_kl_reduce(part, expart, A, nb, klo, exo, K, it, graph ? counter : nullptr, s);
+        if (want_kl) launch_kl_reduce(part, expart, kslots, nb, klo, exo, K, it, graph ? counter : nullptr, s);
         if (kl_only) return;
         BpArgs b;
         b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
@@ -862,6 +865,7 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
     float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
     float* normil = static_cast<float*>(D.workspace(3, (size_t)Gb * img * 16));
     float* tmp = static_cast<float*>(D.workspace(5, (size_t)B * img * 4));
+    const int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
 
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
@@ -883,7 +887,7 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
             p.img = xil; p.imgT = xT; p.out = rows; p.y = d_flat + (size_t)s0 * R; p.wa = dwa.as<float>();
             p.kl_part = nullptr; p.ex_part = nullptr;
             p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
-            p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width;
+            p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width; p.lds_rc = prc;
             {
                 ProfScope ps(KF_PROJ, st);
                 launch_project(p, 2, st);
@@ -922,6 +926,7 @@ struct BatchBucket {
     // eEM
     BpGeom ge;
     RayGeom rg;
+    int prc = 0; // staged projector plan (ray chunks), 0 = direct form
     DevBuf dc, ds, ecA, ecB, ewa, erows, xil, xT, ones, y, x;
     hipStream_t s = nullptr;
     hipEvent_t done = nullptr;
@@ -1017,6 +1022,7 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             upload(b->ecB, g.cB, st);
             upload(b->ewa, wa, st);
             b->rg = ray_geom_em(N, A, N);
+            b->prc = proj_plan(b->rg, cv.data(), sv.data(), S, true);
             const int G = (S + 3) / 4;
             b->erows.alloc((size_t)G * A * g.width * 16);
             b->xil.alloc((size_t)G * img * 16);
@@ -1082,7 +1088,7 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
                 p.img = b->xil.as<float>(); p.imgT = b->xT.as<float>(); p.out = b->erows.as<float>();
                 p.y = b->y.as<float>(); p.wa = b->ewa.as<float>(); p.kl_part = nullptr; p.ex_part = nullptr;
                 p.cosv = b->dc.as<double>(); p.sinv = b->ds.as<double>();
-                p.g = b->rg; p.S = S; p.tlo = g.tlo; p.W = g.width;
+                p.g = b->rg; p.S = S; p.tlo = g.tlo; p.W = g.width; p.lds_rc = b->prc;
                 launch_project(p, 1, s);
                 BpArgs bu;
                 bu.q = b->erows.as<float>(); bu.out = b->xil.as<float>(); bu.x = b->xil.as<float>(); bu.norm = ones;

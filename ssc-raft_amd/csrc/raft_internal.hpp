@@ -167,9 +167,16 @@ struct ProjArgs {
     RayGeom g;
     int S;
     int tlo, W;         // eEM ratio-row layout
+    int lds_rc = 0;     // > 0: staged form with this many ray chunks (from proj_plan); 0: direct form
 };
-// em_mode 0 = Radon, 1 = eEM ratio (+ KL), 2 = tEM flat * exp(-v)
+// em_mode 0 = Radon, 1 = eEM ratio (+ KL), 2 = tEM flat * exp(-v).
+// KL partials: kl_part[s * proj_kl_slots(a) + slot], reduced by launch_kl_reduce.
 void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
+// Staged-form plan for S slices and host angle tables c, s: the number of ray
+// chunks, or 0 when a chunk of 8 angles cannot be staged (sparse angles) or
+// RAFT_PROJ_LDS=0.  have_T: the transposed image will be passed as imgT.
+int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool have_T);
+inline int proj_kl_slots(int nangles, int lds_rc) { return nangles * (lds_rc > 0 ? lds_rc : 1); }
 
 // planar [S][rows][cols] -> interleaved [G][rows][cols][4] (or its transpose [G][cols][rows][4])
 void launch_to_interleaved(const float* in, float* out, int S, int rows, int cols, bool transpose, hipStream_t st);

@@ -155,6 +155,47 @@ def test_radon_random_angles(gpu):
     assert rel(out, ref) <= FP32_TOL
 
 
+@pytest.mark.parametrize("form", ["default", "direct"])
+@pytest.mark.parametrize("ny,nx,a", [(64, 64, 64), (80, 96, 100), (128, 128, 256), (200, 200, 200), (31, 31, 40)])
+def test_radon_projector_forms(gpu, monkeypatch, form, ny, nx, a):
+    """Dense angle sets (all the cases here) take the staged (LDS) projector by
+    default, and RAFT_PROJ_LDS=0 forces the direct one: both against the FP64
+    oracle.  Sparse sets (test_radon's (100, 100, 64)) take the direct one."""
+    import sscRaft
+
+    if form == "direct":
+        monkeypatch.setenv("RAFT_PROJ_LDS", "0")
+    rng = np.random.default_rng(ny + nx + a)
+    img = rng.random((5, ny, nx)).astype(np.float32)
+    ang = ph.uniform_angles(a)
+    out = sscRaft.radon_RT(img, ang, [0], pixel=2.0 / nx)
+    ref = oracle.radon(img, ang)
+    e = rel(out, ref)
+    print(f"radon {form} {ny}x{nx} a={a}: rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
+@pytest.mark.parametrize("n,a,k", [(64, 64, 5), (40, 48, 3)])
+def test_eem_projector_forms_agree(gpu, monkeypatch, n, a, k):
+    """eEM with the staged projector (default for these dense sets) and with
+    the direct one (forced) agree to FP32 rounding, and both track the oracle."""
+    import sscRaft
+
+    img = ph.phantoms(n, 3, config=3, first=n + 7)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n + 7), 0)
+    x1, kl1, ex1 = sscRaft.eEMRT_GPU_KL(y, ang, k, [0], 0)
+    monkeypatch.setenv("RAFT_PROJ_LDS", "0")
+    x2, kl2, ex2 = sscRaft.eEMRT_GPU_KL(y, ang, k, [0], 0)
+    xr, klr, exr = oracle.eem(y, ang, k)
+    print(f"eem forms n={n} a={a}: staged {rel(x1, xr):.2e}, direct {rel(x2, xr):.2e}, between {rel(x1, x2):.2e}")
+    assert rel(x1, xr) <= FP32_TOL and rel(x2, xr) <= FP32_TOL
+    assert rel(x1, x2) <= FP32_TOL
+    assert float(np.max(np.abs(kl1 - klr) / np.abs(klr))) <= 1e-4
+    np.testing.assert_array_equal(ex1, exr)
+    np.testing.assert_array_equal(ex2, exr)
+
+
 @pytest.mark.parametrize("n,a,k", [(16, 16, 1), (33, 17, 5), (64, 64, 20), (47, 60, 5)])
 def test_eem(gpu, n, a, k):
     import sscRaft
@@ -283,7 +324,7 @@ def test_lowpass_device_in_place_and_offset_ignored(gpu):
     assert rel(t.cpu().numpy(), oracle.lowpass(tomo, pf, pi)) <= FP32_TOL
 
 
-@pytest.mark.parametrize("n,a,k", [(32, 30, 1), (48, 40, 4), (33, 20, 3)])
+@pytest.mark.parametrize("n,a,k", [(32, 30, 1), (48, 40, 4), (33, 20, 3), (40, 48, 3)])
 def test_tem(gpu, n, a, k):
     """get_tEM_RT_MultiGPU vs the oracle (slice-own flat rows)."""
     import sscRaft
