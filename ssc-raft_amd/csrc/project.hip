@@ -19,10 +19,10 @@
 // samples read staged zeros, the inner loop has no range test.  The gathers
 // of a direct form are limited by the L1 tag rate (a wave's 64 gathers touch
 // ~38 distinct 64-byte lines); staged, a step costs ~17 lines per staged row
-// and the gathers become LDS reads.  Rows are staged at a pitch = 2 (mod 8)
-// 16-byte entries, so rays stepping along a row, down a column or along a
-// diagonal fall into distinct LDS banks.  Near-vertical chunks stage from
-// the transposed image, so that neighbouring rays always step along a row.
+// and the gathers become LDS reads.  Rows are staged at an odd pitch
+// (= 1 mod 4 entries of 16 bytes; fewer bank conflicts measured than with
+// even pitches), and near-vertical chunks stage from the transposed image,
+// so that neighbouring rays always step mostly along a staged row.
 // The host admits a launch to this form only when every chunk's union box
 // fits PL_CAP entries at every patch position (proj_plan); sparse angle sets
 // use the direct form.
@@ -423,26 +423,26 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                 int nr = u.y - u.x;
                 const int w = u.w - u.z;
                 if (nr <= 0) continue; // no wave has samples here (uniform)
-                const int pitch = w + ((2 - w) & 7);
+                const int pitch = w + ((1 - w) & 3);
                 // the launch was admitted only if every union fits; the clamp
                 // keeps every LDS access in bounds regardless
                 nr = min(nr, PL_CAP / pitch);
                 // stage rows [u.x, u.x + nr) x cols [u.z, u.w): one wave per row,
-                // four rows' loads in flight before their LDS writes
+                // eight rows' loads in flight before their LDS writes
                 {
                     const int C = u.z + lane;
                     const bool cin = lane < w && (unsigned)C < (unsigned)nCol;
-                    for (int rr0 = wave; rr0 < nr; rr0 += 4 * PROJ_ANGLES) {
-                        float4 v[4];
+                    for (int rr0 = wave; rr0 < nr; rr0 += 8 * PROJ_ANGLES) {
+                        float4 v[8];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
+                        for (int q = 0; q < 8; ++q) {
                             const int rr = rr0 + q * PROJ_ANGLES;
                             const int R = u.x + rr;
                             v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
                             if (cin && rr < nr && (unsigned)R < (unsigned)nRow) v[q] = src[(size_t)R * nCol + C];
                         }
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) {
+                        for (int q = 0; q < 8; ++q) {
                             const int rr = rr0 + q * PROJ_ANGLES;
                             if (lane < w && rr < nr) tile[rr * pitch + lane] = v[q];
                         }
@@ -460,28 +460,35 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                 }
                 __syncthreads();
                 if (have && ks < wkh && ks + PL_K > wkl) {
-                    const int off = u.x * pitch + u.z;
-                    const unsigned cap = (unsigned)(PL_CAP - 1);
+                    // byte offset of sample (R, C) in the tile: R * pitch16 - off16 + C * 16
+                    // (24-bit multiply-add, then shift-add), clamped to the tile (the box holds
+                    // every sample; the clamp is a guard)
+                    const int pitch16 = pitch << 4;
+                    const int moff16 = -((u.x * pitch + u.z) << 4);
+                    const unsigned cap16 = (unsigned)(PL_CAP - 1) << 4;
+                    const char* tb = reinterpret_cast<const char*>(tile);
                     double kr = (double)(ks + kp) - a.ck;
-                    if (ks + PL_K <= a.nk) {
+                    float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f); // this step's sum (two-level summation)
+                    const int jmax = ks + PL_K <= a.nk ? PL_K / 2 : (a.nk - ks - kp + 1) >> 1; // ks + kp + 2j < nk
+                    if (jmax == PL_K / 2) {
 #pragma unroll 8
                         for (int j = 0; j < PL_K / 2; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const float4 v = tile[min((unsigned)(R * pitch + C - off), cap)];
-                            acc = add4(acc, v);
+                            const unsigned e = min(((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16), cap16);
+                            sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
                     } else {
-                        const int jmax = (a.nk - ks - kp + 1) >> 1; // samples ks + kp + 2j < nk
                         for (int j = 0; j < jmax; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const float4 v = tile[min((unsigned)(R * pitch + C - off), cap)];
-                            acc = add4(acc, v);
+                            const unsigned e = min(((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16), cap16);
+                            sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
                     }
+                    acc = add4(acc, sacc);
                 }
                 __syncthreads();
             }
@@ -608,7 +615,7 @@ int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool ha
         const double rows = std::floor(spr_r + ext_r) + 5.0, cols = std::floor(spr_c + ext_c) + 5.0;
         if (!(rows < 4096.0 && cols < 4096.0)) return 0;
         const int ic = (int)cols;
-        const long long pitch = ic + ((2 - ic) & 7);
+        const long long pitch = ic + ((1 - ic) & 3);
         if ((long long)rows * pitch > PL_CAP) return 0;
     }
     // ray chunks: enough workgroups to fill the device several times
