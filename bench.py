@@ -410,10 +410,11 @@ def unet_bench(args, torch):
     torch.cuda.synchronize()
     tt = (time.perf_counter() - t0) / 5
     r["train_step_b8_ms"] = tt * 1e3
-    r["train_step_b8_tflops"] = 3 * net.flops_forward(8, 512, 512) / tt / 1e12
+    r["train_step_b8_tflops"] = 3 * net.flops_forward(8, 512, 512, fused=False) / tt / 1e12
     r["roofline"] = {"bound": "mfma", "achieved": r["forward_b32_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": r["forward_b32_tflops"] / 2500.0,
-                     "note": "forward pass, batch 32, FLOP = 2*9*Cin*Cout per output pixel over the 14 layers"}
+                     "note": "fused inference forward pass, batch 32, FLOP = 2*9*Cin*Cout per output pixel over the "
+                             "14 layers (first layer Cin = 1); training counts 3x the padded-input forward"}
     log(f"U-Net 512^2: forward {r['forward_b32_tflops']:.0f} TFLOP/s (batch 32), training step "
         f"{r['train_step_b8_tflops']:.0f} TFLOP/s (batch 8)")
     return r

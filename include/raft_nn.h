@@ -38,6 +38,23 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
                     unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
                     void* stream);
 
+/* raft_nn_conv3x3 with its input assembled on the fly (inference path):
+ * input channels [0, split) come from x (channel stride ldx, offset cx0) --
+ * at H x W, or, when pool != 0, as the 2x2 average of x at 2H x 2W (the values
+ * raft_nn_pool2 would write); channels [split, Cin) come from x2 (ldx2, cx2)
+ * at H/2 x W/2 with nearest 2x upsampling (the values raft_nn_up2 would write).
+ * split is a multiple of 32 in [32, Cin]; split < Cin needs x2 and even H, W. */
+int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, const unsigned short* x2, int ldx2,
+                       int cx2, int split, const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0,
+                       int N, int H, int W, int Cin, int Cout, int relu, int variant, void* stream);
+
+/* First layer from the FP32 image [N][H][W] (one input channel):
+ * y[p][cy0 + co] = act(b[co] + sum_tap bf16(img[p + off(tap)] * scale) * w[(co * 9 + tap) * ldw]),
+ * zero padding; w is the layer's [Cout][9][ldw] bf16 weights of which input
+ * channel 0 is used.  Cout a multiple of 8, at most 64; ldy, cy0 multiples of 8. */
+int raft_nn_conv_first(const float* img, float scale, const unsigned short* w, int ldw, const float* b,
+                       unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cout, int relu, void* stream);
+
 /* wt[ci][8 - tap][co] = w[co][tap][ci]: backward-data weights, used as
  * raft_nn_conv3x3(g, ..., wt, ...) with Cin = Cout of the forward layer. */
 int raft_nn_conv_weight_flip(const unsigned short* w, unsigned short* wt, int Cout, int Cin, void* stream);

@@ -80,6 +80,29 @@ __global__ void k_mfma_probe(const u16* A16, const u16* B16, float* out16, const
     for (int i = 0; i < 16; ++i) out32[l * 16 + i] = d[i];
 }
 
+// 8 bf16 channels (16 bytes) <-> 8 floats.  The elementwise kernels move 8
+// channels per thread; every channel count, stride and offset they see is a
+// multiple of 8.
+__device__ __forceinline__ void unpack8(uint4 v, float f[8])
+{
+    const unsigned e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        f[2 * j] = __uint_as_float(e[j] << 16);
+        f[2 * j + 1] = __uint_as_float(e[j] & 0xffff0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8])
+{
+    unsigned e[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) e[j] = (unsigned)bf16_bits(f[2 * j]) | ((unsigned)bf16_bits(f[2 * j + 1]) << 16);
+    return make_uint4(e[0], e[1], e[2], e[3]);
+}
+__device__ __forceinline__ uint4 ld8(const u16* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ __forceinline__ void st8(u16* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+
+
 // ---------------------------------------------------------------------------
 // 3x3 convolution, forward (and backward-data with flipped weights):
 //   y[n][h][w][cy0 + co] = act(b[co] + sum_{dy,dx,ci} x[n][h+dy][w+dx][cx0 + ci] W[co][tap][ci])
@@ -98,7 +121,43 @@ struct ConvArgs {
     const float* b;
     u16* y;
     int N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu;
+    // fused input sources: input channels [0, split) come from x -- at H x W,
+    // or (pool) as the 2x2 average of x at 2H x 2W; channels [split, Cin) come
+    // from x2 at H/2 x W/2, read with nearest 2x upsampling (a skip
+    // concatenation [x | up2(x2)] that is never written out)
+    const u16* x2;
+    int ldx2, cx2, split, pool;
 };
+
+// 16-byte unit (8 input channels c .. c+7) of the convolution input at pixel
+// (gy, gx) of image n, inside the image.  c and split are multiples of 8 and
+// split a multiple of 32, so the branch is uniform over a 32-channel chunk.
+// The pooled and upsampled values equal k_pool2 / k_up2 outputs bit for bit.
+__device__ __forceinline__ uint4 conv_src(const ConvArgs& a, int n, int gy, int gx, int c)
+{
+    if (c >= a.split) {
+        const int Hl = a.H >> 1, Wl = a.W >> 1;
+        const size_t p = ((size_t)n * Hl + (gy >> 1)) * Wl + (gx >> 1);
+        return *reinterpret_cast<const uint4*>(a.x2 + p * a.ldx2 + a.cx2 + (c - a.split));
+    }
+    if (a.pool) {
+        const int W2 = 2 * a.W;
+        const size_t base = ((size_t)n * 2 * a.H + 2 * gy) * W2 + 2 * gx;
+        const size_t src[4] = {base, base + 1, base + W2, base + W2 + 1};
+        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unpack8(ld8(a.x + src[k] * a.ldx + a.cx0 + c), f);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += f[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] *= 0.25f;
+        return pack8(acc);
+    }
+    const size_t p = ((size_t)n * a.H + gy) * a.W + gx;
+    return *reinterpret_cast<const uint4*>(a.x + p * a.ldx + a.cx0 + c);
+}
 
 template <int WR, int WC>
 constexpr int conv_lds_bytes()
@@ -137,8 +196,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
             const int pix = u >> 2, q = u & 3;
             const int gy = ty0 + pix / HW - 1, gx = tx0 + pix % HW - 1;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                v = *reinterpret_cast<const uint4*>(a.x + (img + (size_t)gy * a.W + gx) * a.ldx + a.cx0 + c0 + 8 * q);
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, c0 + 8 * q);
             *reinterpret_cast<uint4*>(s_in + pix * CONV_PS + 8 * q) = v;
         }
         for (int u = tid; u < 9 * CO_T * 4; u += 256) {
@@ -249,7 +307,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
         const int t = t0 + it / nch, ch = it % nch;
         const int tx0 = (t % pa.tiles_x) * 32;
         const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
-        const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+        const int n = t / (pa.tiles_x * pa.tiles_y);
 #pragma unroll
         for (int k = 0; k < K::HU; ++k) {
             const int u = tid + 256 * k;
@@ -257,9 +315,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
             if (u < K::HALO_UNITS) {
                 const int pix = u >> 2, q = u & 3;
                 const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
-                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                    v = *reinterpret_cast<const uint4*>(a.x + (img + (size_t)gy * a.W + gx) * a.ldx + a.cx0 + 32 * ch +
-                                                        8 * q);
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, 32 * ch + 8 * q);
             }
             hreg[k] = v;
         }
@@ -543,27 +599,6 @@ struct CView {
     const u16* p;
     int ld, c0;
 };
-
-// The elementwise kernels below move 8 channels (16 bytes) per thread; every
-// channel count, stride and offset they see is a multiple of 8.
-__device__ __forceinline__ void unpack8(uint4 v, float f[8])
-{
-    const unsigned e[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        f[2 * j] = __uint_as_float(e[j] << 16);
-        f[2 * j + 1] = __uint_as_float(e[j] & 0xffff0000u);
-    }
-}
-__device__ __forceinline__ uint4 pack8(const float f[8])
-{
-    unsigned e[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) e[j] = (unsigned)bf16_bits(f[2 * j]) | ((unsigned)bf16_bits(f[2 * j + 1]) << 16);
-    return make_uint4(e[0], e[1], e[2], e[3]);
-}
-__device__ __forceinline__ uint4 ld8(const u16* p) { return *reinterpret_cast<const uint4*>(p); }
-__device__ __forceinline__ void st8(u16* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
 
 // out = g * (act > 0)
 __global__ void k_relu_mask(CView g, CView act, View out, long long P, int C)
@@ -855,6 +890,53 @@ int grid_for(long long n, int block = 256)
     return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
 }
 
+// ---------------------------------------------------------------------------
+// First 3x3 convolution straight from the FP32 image (one input channel):
+//   y[p][cy0 + co] = act(b[co] + sum_tap bf16(img[p + off(tap)] * scale) W[co][tap][0])
+// with W read from a [Cout][9][ldw] bf16 tensor (the layer's weights, whose
+// other input channels multiply the zero-padded channels of the generic
+// path).  One thread per pixel, 8 output channels per 16-byte store; products
+// of two bf16 values are exact in FP32 and are summed over taps in order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_first(const float* img, float scale, const u16* w, int ldw,
+                                                   const float* b, u16* y, int ldy, int cy0, int N, int H, int W,
+                                                   int Cout, int relu)
+{
+    __shared__ float s_w[9 * 64];
+    __shared__ float s_b[64];
+    for (int e = threadIdx.x; e < 9 * Cout; e += blockDim.x) {
+        const int co = e / 9, tap = e % 9;
+        s_w[tap * Cout + co] = bf16_float(w[((size_t)co * 9 + tap) * ldw]);
+    }
+    for (int e = threadIdx.x; e < Cout; e += blockDim.x) s_b[e] = b ? b[e] : 0.0f;
+    __syncthreads();
+    const long long P = (long long)N * H * W;
+    for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(p % W), yy = (int)((p / W) % H);
+        float in[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const int gy = yy + dy, gx = x + dx;
+            in[tap] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? bf16_float(bf16_bits(img[p + dy * W + dx] * scale))
+                                                                : 0.0f;
+        }
+        u16* yp = y + p * ldy + cy0;
+        for (int c8 = 0; c8 < Cout; c8 += 8) {
+            float o[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int tap = 0; tap < 9; ++tap) acc += in[tap] * s_w[tap * Cout + c8 + j];
+                float v = acc + s_b[c8 + j];
+                o[j] = relu ? fmaxf(v, 0.0f) : v;
+            }
+            st8(yp + c8, pack8(o));
+        }
+    }
+}
+
 hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 
 int launched() { return hipGetLastError() == hipSuccess ? RAFT_OK : RAFT_EHIP; }
@@ -932,6 +1014,50 @@ int raft_nn_mfma_probe(const unsigned short* A16, const unsigned short* B16, flo
     return launched();
 }
 
+static int conv_dispatch(const ConvArgs& a, int variant, hipStream_t st)
+{
+    // measured on gfx950 (batch 8, 512^2 .. 64^2 layers): the persistent 16x32
+    // form is fastest or within 2% everywhere except the wide-input 64-channel
+    // decoder layer, where the one-tile 8x64 form wins
+    if (variant == 0) variant = (a.Cout % 64 == 0 && a.Cin >= 192 && a.Cout < 128) ? 1 : 6;
+    switch (variant) {
+    case 6:
+        return launch_convp<4, 1>(a, st);
+    case 7:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_convp<2, 2>(a, st);
+    case 8:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_convp<4, 2>(a, st);
+    case 9:
+        return launch_convp<2, 1>(a, st);
+    case 10:
+        return launch_convp<4, 1, false>(a, st);
+    case 11:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_convp<2, 2, false>(a, st);
+    case 12:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_convp<4, 2, false>(a, st);
+    case 13:
+        return launch_convp<2, 1, false>(a, st);
+    case 1:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_conv<2, 2>(a, st);
+    case 2:
+        return launch_conv<4, 1>(a, st);
+    case 3:
+        return launch_conv<1, 1>(a, st);
+    case 4:
+        if (a.Cout % 64) return RAFT_EINVAL;
+        return launch_conv<4, 2>(a, st);
+    case 5:
+        return launch_conv<2, 1>(a, st);
+    default:
+        return RAFT_EINVAL;
+    }
+}
+
 int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b,
                     unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
                     void* stream)
@@ -939,47 +1065,33 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
     if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cout < 32 || Cin % 32 || Cout % 32 || ldx % 8 || cx0 % 8 ||
         ldy % 4 || cy0 % 4 || cx0 + Cin > ldx || cy0 + Cout > ldy)
         return RAFT_EINVAL;
-    ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu};
-    // measured on gfx950 (batch 8, 512^2 .. 64^2 layers): the persistent 16x32
-    // form is fastest or within 2% everywhere except the wide-input 64-channel
-    // decoder layer, where the one-tile 8x64 form wins
-    if (variant == 0) variant = (Cout % 64 == 0 && Cin >= 192 && Cout < 128) ? 1 : 6;
-    switch (variant) {
-    case 6:
-        return launch_convp<4, 1>(a, S(stream));
-    case 7:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_convp<2, 2>(a, S(stream));
-    case 8:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_convp<4, 2>(a, S(stream));
-    case 9:
-        return launch_convp<2, 1>(a, S(stream));
-    case 10:
-        return launch_convp<4, 1, false>(a, S(stream));
-    case 11:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_convp<2, 2, false>(a, S(stream));
-    case 12:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_convp<4, 2, false>(a, S(stream));
-    case 13:
-        return launch_convp<2, 1, false>(a, S(stream));
-    case 1:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_conv<2, 2>(a, S(stream));
-    case 2:
-        return launch_conv<4, 1>(a, S(stream));
-    case 3:
-        return launch_conv<1, 1>(a, S(stream));
-    case 4:
-        if (Cout % 64) return RAFT_EINVAL;
-        return launch_conv<4, 2>(a, S(stream));
-    case 5:
-        return launch_conv<2, 1>(a, S(stream));
-    default:
+    ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu, nullptr, 0, 0, Cin, 0};
+    return conv_dispatch(a, variant, S(stream));
+}
+
+int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, const unsigned short* x2, int ldx2,
+                       int cx2, int split, const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0,
+                       int N, int H, int W, int Cin, int Cout, int relu, int variant, void* stream)
+{
+    if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cout < 32 || Cin % 32 || Cout % 32 || ldx % 8 || cx0 % 8 ||
+        ldy % 4 || cy0 % 4 || cy0 + Cout > ldy || split < 32 || split > Cin || split % 32 || cx0 + split > ldx)
         return RAFT_EINVAL;
-    }
+    if (split < Cin && (!x2 || H % 2 || W % 2 || ldx2 % 8 || cx2 % 8 || cx2 + (Cin - split) > ldx2))
+        return RAFT_EINVAL;
+    ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu, split < Cin ? x2 : nullptr, ldx2, cx2, split,
+               pool ? 1 : 0};
+    return conv_dispatch(a, variant, S(stream));
+}
+
+int raft_nn_conv_first(const float* img, float scale, const unsigned short* w, int ldw, const float* b,
+                       unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cout, int relu, void* stream)
+{
+    if (N < 1 || H < 1 || W < 1 || Cout < 8 || Cout > 64 || Cout % 8 || ldw < 1 || ldy % 8 || cy0 % 8 ||
+        cy0 + Cout > ldy)
+        return RAFT_EINVAL;
+    hipLaunchKernelGGL(k_conv_first, dim3(grid_for((long long)N * H * W)), dim3(256), 0, S(stream), img, scale, w, ldw,
+                       b, y, ldy, cy0, N, H, W, Cout, relu);
+    return launched();
 }
 
 int raft_nn_conv_weight_flip(const unsigned short* w, unsigned short* wt, int Cout, int Cin, void* stream)

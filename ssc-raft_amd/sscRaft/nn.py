@@ -31,6 +31,9 @@ _ll = ctypes.c_longlong
 _SIGNATURES = {
     "raft_nn_mfma_probe": ([_vp] * 7, _i),
     "raft_nn_conv3x3": ([_vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
+    "raft_nn_conv3x3_ex": ([_vp, _i, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp],
+                           _i),
+    "raft_nn_conv_first": ([_vp, _f, _vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
     "raft_nn_conv_weight_flip": ([_vp, _vp, _i, _i, _vp], _i),
     "raft_nn_wgrad_workspace": ([_i, _i, _i, _i, _i], _ll),
     "raft_nn_conv3x3_wgrad": ([_vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp, _ll, _i, _i, _i, _i, _i, _i, _vp], _i),
@@ -123,6 +126,8 @@ class UNet:
                      for name, co, ci in self.layers}
         self._bufs = None
         self._shape = None
+        self._fbufs = None
+        self._fshape = None
         self.tuned = {}  # (n, h, w, Cin, Cout) -> conv tile variant (see autotune)
         self.refresh()
 
@@ -214,6 +219,17 @@ class UNet:
         nn_call("raft_nn_conv3x3", x.ptr(), x.ch, cx0, self.params_bf.data_ptr() + 2 * w0,
                 self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, cy0, n, hw[0], hw[1], ci, co, int(relu), v, st)
 
+    def _conv_ex(self, name, x, pool, x2, y, n, hw, st, relu=True):
+        """Fused-input convolution (inference): x pooled 2x2 when `pool`, and
+        input channels past x's own read from x2 upsampled 2x."""
+        co, ci = self.offsets[name][2], self.offsets[name][3]
+        w0, b0 = self.offsets[name][0], self.offsets[name][1]
+        v = self.tuned.get((n, hw[0], hw[1], ci, co), 0)
+        split = ci if x2 is None else x.ch
+        nn_call("raft_nn_conv3x3_ex", x.ptr(), x.ch, 0, int(pool), None if x2 is None else x2.ptr(),
+                0 if x2 is None else x2.ch, 0, split, self.params_bf.data_ptr() + 2 * w0,
+                self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, 0, n, hw[0], hw[1], ci, co, int(relu), v, st)
+
     def _conv_bwd_data(self, name, g, gx, cgx0, n, hw, st):
         co, ci = self.offsets[name][2], self.offsets[name][3]
         v = self.tuned.get((n, hw[0], hw[1], co, ci), 0)
@@ -302,6 +318,46 @@ class UNet:
         self._conv("dec1b", b["a7"], 0, b["d1"], 0, n, L[0], st)
         return b
 
+    def _forward_fused(self, img, n, h, w, st):
+        """Inference forward with the input conversion, pooling, upsampling and
+        skip concatenations folded into the convolutions' input loads: the
+        same values as _forward (pooling / upsampling are bit-exact; the first
+        layer sums its 9 exact products in a different order than the MFMA)."""
+        b = self._fused_buffers(n, h, w)
+        L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
+        w0, b0 = self.offsets["enc1a"][0], self.offsets["enc1a"][1]
+        nn_call("raft_nn_conv_first", img.data_ptr(), 1.0, self.params_bf.data_ptr() + 2 * w0, 32,
+                self.params.data_ptr() + 4 * b0, b["a1"].ptr(), b["a1"].ch, 0, n, h, w, self.base, 1, st)
+        self._conv_ex("enc1b", b["a1"], 0, None, b["e1"], n, L[0], st)
+        self._conv_ex("enc2a", b["e1"], 1, None, b["a2"], n, L[1], st)
+        self._conv_ex("enc2b", b["a2"], 0, None, b["e2"], n, L[1], st)
+        self._conv_ex("enc3a", b["e2"], 1, None, b["a3"], n, L[2], st)
+        self._conv_ex("enc3b", b["a3"], 0, None, b["e3"], n, L[2], st)
+        self._conv_ex("bot_a", b["e3"], 1, None, b["a4"], n, L[3], st)
+        self._conv_ex("bot_b", b["a4"], 0, None, b["e4"], n, L[3], st)
+        self._conv_ex("dec3a", b["e3"], 0, b["e4"], b["a5"], n, L[2], st)
+        self._conv_ex("dec3b", b["a5"], 0, None, b["d3"], n, L[2], st)
+        self._conv_ex("dec2a", b["e2"], 0, b["d3"], b["a6"], n, L[1], st)
+        self._conv_ex("dec2b", b["a6"], 0, None, b["d2"], n, L[1], st)
+        self._conv_ex("dec1a", b["e1"], 0, b["d2"], b["a7"], n, L[0], st)
+        self._conv_ex("dec1b", b["a7"], 0, None, b["d1"], n, L[0], st)
+        return b
+
+    def _fused_buffers(self, n, h, w):
+        if self._fshape == (n, h, w):
+            return self._fbufs
+        if h % 8 or w % 8:
+            raise ValueError("image height and width must be multiples of 8 (three 2x2 poolings)")
+        c1, c2, c3, c4 = self.base, 2 * self.base, 4 * self.base, 8 * self.base
+        L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
+        spec = {"a1": (0, c1), "e1": (0, c1), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
+                "a4": (3, c4), "e4": (3, c4), "a5": (2, c3), "d3": (2, c3), "a6": (1, c2), "d2": (1, c2),
+                "a7": (0, c1), "d1": (0, c1)}
+        self._fbufs = None
+        self._fbufs = {k: _Act(self.torch, n, L[lv][0], L[lv][1], ch, self.dev) for k, (lv, ch) in spec.items()}
+        self._fshape = (n, h, w)
+        return self._fbufs
+
     def _check_img(self, img):
         t = self.torch
         if not (isinstance(img, t.Tensor) and img.is_cuda and img.dtype == t.float32 and img.is_contiguous()):
@@ -312,15 +368,17 @@ class UNet:
             raise ValueError("expected [n, h, w]")
         return img
 
-    def forward(self, img, out=None, floor=None, stream=None):
+    def forward(self, img, out=None, floor=None, stream=None, fused=True):
         """pred = img + head(net(img)) for img [n, h, w] float32 on the device.
 
         floor: clamp the prediction from below (an eEM start must be positive).
+        fused: fold input conversion, pooling, upsampling and concatenations
+        into the convolutions (False: the training path's separate kernels).
         """
         img = self._check_img(img)
         n, h, w = img.shape
         st = _stream(stream)
-        b = self._forward(img, n, h, w, st)
+        b = self._forward_fused(img, n, h, w, st) if fused else self._forward(img, n, h, w, st)
         if out is None:
             out = self.torch.empty_like(img)
         h0, hb = self.offsets["head"][0], self.offsets["head"][1]
@@ -425,11 +483,15 @@ class UNet:
         self.adam_step(lr, stream=stream)
         return loss
 
-    def flops_forward(self, n, h, w):
-        """Multiply-add FLOPs (x2) of the 3x3 convolutions of one forward pass."""
+    def flops_forward(self, n, h, w, fused=True):
+        """Multiply-add FLOPs (x2) of the 3x3 convolutions of one forward pass.
+        fused: the first layer has its one real input channel (the training
+        path convolves the 32-channel zero-padded input)."""
         lv = {"enc1": 0, "dec1": 0, "enc2": 1, "dec2": 1, "enc3": 2, "dec3": 2, "bot_": 3}
         tot = 0
         for name, co, ci in self.layers:
             s = 2 ** lv[name[:4]]
+            if fused and name == "enc1a":
+                ci = 1
             tot += 2 * 9 * co * ci * n * (h // s) * (w // s)
         return tot

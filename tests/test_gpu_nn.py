@@ -108,6 +108,66 @@ def test_conv3x3_forward(gpu, variant, n, h, w, ci, co, ldx, cx0, ldy, cy0):
     assert np.all(outside == 1.0), "wrote outside its channel window"
 
 
+@pytest.mark.parametrize("variant", [2, 5, 6, 9, 10])
+def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
+    """raft_nn_conv3x3_ex reading a 2x2-pooled input, or a skip concatenation
+    [x | up2(x2)] assembled in its input load, gives bit for bit what
+    raft_nn_pool2 / raft_nn_up2 followed by raft_nn_conv3x3 give."""
+    from sscRaft.nn import nn_call
+
+    rng = np.random.default_rng(40 + variant)
+    n, h, w = 2, 10, 36
+    # pooled input: x at 2h x 2w with 48 channels, the 32 used at offset 16
+    x = val(bits(rng.standard_normal((n, 2 * h, 2 * w, 48)).astype(np.float32)))
+    _, wt, b = conv_case(rng, n, h, w, 32, 64)
+    xd, wd, bd = dev_bits(x), dev_bits(wt), torch.from_numpy(b).cuda()
+    p = torch.zeros(n * h * w * 32, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_pool2", xd.data_ptr(), 48, 16, p.data_ptr(), 32, 0, n, h, w, 32, stream())
+    y1 = torch.zeros(n * h * w * 64, dtype=torch.int16, device="cuda")
+    y2 = torch.ones(n * h * w * 64, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv3x3", p.data_ptr(), 32, 0, wd.data_ptr(), bd.data_ptr(), y1.data_ptr(), 64, 0, n, h, w,
+            32, 64, 1, variant, stream())
+    nn_call("raft_nn_conv3x3_ex", xd.data_ptr(), 48, 16, 1, None, 0, 0, 32, wd.data_ptr(), bd.data_ptr(),
+            y2.data_ptr(), 64, 0, n, h, w, 32, 64, 1, variant, stream())
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2), "pooled input"
+    # skip concatenation: 32 skip channels + 64 upsampled channels -> 96
+    skip = val(bits(rng.standard_normal((n, h, w, 32)).astype(np.float32)))
+    low = val(bits(rng.standard_normal((n, h // 2, w // 2, 64)).astype(np.float32)))
+    _, wt2, b2 = conv_case(rng, n, h, w, 96, 32)
+    cat = np.concatenate([skip, low.repeat(2, axis=1).repeat(2, axis=2)], axis=-1)
+    sd, ld, cd = dev_bits(skip), dev_bits(low), dev_bits(cat)
+    wd2, bd2 = dev_bits(wt2), torch.from_numpy(b2).cuda()
+    z1 = torch.zeros(n * h * w * 32, dtype=torch.int16, device="cuda")
+    z2 = torch.ones(n * h * w * 32, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv3x3", cd.data_ptr(), 96, 0, wd2.data_ptr(), bd2.data_ptr(), z1.data_ptr(), 32, 0, n, h, w,
+            96, 32, 1, variant, stream())
+    nn_call("raft_nn_conv3x3_ex", sd.data_ptr(), 32, 0, 0, ld.data_ptr(), 64, 0, 32, wd2.data_ptr(), bd2.data_ptr(),
+            z2.data_ptr(), 32, 0, n, h, w, 96, 32, 1, variant, stream())
+    torch.cuda.synchronize()
+    assert torch.equal(z1, z2), "skip concatenation"
+
+
+def test_conv_first_layer(gpu):
+    """raft_nn_conv_first (FP32 image, one input channel) against the float64
+    convolution of the bf16-rounded image with input channel 0 of the weights."""
+    from sscRaft.nn import nn_call
+
+    rng = np.random.default_rng(8)
+    n, h, w, co = 2, 17, 40, 32
+    img = rng.random((n, h, w)).astype(np.float32) * 3.0
+    _, wt, b = conv_case(rng, n, h, w, 32, co)
+    imd, wd, bd = torch.from_numpy(img).cuda(), dev_bits(wt), torch.from_numpy(b).cuda()
+    y = torch.zeros(n * h * w * 40, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv_first", imd.data_ptr(), 0.5, wd.data_ptr(), 32, bd.data_ptr(), y.data_ptr(), 40, 8,
+            n, h, w, co, 1, stream())
+    got = host_val(y).reshape(n, h, w, 40)
+    x1 = val(bits(img * np.float32(0.5)))[..., None]
+    ref = torch_conv(x1, wt[:, :, :1], b, relu=True)
+    assert_bf16_close(got[..., 8:8 + co], ref, "first layer")
+    assert np.all(got[..., :8] == 0) and np.all(got[..., 8 + co:] == 0)
+
+
 def test_conv3x3_backward_data(gpu):
     from sscRaft.nn import nn_call
 
@@ -315,9 +375,15 @@ def test_unet_loss_and_gradients_match_reference(gpu):
         print(f"  grad {name}: rel-L2 {e:.2e}")
         worst = max(worst, e)
     assert worst <= 5e-2
-    # forward-only path gives the same prediction
-    p2 = net.forward(ti)
+    # forward-only path gives the same prediction (same kernels as training)
+    p2 = net.forward(ti, fused=False)
     np.testing.assert_allclose(p2.cpu().numpy(), pred.cpu().numpy(), rtol=0, atol=1e-6)
+    # the fused inference path: pooling / upsampling identical, first layer
+    # summed in another order (one bf16 rounding may differ here and there)
+    p3 = net.forward(ti)
+    ef = np.linalg.norm(p3.cpu().numpy() - p2.cpu().numpy()) / np.linalg.norm(p2.cpu().numpy() - img)
+    print(f"fused forward vs unfused: rel-L2 of the head output {ef:.2e}")
+    assert ef <= 1e-2
 
 
 def test_unet_training_reduces_loss(gpu):
