@@ -25,6 +25,8 @@ Also reported on the same JSON line:
                 in ssc-raft_amd/sscRaft/weights/ (null if they are absent);
   c1            one 512^2 slice (512 angles): GPU call time and the FP32 host
                 restatement's time;
+  next_rows     low-pass filter export, BST FBP and flat/dark correction at
+                64 x 2048^2, transmission EM at 16 x 1024^2;
   batch         256 uneven scans (N in {256..1024}): FBP + 200 eEM iterations
                 of all of them in one captured graph launch;
   unet          the warm start U-Net's forward / training throughput against
@@ -258,6 +260,74 @@ def fbp_bench(args, torch, dv, ph):
     return res, roof, cpu, radon, pcie
 
 
+def next_rows_bench(args, torch, dv, ph):
+    """The rows next to the hot path, device-resident, at the headline shapes:
+    low-pass filter export, BST FBP and flat/dark correction on 64 x 2048 x
+    2048 data, transmission EM on 16 x 1024^2 (1024 angles)."""
+    from sscRaft.geometries.parallel.fbp_methods import fbp_params
+
+    dev = torch.device("cuda:0")
+    reps = max(2, min(args.steps, 5))
+
+    def timed(fn, reset=None):
+        if reset:
+            reset()
+        fn()
+        torch.cuda.synchronize()
+        tot = 0.0
+        for _ in range(reps):
+            if reset:
+                reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t0
+        return tot / reps
+
+    r = {}
+    N = A = 2048
+    S = 64
+    ang = ph.uniform_angles(A)
+    img = ph.phantoms(N, S, config=2, device=dev)
+    y = ph.poisson_log_torch(dv.radon_device(img, ang), 1.0e4, seed=2000)
+    del img
+    paramf, parami = fbp_params(N, A, S, fbp_dic(N))
+    io = 2.0 * S * A * N * 4
+    work = torch.empty_like(y)
+    dt = timed(lambda: dv.lowpass_device(work, paramf, parami), reset=lambda: work.copy_(y))
+    r["lowpass"] = dict(config="64 x 2048 angles x 2048 bins, pad 2, ramp, in place", seconds=dt,
+                        slices_per_s=S / dt, hbm_gbps_in_out=io / dt / 1e9,
+                        roofline_frac_in_out=io / dt / HBM_PEAK)
+    out = torch.empty((S, N, N), dtype=torch.float32, device=dev)
+    dt = timed(lambda: dv.bst_device(y, paramf, parami, out=out))
+    r["bst"] = dict(config="64 x 2048 angles x 2048 bins -> 64 x 2048^2, pad 2, ramp", seconds=dt,
+                    slices_per_s=S / dt, padded_size=N * 3)
+    del out
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    raw = torch.rand((S, A, N), generator=g, device=dev) * 900.0 + 100.0
+    flat = torch.rand((S, 2, N), generator=g, device=dev) * 100.0 + 1000.0
+    dark = torch.rand((S, N), generator=g, device=dev) * 10.0
+    dt = timed(lambda: dv.background_device(work, flat, dark, is_log=True), reset=lambda: work.copy_(raw))
+    r["background"] = dict(config="64 x 2048 x 2048 frames, 2 flats blended by angle, dark, -log, in place",
+                           seconds=dt, gb_per_s=io / dt / 1e9, roofline_frac=io / dt / HBM_PEAK)
+    del raw, flat, dark, work, y
+    n3 = A3 = 1024
+    S3, K = 16, 20
+    ang3 = ph.uniform_angles(A3)
+    img3 = ph.phantoms(n3, S3, config=3, device=dev) * 0.5
+    count = torch.exp(-ph.poisson_log_torch(dv.radon_device(img3, ang3), 1.0e4, seed=3000)).contiguous()
+    flat3 = torch.ones((S3, n3), dtype=torch.float32, device=dev)
+    obj = torch.ones((S3, n3, n3), dtype=torch.float32, device=dev)
+    dt = timed(lambda: dv.tem_device(count, flat3, ang3, K, obj=obj), reset=lambda: obj.fill_(1.0))
+    r["tem"] = dict(config=f"16 x 1024^2, 1024 angles x 1024 bins, {K} iterations", seconds=dt,
+                    ms_per_iteration=1e3 * dt / K, samples_per_s=2.0 * S3 * A3 * n3 * n3 * K / dt)
+    log(f"next rows: lowpass {r['lowpass']['slices_per_s']:.0f} slices/s, BST {r['bst']['slices_per_s']:.0f} "
+        f"slices/s, background {r['background']['gb_per_s']:.0f} GB/s, tEM {r['tem']['ms_per_iteration']:.2f} ms/it")
+    return r
+
+
 def em_bench(args, torch, dv, ph):
     N = A = 1024
     S = 16
@@ -449,9 +519,10 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--no-next", action="store_true", help="skip the low-pass / BST / flat-dark / tEM timings")
     ap.add_argument("--no-em", action="store_true")
     ap.add_argument("--em-iters", type=int, default=200)
-    ap.add_argument("--only", choices=["all", "fbp", "em", "batch"], default="all",
+    ap.add_argument("--only", choices=["all", "fbp", "em", "batch", "next"], default="all",
                     help="profiling runs: time only one workload (the JSON line keeps the same fields)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of the 64 x 2048 x 2048 sinogram (known byte count) so "
@@ -474,6 +545,9 @@ def main():
     if args.only == "batch":
         print(json.dumps({"metric": METRIC, "batch": batch_bench(args, torch, dv, ph)}), flush=True)
         return
+    if args.only == "next":
+        print(json.dumps({"metric": METRIC, "next_rows": next_rows_bench(args, torch, dv, ph)}), flush=True)
+        return
     if args.only == "em":
         em = em_bench(args, torch, dv, ph)
         if not args.no_learned:
@@ -485,6 +559,7 @@ def main():
     if em is not None and not args.no_learned:
         em["learned_init"] = learned_bench(args)
     c1 = None if args.only == "fbp" else c1_bench(args, torch, dv, ph)
+    nxt = None if (args.only == "fbp" or args.no_next) else next_rows_bench(args, torch, dv, ph)
     unet = None if args.only == "fbp" else unet_bench(args, torch)
     batch = None if (args.no_batch or args.only == "fbp") else batch_bench(args, torch, dv, ph)
     r32 = res["f32"]
@@ -513,6 +588,7 @@ def main():
         "radon": radon,
         "em": em,
         "c1": c1,
+        "next_rows": nxt,
         "unet": unet,
         "batch": batch,
     }

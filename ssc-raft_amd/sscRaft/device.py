@@ -5,7 +5,8 @@ import numpy
 
 from .rafttypes import CNICE, libraft, raft_check, ctypes
 
-__all__ = ["fbp_device", "radon_device", "eem_device", "set_profiling", "last_kernel_ms", "fbp_plan_info",
+__all__ = ["fbp_device", "radon_device", "eem_device", "eem_kl_device", "batch_device", "lowpass_device",
+           "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info",
            "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
 
 KF_FILTER, KF_BP, KF_PROJ, KF_EMBP = 0, 1, 2, 3
@@ -137,6 +138,62 @@ def lowpass_device(tomo, paramf, parami, stream=None):
     if rc:
         raft_check("raft_lowpass_device")
     return tomo
+
+
+def bst_device(tomo, paramf, parami, out=None, stream=None):
+    """FBP by the Backprojection Slice Theorem (raft_bst_device) on tomo [S, A, N]
+    (angles uniform on [0, pi)); returns out [S, N, N]."""
+    import torch
+
+    _check_tensor(tomo, "tomo")
+    S, A, N = tomo.shape
+    if out is None:
+        out = torch.empty((S, N, N), dtype=torch.float32, device=tomo.device)
+    _check_tensor(out, "out")
+    paramf = CNICE(numpy.asarray(paramf), numpy.float32)
+    parami = CNICE(numpy.asarray(parami), numpy.int32)
+    rc = libraft.raft_bst_device(tomo.device.index or 0, ctypes.c_void_p(out.data_ptr()),
+                                 ctypes.c_void_p(tomo.data_ptr()), paramf.ctypes.data_as(ctypes.c_void_p),
+                                 parami.ctypes.data_as(ctypes.c_void_p), _stream(tomo, stream))
+    if rc:
+        raft_check("raft_bst_device")
+    return out
+
+
+def tem_device(count, flat, angles, iterations, obj=None, stream=None):
+    """Transmission EM (raft_tem_device): count [S, A, R] (= exp(-data)), flat
+    [S, R], obj [S, n, n] in/out (ones when None); returns obj."""
+    import torch
+
+    _check_tensor(count, "count")
+    _check_tensor(flat, "flat")
+    S, A, R = count.shape
+    if obj is None:
+        obj = torch.ones((S, R, R), dtype=torch.float32, device=count.device)
+    _check_tensor(obj, "obj")
+    angles = CNICE(numpy.asarray(angles))
+    parami = CNICE(numpy.array([R, A, S, obj.shape[-1], 0, 0, 0, 1, iterations, 0]), numpy.int32)
+    rc = libraft.raft_tem_device(count.device.index or 0, ctypes.c_void_p(obj.data_ptr()),
+                                 ctypes.c_void_p(count.data_ptr()), ctypes.c_void_p(flat.data_ptr()),
+                                 angles.ctypes.data_as(ctypes.c_void_p), parami.ctypes.data_as(ctypes.c_void_p),
+                                 _stream(count, stream))
+    if rc:
+        raft_check("raft_tem_device")
+    return obj
+
+
+def background_device(frames, flat, dark, is_log=False, stream=None):
+    """Flat/dark correction in place on frames [S, A, R] (raft_background_device),
+    flat [S, F, R], dark [S, R]; returns frames."""
+    for t, name in ((frames, "frames"), (flat, "flat"), (dark, "dark")):
+        _check_tensor(t, name)
+    S, A, R = frames.shape
+    rc = libraft.raft_background_device(frames.device.index or 0, ctypes.c_void_p(frames.data_ptr()),
+                                        ctypes.c_void_p(flat.data_ptr()), ctypes.c_void_p(dark.data_ptr()), R, A, S,
+                                        flat.shape[1], int(bool(is_log)), _stream(frames, stream))
+    if rc:
+        raft_check("raft_background_device")
+    return frames
 
 
 def eem_kl_device(tomo, angles, obj, stream=None):
