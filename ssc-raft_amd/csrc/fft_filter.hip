@@ -456,25 +456,28 @@ __global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int 
 }
 
 // out[sl][x][y] = in[sl][y][x] for s x s complex planes (32 x 32 LDS tiles)
-__global__ __launch_bounds__(256) void k_transpose_c(const float2* in, float2* out, int s)
+// out[sl][c][y] = in[sl][y][c0 + c] for the nc columns [c0, c0 + nc) of an
+// s x s slice: the columns the crop keeps, as rows for the second 1-D pass
+__global__ __launch_bounds__(256) void k_transpose_c(const float2* in, float2* out, int s, int c0, int nc)
 {
     __shared__ float2 t[32][33];
     const int sl = blockIdx.z;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
     const float2* src = in + (size_t)sl * s * s;
-    float2* dst = out + (size_t)sl * s * s;
+    float2* dst = out + (size_t)sl * nc * s;
     for (int k = threadIdx.x; k < 32 * 32; k += 256) {
         const int dy = k / 32, dx = k % 32;
-        if (y0 + dy < s && x0 + dx < s) t[dy][dx] = src[(size_t)(y0 + dy) * s + x0 + dx];
+        if (y0 + dy < s && x0 + dx < nc) t[dy][dx] = src[(size_t)(y0 + dy) * s + c0 + x0 + dx];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 32 * 32; k += 256) {
         const int dx = k / 32, dy = k % 32;
-        if (y0 + dy < s && x0 + dx < s) dst[(size_t)(x0 + dx) * s + y0 + dy] = t[dy][dx];
+        if (y0 + dy < s && x0 + dx < nc) dst[(size_t)(x0 + dx) * s + y0 + dy] = t[dy][dx];
     }
 }
 
-// obj[sl][j][i] = Re(R[sl][i + L][j + L]) / scale, R the transposed 2-D inverse FFT
+// obj[sl][j][i] = Re(R[sl][i][j + L]) / scale, R the inverse FFT along the
+// rows of the kept columns ([sl][N][s])
 __global__ void k_bst_extract(const float2* R, float* obj, int s, int N, int L, float scale, int S)
 {
     const long long total = (long long)S * N * N;
@@ -482,7 +485,7 @@ __global__ void k_bst_extract(const float2* R, float* obj, int s, int N, int L, 
          e += (long long)gridDim.x * blockDim.x) {
         const int i = (int)(e % N), j = (int)((e / N) % N);
         const long long sl = e / ((long long)N * N);
-        obj[e] = R[(sl * s + i + L) * (long long)s + j + L].x / scale;
+        obj[e] = R[(sl * N + i) * (long long)s + j + L].x / scale;
     }
 }
 
@@ -596,9 +599,11 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
     hipLaunchKernelGGL(k_bst_p2c, dim3((unsigned)std::min<long long>((cells + 255) / 256, 65536)), dim3(256), 0, st,
                        a.polar, a.cart, n, 2 * a.A, s, a.S);
     RAFT_HIP(hipGetLastError());
+    // 2-D inverse FFT: all rows, then only the N columns [L, L + N) the crop
+    // keeps (transposed to rows); the other columns are never needed
     for (int pass = 0; pass < 2; ++pass) {
         float2* d = pass == 0 ? a.cart : a.cartT;
-        const long long rows = (long long)a.S * s;
+        const long long rows = (long long)a.S * (pass == 0 ? s : a.N);
         if (n > 8192) launch_ifft_t<1024, 16>(d, a.tw, p, rows, lds, st);
         else {
             switch (plan.threads) {
@@ -611,8 +616,8 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
         }
         RAFT_HIP(hipGetLastError());
         if (pass == 0) {
-            dim3 g((s + 31) / 32, (s + 31) / 32, a.S);
-            hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s);
+            dim3 g((a.N + 31) / 32, (s + 31) / 32, a.S);
+            hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s, a.L, a.N);
             RAFT_HIP(hipGetLastError());
         }
     }
