@@ -131,7 +131,7 @@ template <> struct Dft<5> : DftGeneric<5> {};
 
 // One Stockham pass (decimation in time, autosorting):
 //   butterfly j (0 <= j < n/R), jm = j mod Ns:
-//   in  x_r = z[j + r n/R] * W_{Ns R}^{jm r}
+//   in  x_r = z[j + r n/R] * W_{Ns R}^{jm r}   (W^{jm r} = (W^{jm})^r)
 //   out z[(j - jm) R + jm + r Ns] = DFT_R(x)_r
 // n / THREADS <= E, so a thread holds at most ceil(E/R) butterflies.
 // Ns is a power of two for every pass before the first odd radix (radices
@@ -149,12 +149,15 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns)
         const int j = threadIdx.x + b * THREADS;
         if (j < nb) {
             const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
-            const int ts = jm * twstep;
+            // W^(r ts) = (W^ts)^r: one table twiddle per butterfly, the rest by
+            // recurrence (at most R - 2 extra roundings)
+            const float2 w1 = tw(jm * twstep);
+            float2 w = w1;
+            v[b][0] = z[pix(j)];
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float2 x = z[pix(j + r * nb)];
-                if (r) x = cmul(x, tw(r * ts));
-                v[b][r] = x;
+            for (int r = 1; r < R; ++r) {
+                v[b][r] = cmul(z[pix(j + r * nb)], w);
+                if (r + 1 < R) w = cmul(w, w1);
             }
         }
     }
@@ -336,8 +339,9 @@ void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipSt
     hipLaunchKernelGGL((k_filter_fft<OT, THREADS, E>), dim3(blocks), dim3(THREADS), lds, st, k, p);
 }
 
-// THREADS = max(64, pow2 >= n/8) up to 1024 with E = 8 elements per thread;
-// n in (8192, 16384] runs 1024 threads with E = 16
+// THREADS = the smallest of 64, 128, 192, 256, 384, 512, 768, 1024 that is
+// >= n/8, with E = 8 elements per thread; n in (8192, 16384] runs 1024
+// threads with E = 16
 template <typename OT>
 void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
 {
@@ -348,8 +352,11 @@ void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t
     switch (threads) {
     case 64: launch_fft_t<OT, 64, 8>(k, p, blocks, lds, st); break;
     case 128: launch_fft_t<OT, 128, 8>(k, p, blocks, lds, st); break;
+    case 192: launch_fft_t<OT, 192, 8>(k, p, blocks, lds, st); break;
     case 256: launch_fft_t<OT, 256, 8>(k, p, blocks, lds, st); break;
+    case 384: launch_fft_t<OT, 384, 8>(k, p, blocks, lds, st); break;
     case 512: launch_fft_t<OT, 512, 8>(k, p, blocks, lds, st); break;
+    case 768: launch_fft_t<OT, 768, 8>(k, p, blocks, lds, st); break;
     default: launch_fft_t<OT, 1024, 8>(k, p, blocks, lds, st); break;
     }
 }
@@ -532,8 +539,15 @@ FftPlan fft_plan(int n)
         }
     }
     if (m != 1) return p;
-    int t = 64;
-    while (t < 1024 && t * 8 < n) t *= 2;
+    // threads = n / 8 when that is an allowed block size, so every radix-8
+    // butterfly of a pass has its own thread (6144 -> 768, 3072 -> 384)
+    static const int tsz[] = {64, 128, 192, 256, 384, 512, 768, 1024};
+    int t = 1024;
+    for (int c : tsz)
+        if (c * 8 >= n) {
+            t = c;
+            break;
+        }
     p.threads = t;
     p.ok = true;
     return p;
@@ -589,8 +603,11 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
         switch (plan.threads) {
         case 64: launch_bst_polar_t<64, 8>(k, p, lds, st); break;
         case 128: launch_bst_polar_t<128, 8>(k, p, lds, st); break;
+        case 192: launch_bst_polar_t<192, 8>(k, p, lds, st); break;
         case 256: launch_bst_polar_t<256, 8>(k, p, lds, st); break;
+        case 384: launch_bst_polar_t<384, 8>(k, p, lds, st); break;
         case 512: launch_bst_polar_t<512, 8>(k, p, lds, st); break;
+        case 768: launch_bst_polar_t<768, 8>(k, p, lds, st); break;
         default: launch_bst_polar_t<1024, 8>(k, p, lds, st); break;
         }
     }
@@ -609,8 +626,11 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
             switch (plan.threads) {
             case 64: launch_ifft_t<64, 8>(d, a.tw, p, rows, lds, st); break;
             case 128: launch_ifft_t<128, 8>(d, a.tw, p, rows, lds, st); break;
+            case 192: launch_ifft_t<192, 8>(d, a.tw, p, rows, lds, st); break;
             case 256: launch_ifft_t<256, 8>(d, a.tw, p, rows, lds, st); break;
+            case 384: launch_ifft_t<384, 8>(d, a.tw, p, rows, lds, st); break;
             case 512: launch_ifft_t<512, 8>(d, a.tw, p, rows, lds, st); break;
+            case 768: launch_ifft_t<768, 8>(d, a.tw, p, rows, lds, st); break;
             default: launch_ifft_t<1024, 8>(d, a.tw, p, rows, lds, st); break;
             }
         }
