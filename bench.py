@@ -365,6 +365,36 @@ def em_bench(args, torch, dv, ph):
              kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms)
     log(f"EM plain: {K} iterations in {dt:.2f} s ({r['ms_per_iteration']:.2f} ms/it), KL {r['kl_first']:.4e} -> "
         f"{r['kl_last']:.4e}; projector {proj_ms:.3f} ms, backprojection+update {embp_ms:.3f} ms")
+    if not args.no_cpu:
+        r["cpu_baseline"] = cpu_baseline_em(torch, dv, y[:1].contiguous(), ang)
+    return r
+
+
+def cpu_baseline_em(torch, dv, y1, ang, K=5):
+    """eEM CPU-baseline leg: the FP32 restatement in oracle/ timed on one C3
+    slice for one iteration, and the FP64 oracle as the checker of K GPU
+    iterations (iterate and per-iteration KL) on the same slice."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    yh = y1.cpu().numpy()
+    cores = host_cores()
+    t0 = time.perf_counter()
+    oracle.eem(yh, ang, 1, prec="f32", with_kl=False, nthreads=cores)
+    dt = time.perf_counter() - t0
+    xg, klg, exg = dv.eem_device(y1, ang, K)
+    xg = xg.cpu().numpy()
+    t0 = time.perf_counter()
+    xr, klr, exr = oracle.eem(yh, ang, K, nthreads=cores)
+    dto = time.perf_counter() - t0
+    e = _err(xg, xr)
+    r = dict(value=1.0 / dt, unit="slice-iterations/s", cores=cores, kind="port",
+             sample="1 slice of the 16 (1024^2, 1024 angles), 1 iteration, FP32 C restatement",
+             errors_vs_fp64_oracle=dict(slices=1, iterations=K, iterate=e,
+                                        kl_max_rel=float(np.max(np.abs(klg - klr) / np.abs(klr))),
+                                        excluded_equal=bool(np.array_equal(exg, exr))))
+    log(f"EM CPU: {r['value']:.3f} slice-iterations/s; GPU vs FP64 oracle after {K} iterations: rel-L2 "
+        f"{e['rel_l2']:.2e}, KL max rel {r['errors_vs_fp64_oracle']['kl_max_rel']:.2e} ({dto:.1f} s)")
     return r
 
 

@@ -301,6 +301,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--weights", default=None, help="output (train) / input (evaluate) .npz")
+    ap.add_argument("--init", default=None, help="train: start from these weights instead of a random init")
     ap.add_argument("--log", default=None, help="JSON log file")
     args = ap.parse_args(argv)
     path = args.weights or default_weights_path()
@@ -312,7 +313,8 @@ def main(argv=None):
     if args.command == "train":
         inp, target, _, _, info = make_pairs(args.count, WarmStartConfig.train_first, log=log)
         record["pairs"] = {"count": args.count, "best_iter_median": float(np.median(info["best_iter"]))}
-        net = UNet(seed=args.seed)
+        net = UNet.load(args.init) if args.init else UNet(seed=args.seed)
+        record["init"] = args.init
         t0 = time.time()
         hist = train(net, inp, target, args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
         record["train_seconds"] = time.time() - t0
