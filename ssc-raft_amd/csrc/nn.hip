@@ -29,6 +29,10 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+// native 16-byte vector for values held in registers across a loop (the
+// struct-with-union uint4 defeats scalar replacement there and lands in scratch)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 as_u32x4(uint4 v) { return u32x4{v.x, v.y, v.z, v.w}; }
 typedef unsigned short u16;
 
 __device__ __forceinline__ u16 bf16_bits(float f)
@@ -302,8 +306,8 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
             bias[c][i] = a.b ? a.b[co] : 0.0f;
         }
 
-    uint4 hreg[K::HU], wreg[K::WU];
-    auto load_item = [&](int it, bool weights) {
+    u32x4 hreg[K::HU], wreg[K::WU];
+    auto load_item = [&](int it, bool weights) __attribute__((always_inline)) {
         const int t = t0 + it / nch, ch = it % nch;
         const int tx0 = (t % pa.tiles_x) * 32;
         const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
                 const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
                 if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, 32 * ch + 8 * q);
             }
-            hreg[k] = v;
+            hreg[k] = as_u32x4(v);
         }
         if (weights) {
 #pragma unroll
@@ -325,17 +329,17 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
                 const int u = tid + 256 * k;
                 if (u < K::W_UNITS) {
                     const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
-                    wreg[k] = *reinterpret_cast<const uint4*>(a.w + ((size_t)(co0 + co) * 9 + tap) * a.Cin + 32 * ch +
+                    wreg[k] = *reinterpret_cast<const u32x4*>(a.w + ((size_t)(co0 + co) * 9 + tap) * a.Cin + 32 * ch +
                                                               8 * q);
                 }
             }
         }
     };
-    auto store_item = [&](bool weights) {
+    auto store_item = [&](bool weights) __attribute__((always_inline)) {
 #pragma unroll
         for (int k = 0; k < K::HU; ++k) {
             const int u = tid + 256 * k;
-            if (u < K::HALO_UNITS) *reinterpret_cast<uint4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
+            if (u < K::HALO_UNITS) *reinterpret_cast<u32x4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
         }
         if (weights) {
 #pragma unroll
@@ -343,7 +347,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
                 const int u = tid + 256 * k;
                 if (u < K::W_UNITS) {
                     const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
-                    *reinterpret_cast<uint4*>(s_w + (tap * K::CO_T + co) * CONV_PS + 8 * q) = wreg[k];
+                    *reinterpret_cast<u32x4*>(s_w + (tap * K::CO_T + co) * CONV_PS + 8 * q) = wreg[k];
                 }
             }
         }
@@ -1016,10 +1020,10 @@ int raft_nn_mfma_probe(const unsigned short* A16, const unsigned short* B16, flo
 
 static int conv_dispatch(const ConvArgs& a, int variant, hipStream_t st)
 {
-    // measured on gfx950 (batch 8, 512^2 .. 64^2 layers): the persistent 16x32
-    // form is fastest or within 2% everywhere except the wide-input 64-channel
-    // decoder layer, where the one-tile 8x64 form wins
-    if (variant == 0) variant = (a.Cout % 64 == 0 && a.Cin >= 192 && a.Cout < 128) ? 1 : 6;
+    // measured on gfx950 (batch 32, 512^2 .. 64^2 layers): the persistent 8x64
+    // form for 64+ output channels, the persistent 8x32 form (dx loop not
+    // unrolled) for 32
+    if (variant == 0) variant = (a.Cout % 64 == 0) ? 7 : 13;
     switch (variant) {
     case 6:
         return launch_convp<4, 1>(a, st);
