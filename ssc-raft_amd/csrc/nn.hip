@@ -35,13 +35,14 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ u32x4 as_u32x4(uint4 v) { return u32x4{v.x, v.y, v.z, v.w}; }
 typedef unsigned short u16;
 
-__device__ __forceinline__ u16 bf16_bits(float f)
+// FP32 -> bf16, round to nearest even: one v_cvt_pk_bf16_f32 per two values
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned bf16_pack2(float lo, float hi)
 {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u)); // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
-    return (u16)(u >> 16);
+    const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(unsigned, v);
 }
+__device__ __forceinline__ u16 bf16_bits(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 __device__ __forceinline__ float bf16_float(u16 b) { return __uint_as_float((unsigned)b << 16); }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
@@ -98,10 +99,7 @@ __device__ __forceinline__ void unpack8(uint4 v, float f[8])
 }
 __device__ __forceinline__ uint4 pack8(const float f[8])
 {
-    unsigned e[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) e[j] = (unsigned)bf16_bits(f[2 * j]) | ((unsigned)bf16_bits(f[2 * j + 1]) << 16);
-    return make_uint4(e[0], e[1], e[2], e[3]);
+    return make_uint4(bf16_pack2(f[0], f[1]), bf16_pack2(f[2], f[3]), bf16_pack2(f[4], f[5]), bf16_pack2(f[6], f[7]));
 }
 __device__ __forceinline__ uint4 ld8(const u16* p) { return *reinterpret_cast<const uint4*>(p); }
 __device__ __forceinline__ void st8(u16* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
@@ -241,16 +239,16 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int co = co0 + 32 * c + 8 * g + 4 * hi;
-                u16 o[4];
+                float o[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     float v = acc[r][c][4 * g + i] + (a.b ? a.b[co + i] : 0.0f);
                     if (a.relu) v = fmaxf(v, 0.0f);
-                    o[i] = bf16_bits(v);
+                    o[i] = v;
                 }
                 uint2 pk;
-                pk.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-                pk.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+                pk.x = bf16_pack2(o[0], o[1]);
+                pk.y = bf16_pack2(o[2], o[3]);
                 *reinterpret_cast<uint2*>(yp + co) = pk;
             }
     }
@@ -406,16 +404,16 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
 #pragma unroll
                             for (int g = 0; g < 4; ++g) {
                                 const int co = co0 + 32 * c + 8 * g + 4 * hi;
-                                u16 o[4];
+                                float o[4];
 #pragma unroll
                                 for (int i = 0; i < 4; ++i) {
                                     float v = acc[r][c][4 * g + i] + bias[c][4 * g + i];
                                     if (a.relu) v = fmaxf(v, 0.0f);
-                                    o[i] = bf16_bits(v);
+                                    o[i] = v;
                                 }
                                 uint2 pk;
-                                pk.x = (unsigned)o[0] | ((unsigned)o[1] << 16);
-                                pk.y = (unsigned)o[2] | ((unsigned)o[3] << 16);
+                                pk.x = bf16_pack2(o[0], o[1]);
+                                pk.y = bf16_pack2(o[2], o[3]);
                                 *reinterpret_cast<uint2*>(yp + co) = pk;
                             }
                     }
