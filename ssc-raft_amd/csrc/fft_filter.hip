@@ -203,7 +203,7 @@ struct FilterK {
     void* q;
     const float2* H;
     const float2* tw; // [128 + n/128 + 1]: lo table then hi table
-    const float* g;
+    const double* g; // direct-convolution kernel (FP64)
     const float* wa;
     int N, A, S, npad, L, tlo, W, none;
     int SB; // slices per interleaved group
@@ -316,18 +316,20 @@ __global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
     const float wv = a.wa[ix.a];
     for (int w = threadIdx.x; w < a.W; w += 256) {
         const int T = a.tlo + w;
-        float s0 = 0.0f, s1 = 0.0f;
+        // FP64 kernel and sums: the kernel sums to ~0 (no DC), so n FP32
+        // products would lose ~sqrt(n) ulps of a cancelling sum
+        double s0 = 0.0, s1 = 0.0;
         if (T >= 0 && T < a.npad) {
             int d = T; // (T - j) mod n, j = 0..n-1
             for (int j = 0; j < a.npad; ++j) {
-                const float gv = a.g[d];
+                const double gv = a.g[d];
                 const float2 zj = z[pix(j)];
-                s0 = fmaf(gv, zj.x, s0);
-                s1 = fmaf(gv, zj.y, s1);
+                s0 = fma(gv, (double)zj.x, s0);
+                s1 = fma(gv, (double)zj.y, s1);
                 d = (d == 0) ? a.npad - 1 : d - 1;
             }
         }
-        dst[(size_t)w * stride] = Pair2<OT>::make(s0 * wv, s1 * wv);
+        dst[(size_t)w * stride] = Pair2<OT>::make((float)s0 * wv, (float)s1 * wv);
     }
 }
 

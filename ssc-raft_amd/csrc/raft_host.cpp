@@ -246,7 +246,7 @@ static double filter_apply(int type, double w, double reg, double pag)
 // per-device state: lock, cached filter tables, grow-only workspace
 // ---------------------------------------------------------------------------
 struct FilterTables {
-    DevBuf H, tw, g; // float2[n], float2[n], float[n]
+    DevBuf H, tw, g; // float2[n], float2[n], double[n]
     FftPlan plan;
 };
 
@@ -349,7 +349,7 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
     upload(F->tw, tw, st);
     if (!F->plan.ok && c.filter != 0) {
         // g[d] = sum_k H_k exp(2 pi i k d / n): real because H is Hermitian
-        std::vector<float> g(n);
+        std::vector<double> g(n);
         for (int d = 0; d < n; ++d) {
             double acc = 0.0;
             for (int k = 0; k < n; ++k) {
@@ -357,7 +357,7 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
                 const double ang = 2.0 * M_PI * (double)kd / (double)n;
                 acc += Hr[k] * std::cos(ang) - Hi[k] * std::sin(ang);
             }
-            g[d] = (float)acc;
+            g[d] = acc;
         }
         upload(F->g, g, st);
     }
@@ -431,7 +431,7 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
         fa.q = q;
         fa.H = F.H.as<float2>();
         fa.tw = F.tw.as<float2>();
-        fa.g = F.g.as<float>();
+        fa.g = F.g.as<double>();
         fa.wa = dwa.as<float>();
         fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = g.tlo; fa.W = g.width;
         fa.none = (c.filter == 0);
@@ -557,7 +557,7 @@ static void lowpass_device(int dev, float* d_tomo, const float* paramf, const in
         fa.q = q;
         fa.H = F.H.as<float2>();
         fa.tw = F.tw.as<float2>();
-        fa.g = F.g.as<float>();
+        fa.g = F.g.as<double>();
         fa.wa = dwa.as<float>();
         fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = c.L; fa.W = N;
         fa.none = 0;
@@ -1049,7 +1049,7 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             fa.q = b->frows.p;
             fa.H = b->F->H.as<float2>();
             fa.tw = b->F->tw.as<float2>();
-            fa.g = b->F->g.as<float>();
+            fa.g = b->F->g.as<double>();
             fa.wa = b->fwa.as<float>();
             fa.N = N; fa.A = A; fa.S = S; fa.npad = b->cf.npad; fa.L = b->cf.L; fa.tlo = b->gf.tlo; fa.W = b->gf.width;
             fa.none = (b->cf.filter == 0);

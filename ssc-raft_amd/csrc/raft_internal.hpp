@@ -117,7 +117,7 @@ struct FilterArgs {
     void* q;            // [G][A][W][SB] float (SB = 4) or __half (SB = 8)
     const float2* H;    // [npad] Hermitian filter incl. 1/(npad*pixel)
     const float2* tw;   // two-level twiddles: lo[128] = W^k, hi[npad/128 + 1] = W^(128k), W = exp(-2 pi i/npad)
-    const float* g;     // direct-convolution kernel (real, [npad]) or null
+    const double* g;    // direct-convolution kernel (real, [npad], FP64) or null
     const float* wa;    // [A] per-angle weight (dtheta * angular scale)
     int N, A, S, npad, L, tlo, W;
     int none;           // filter 'none': no filtering, no 1/(npad h)

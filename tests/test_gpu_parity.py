@@ -311,6 +311,27 @@ def test_lowpass(gpu, filt, n, a, pad):
     assert e <= FP32_TOL
 
 
+@pytest.mark.parametrize("n,pad", [(4096, 2), (2730, 5)])
+def test_filter_largest_fft_sizes(gpu, n, pad):
+    """Padded rows above 8192 points (12288 = 2^12 * 3, 16380 = 2^2 * 3^2 * 5 * 7 * 13
+    -> the direct-convolution path, and the 16-elements-per-thread LDS FFT)."""
+    import sscRaft
+
+    rng = np.random.default_rng(n)
+    a = 6
+    tomo = rng.standard_normal((2, a, n)).astype(np.float32)
+    dic = {"gpu": [0], "filter": "hann", "padding": pad, "regularization": 0.0, "beta/delta": 0.0,
+           "energy[eV]": 2.0e4, "z2[m]": 0.4, "detectorPixel[m]": 0.3, "blocksize": 0}
+    from sscRaft.processing.filters import lowpass_params
+
+    pf, pi = lowpass_params(n, a, 2, sscRaft.SetDictionary(dict(dic), ("gpu",), (), ()))
+    ref = oracle.lowpass(tomo, pf, pi)
+    out = sscRaft.lowpass(tomo.copy(), dic=dic)
+    e = rel(out, ref)
+    print(f"lowpass n={n} pad={pad} (npad {n * (1 + pad)}): rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+
+
 def test_lowpass_device_in_place_and_offset_ignored(gpu):
     import torch
     from sscRaft import device as dv
@@ -442,3 +463,41 @@ def test_bst(gpu, n, a, pad, filt, offset):
     e = rel(out, ref)
     print(f"bst n={n} a={a} pad={pad} {filt} offset={offset}: rel-L2 {e:.2e}")
     assert e <= 2e-5
+
+
+def test_blocked_calls_match_one_block(gpu):
+    """Volumes split into blocks of slices (parami[9] = blocksize, the
+    reference's memory-driven blocking) give what one block gives: FBP and
+    the eEM iterate bit for bit, the FP64 KL up to summation order."""
+    import sscRaft
+
+    n, a, S = 48, 40, 9
+    y, ang = noisy_sino(n, a, S, seed=21)
+    d0 = dict(fbp_dic(ang, pad=2, pixel=2.0 / n), blocksize=0)
+    d4 = dict(fbp_dic(ang, pad=2, pixel=2.0 / n), blocksize=4)
+    np.testing.assert_array_equal(sscRaft.fbp(y, dic=d4), sscRaft.fbp(y, dic=d0))
+    ye = np.maximum(y, 0).astype(np.float32)
+    x0, kl0, ex0 = sscRaft.eEMRT_GPU_KL(ye, ang, 4, [0], 0)
+    x4, kl4, ex4 = sscRaft.eEMRT_GPU_KL(ye, ang, 4, [0], 4)
+    np.testing.assert_array_equal(x4, x0)
+    np.testing.assert_allclose(kl4, kl0, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(ex4, ex0)
+    r0 = sscRaft.radon_RT(x0, ang, [0], pixel=2.0 / n)
+    np.testing.assert_array_equal(sscRaft.radon_RT(x0[:5], ang, [0], pixel=2.0 / n), r0[:5])
+
+
+def test_zero_slices_are_a_no_op(gpu):
+    """Device entry points accept an empty slice range and touch nothing."""
+    import torch
+    from sscRaft import device as dv
+    from sscRaft.geometries.parallel.fbp_methods import fbp_params
+
+    n = a = 32
+    ang = ph.uniform_angles(a)
+    empty = torch.zeros((0, a, n), dtype=torch.float32, device="cuda")
+    pf, pi = fbp_params(n, a, 0, dict(fbp_dic(ang, pixel=2.0 / n), blocksize=0))
+    out = dv.fbp_device(empty, ang, pf, pi, out=torch.zeros((0, n, n), device="cuda"))
+    assert out.shape == (0, n, n)
+    proj = dv.radon_device(torch.zeros((0, n, n), device="cuda"), ang, out=torch.zeros((0, a, n), device="cuda"))
+    assert proj.shape == (0, a, n)
+    torch.cuda.synchronize()
