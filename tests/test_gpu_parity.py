@@ -196,6 +196,45 @@ def test_eem_projector_forms_agree(gpu, monkeypatch, n, a, k):
     np.testing.assert_array_equal(ex2, exr)
 
 
+@pytest.mark.parametrize("nrays,n", [(48, 64), (64, 60), (33, 40)])
+@pytest.mark.parametrize("form", ["staged", "direct"])
+def test_eem_objsize_differs_from_nrays(gpu, monkeypatch, nrays, n, form):
+    """eEM of an n x n image from nrays-bin rows (objsize != nrays), both projector
+    forms, against the oracle; the image side comes from the initial guess."""
+    import torch
+    from sscRaft import device as dv
+
+    a, k = 40, 4
+    if form == "direct":
+        monkeypatch.setenv("RAFT_PROJ_LDS", "0")
+    img = ph.phantoms(nrays, 3, config=3, first=n)
+    ang = ph.uniform_angles(a)
+    y = np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n), 0).astype(np.float32)
+    x0 = np.random.default_rng(n).uniform(0.5, 1.5, (3, n, n)).astype(np.float32)
+    xt = torch.from_numpy(x0).cuda()
+    x, kl, ex = dv.eem_device(torch.from_numpy(y).cuda(), ang, k, obj=xt)
+    x = x.cpu().numpy()
+    xr, klr, exr = oracle.eem(y, ang, k, obj=x0.astype(np.float64), objsize=n)
+    e = rel(x, xr)
+    ek = float(np.max(np.abs(kl - klr) / np.abs(klr)))
+    print(f"eem objsize {n} nrays {nrays} {form}: x rel-L2 {e:.2e}, KL max rel {ek:.2e}")
+    assert e <= FP32_TOL and ek <= 1e-5
+    np.testing.assert_array_equal(ex, exr)
+
+
+def test_eem_objsize_outside_tile_window_is_refused(gpu):
+    """nrays much larger than objsize spreads a pixel tile over more bins than the
+    backprojection window holds: refused with an error, nothing written."""
+    import torch
+    from sscRaft import device as dv
+
+    y = torch.ones((1, 8, 64), dtype=torch.float32, device="cuda")
+    x = torch.ones((1, 40, 40), dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError, match="objsize"):
+        dv.eem_device(y, ph.uniform_angles(8), 2, obj=x)
+    assert bool(torch.all(x == 1))
+
+
 @pytest.mark.parametrize("n,a,k", [(16, 16, 1), (33, 17, 5), (64, 64, 20), (47, 60, 5)])
 def test_eem(gpu, n, a, k):
     import sscRaft

@@ -348,6 +348,74 @@ def test_em_backprojection_exact_rational():
     np.testing.assert_allclose(oracle.em_backproject(q, ang, n), ref, rtol=1e-12, atol=1e-12)
 
 
+def em_forward_exact(x, ang, nrays):
+    """kernel_radonWithDivision's ray sum (em_RT_kernel.cu:186-223) in exact rationals:
+    the pixel index is taken in units of the ray spacing dt = 2/(nrays-1) whatever
+    the image side n, and tested against [0, n)."""
+    n = x.shape[0]
+    dt = Fr(2, nrays - 1)
+    out = np.zeros((len(ang), nrays))
+    for a, th in enumerate(ang):
+        c, s = cs_fr(th)
+        for i in range(nrays):
+            t = -1 + i * dt
+            acc = 0.0
+            for k in range(nrays):
+                sk = -1 + k * dt
+                X = trunc_fr((t * c - sk * s + 1) / dt)
+                Y = trunc_fr((t * s + sk * c + 1) / dt)
+                if 0 <= X < n and 0 <= Y < n:
+                    acc += x[Y, X]
+            out[a, i] = acc * float(dt)
+    return out
+
+
+@pytest.mark.parametrize("n,nrays", [(6, 9), (9, 7), (8, 8)])
+def test_em_forward_objsize_exact_rational(n, nrays):
+    """objsize != nrays: the forward operator follows the reference's index rule."""
+    ang = np.sort(RNG.uniform(0.1, 3.0, 5)).astype(np.float32)
+    x = RNG.standard_normal((n, n))
+    np.testing.assert_allclose(oracle.em_forward(x, ang, nrays), em_forward_exact(x, ang, nrays),
+                               rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,nrays", [(6, 9), (9, 7)])
+def test_em_backprojection_objsize_exact_rational(n, nrays):
+    """objsize != nrays: pixel (i, j) at (-1 + i dxy, -1 + j dxy), dxy = 2/(n-1), reads
+    bin T = (x cos + y sin + 1)/dt, dt = 2/(nrays-1) (em_RT_kernel.cu:262-291)."""
+    ang = np.sort(RNG.uniform(0.1, 3.0, 5)).astype(np.float32)
+    q = RNG.standard_normal((5, nrays))
+    dxy, dt = Fr(2, n - 1), Fr(2, nrays - 1)
+    dth = dtheta(ang)
+    ref = np.zeros((n, n))
+    for a, th in enumerate(ang):
+        c, s = cs_fr(th)
+        for j in range(n):
+            for i in range(n):
+                T = trunc_fr(((-1 + i * dxy) * c + (-1 + j * dxy) * s + 1) / dt)
+                if 0 <= T < nrays:
+                    ref[j, i] += q[a, T] * float(dth[a])
+    np.testing.assert_allclose(oracle.em_backproject(q, ang, n), ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(oracle.em_backproject(q, ang, n, ones=True), backones_exact(ang, n, nrays),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_eem_objsize_one_update_composes_the_operators():
+    """One eEM update with objsize != nrays equals x * B(y / A x) / B(1) built from
+    the separately pinned forward and backprojection (APPROXINVX with e = 1e-4)."""
+    n, R, A = 11, 8, 6
+    ang = np.sort(RNG.uniform(0.1, 3.0, A)).astype(np.float32)
+    x0 = RNG.uniform(0.5, 1.5, (n, n))
+    y = (oracle.em_forward(x0 * 1.3, ang, R) + 0.01).astype(np.float32)
+    v = oracle.em_forward(x0, ang, R)
+    e = float(np.float32(1e-4))
+    r = np.maximum(y.astype(np.float64) * np.sign(v) / np.sqrt(e * e + v * v), 0)
+    expect = x0 * oracle.em_backproject(r, ang, n) / oracle.em_backproject(r, ang, n, ones=True)
+    x1, kl, _ = oracle.eem(y[None], ang, 1, obj=x0[None], objsize=n)
+    np.testing.assert_allclose(x1[0], expect, rtol=1e-12, atol=1e-12)
+    assert x1.shape == (1, n, n)
+
+
 def test_em_forward_is_radon_geometry():
     """kernel_radonWithDivision samples exactly like Radon_RT with a = 1 and nrays = N."""
     for n in (16, 17):
