@@ -48,6 +48,15 @@ int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, cons
                        int cx2, int split, const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0,
                        int N, int H, int W, int Cin, int Cout, int relu, int variant, void* stream);
 
+/* Last 3x3 convolution (Cout = 32) with the 1x1 output head in its epilogue:
+ * pred[p] = res[p] + hb[0] + sum_c hw[c] * bf16(act(conv(x)[p][c])), summed over
+ * c in order, then max(., floor_value) when clamp.  Equal to raft_nn_conv3x3
+ * followed by raft_nn_head without a target, without writing the 32-channel
+ * activations.  variant: 0, or a persistent one-tile form (6, 9, 10, 13). */
+int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b, int N,
+                         int H, int W, int Cin, int relu, int variant, const float* hw, const float* hb,
+                         const float* res, float* pred, float floor_value, int clamp, void* stream);
+
 /* First layer from the FP32 image [N][H][W] (one input channel):
  * y[p][cy0 + co] = act(b[co] + sum_tap bf16(img[p + off(tap)] * scale) * w[(co * 9 + tap) * ldw]),
  * zero padding; w is the layer's [Cout][9][ldw] bf16 weights of which input

@@ -129,6 +129,16 @@ struct ConvArgs {
     // concatenation [x | up2(x2)] that is never written out)
     const u16* x2;
     int ldx2, cx2, split, pool;
+    // output head folded into the epilogue (persistent form, 32 output channels;
+    // y is then not written): pred[p] = res[p] + hb + sum_c hw[c] bf16(act(.)[c]),
+    // clamped from below to floor_value when clamp -- what k_head computes from
+    // the stored bf16 activations, in the same order
+    const float* hw = nullptr;
+    const float* hb = nullptr;
+    const float* res = nullptr;
+    float* pred = nullptr;
+    float floor_value = 0.0f;
+    int clamp = 0;
 };
 
 // 16-byte unit (8 input channels c .. c+7) of the convolution input at pixel
@@ -279,7 +289,7 @@ struct ConvPArgs {
     int tiles_x, tiles_y, total_tiles, tiles_per_block;
 };
 
-template <int WR, int WC, bool UNROLL_DX>
+template <int WR, int WC, bool UNROLL_DX, bool HEAD = false>
 __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
 {
     using K = ConvP<WR, WC>;
@@ -366,6 +376,22 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
         store_item(it == 0 || nch > 1);
         __syncthreads();
         if (it + 1 < items) load_item(it + 1, nch > 1);
+        // head: the tile's residual values are fetched before its MFMAs
+        float resv[WR];
+        if constexpr (HEAD && WC == 1) {
+            if (ch == nch - 1) {
+                const int t = t0 + it / nch;
+                const int tx0 = (t % pa.tiles_x) * 32;
+                const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
+                const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+#pragma unroll
+                for (int r = 0; r < WR; ++r) {
+                    const int row = ty0 + wave * WR + r;
+                    resv[r] = (a.res && row < a.H && tx0 + l32 < a.W) ? a.res[img + (size_t)row * a.W + tx0 + l32]
+                                                                       : 0.0f;
+                }
+            }
+        }
 #pragma unroll(UNROLL_DX ? 3 : 1)
         for (int dx = 0; dx < 3; ++dx) {
 #pragma unroll
@@ -393,7 +419,37 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
             const int tx0 = (t % pa.tiles_x) * 32;
             const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
             const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
-            if (tx0 + l32 < a.W) {
+            if constexpr (HEAD && WC == 1) {
+                // lane (l32, hi) holds channels 8g + 4hi + i of pixel column l32;
+                // the hi = 0 lane gathers its partner's 16 and sums all 32 in order
+#pragma unroll
+                for (int r = 0; r < WR; ++r) {
+                    if (ty0 + wave * WR + r >= a.H) break; // wave-uniform
+                    float hv[16], pv[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        float v = acc[r][0][i] + bias[0][i];
+                        if (a.relu) v = fmaxf(v, 0.0f);
+                        hv[i] = bf16_float(bf16_bits(v));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) pv[i] = __shfl_xor(hv[i], 32);
+                    if (hi == 0 && tx0 + l32 < a.W) {
+                        float z = a.hb[0];
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) z = fmaf(a.hw[8 * g + i], hv[4 * g + i], z);
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) z = fmaf(a.hw[8 * g + 4 + i], pv[4 * g + i], z);
+                        }
+                        const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
+                        float pr = resv[r] + z;
+                        if (a.clamp) pr = fmaxf(pr, a.floor_value);
+                        a.pred[p] = pr;
+                    }
+                }
+            } else if (tx0 + l32 < a.W) {
 #pragma unroll
                 for (int r = 0; r < WR; ++r) {
                     if (ty0 + wave * WR + r < a.H) {
@@ -939,6 +995,7 @@ __global__ __launch_bounds__(256) void k_conv_first(const float* img, float scal
     }
 }
 
+
 hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 
 int launched() { return hipGetLastError() == hipSuccess ? RAFT_OK : RAFT_EHIP; }
@@ -960,13 +1017,13 @@ int launch_conv(const ConvArgs& a, hipStream_t st)
     return launched();
 }
 
-template <int WR, int WC, bool UNROLL_DX = true>
+template <int WR, int WC, bool UNROLL_DX = true, bool HEAD = false>
 int launch_convp(const ConvArgs& a, hipStream_t st)
 {
     using K = ConvP<WR, WC>;
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX>),
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, K::LDS) != hipSuccess)
             return RAFT_EHIP;
         attr = true;
@@ -982,7 +1039,7 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     long long per = (total * ny + 1023) / 1024;
     pa.tiles_per_block = (int)std::max(1LL, std::min(per, 16LL));
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
-    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
+    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
     return launched();
 }
 
@@ -1083,6 +1140,36 @@ int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, cons
     ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu, split < Cin ? x2 : nullptr, ldx2, cx2, split,
                pool ? 1 : 0};
     return conv_dispatch(a, variant, S(stream));
+}
+
+int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b, int N,
+                         int H, int W, int Cin, int relu, int variant, const float* hw, const float* hb,
+                         const float* res, float* pred, float floor_value, int clamp, void* stream)
+{
+    if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cin % 32 || ldx % 8 || cx0 % 8 || cx0 + Cin > ldx || !hw || !hb ||
+        !pred)
+        return RAFT_EINVAL;
+    ConvArgs a{x, w, b, nullptr, N, H, W, Cin, 32, ldx, cx0, 0, 0, relu, nullptr, 0, 0, Cin, 0};
+    a.hw = hw;
+    a.hb = hb;
+    a.res = res;
+    a.pred = pred;
+    a.floor_value = floor_value;
+    a.clamp = clamp ? 1 : 0;
+    // the persistent forms with one 32-channel output tile
+    switch (variant) {
+    case 6:
+        return launch_convp<4, 1, true, true>(a, S(stream));
+    case 9:
+        return launch_convp<2, 1, true, true>(a, S(stream));
+    case 10:
+        return launch_convp<4, 1, false, true>(a, S(stream));
+    case 0:
+    case 13:
+        return launch_convp<2, 1, false, true>(a, S(stream));
+    default:
+        return RAFT_EINVAL;
+    }
 }
 
 int raft_nn_conv_first(const float* img, float scale, const unsigned short* w, int ldw, const float* b,

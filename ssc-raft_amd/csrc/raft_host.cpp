@@ -255,6 +255,7 @@ struct Device {
     std::mutex mu;
     std::map<std::string, std::unique_ptr<FilterTables>> filters;
     DevBuf ws[kSlots];
+    DevBuf tab; // per-call tables (see TableSet)
     void* workspace(int slot, size_t bytes)
     {
         if (slot < 0 || slot >= kSlots) fail(RAFT_EINVAL, "internal: workspace slot out of range");
@@ -264,7 +265,56 @@ struct Device {
         }
         return ws[slot].p;
     }
+    char* tables(size_t bytes)
+    {
+        if (tab.bytes < bytes) {
+            tab.release();
+            tab.alloc(std::max(bytes, (size_t)64 << 10));
+        }
+        return static_cast<char*>(tab.p);
+    }
 };
+
+// A call's small tables (angle cosines and sines, bin coefficients, weights)
+// packed into one host buffer and copied with one transfer into the device's
+// grow-only table arena: no allocation (and no implicitly synchronising free)
+// per call.  Calls on a device run one at a time under its lock and
+// synchronise their stream before returning, so the next call may overwrite
+// the arena.
+class TableSet;
+struct TabRef {
+    const TableSet* set;
+    size_t off;
+    template <typename T> T* as() const;
+};
+class TableSet {
+  public:
+    explicit TableSet(Device& D) : D_(D) {}
+    template <typename T> TabRef add(const std::vector<T>& v)
+    {
+        const size_t off = (host_.size() + 255) & ~(size_t)255;
+        host_.resize(off + sizeof(T) * v.size());
+        if (!v.empty()) std::memcpy(host_.data() + off, v.data(), sizeof(T) * v.size());
+        return TabRef{this, off};
+    }
+    void upload(hipStream_t st)
+    {
+        base_ = D_.tables(host_.size());
+        if (!host_.empty())
+            RAFT_HIP(hipMemcpyAsync(base_, host_.data(), host_.size(), hipMemcpyHostToDevice, st));
+    }
+    char* base() const
+    {
+        if (!base_) fail(RAFT_EINVAL, "internal: table set used before upload");
+        return base_;
+    }
+
+  private:
+    Device& D_;
+    std::vector<char> host_;
+    char* base_ = nullptr;
+};
+template <typename T> T* TabRef::as() const { return reinterpret_cast<T*>(set->base() + off); }
 
 static std::mutex g_dev_mu;
 static std::map<int, std::unique_ptr<Device>> g_devices;
@@ -410,10 +460,9 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
     fbp_geometry(c, angles, g, wa);
 
     FilterTables& F = filter_tables(D, c, st);
-    DevBuf dcA, dcB, dwa;
-    upload(dcA, g.cA, st);
-    upload(dcB, g.cB, st);
-    upload(dwa, wa, st);
+    TableSet tb(D);
+    const TabRef dcA = tb.add(g.cA), dcB = tb.add(g.cB), dwa = tb.add(wa);
+    tb.upload(st);
 
     const int SB = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
     const size_t per_group = (size_t)A * g.width * 16; // SB values per bin, 16 bytes
@@ -486,10 +535,10 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
         const double ph = -2.0 * M_PI * (double)center * (double)k / (double)n;
         H[k] = make_float2((float)(amp * std::cos(ph)), (float)(amp * std::sin(ph)));
     }
-    DevBuf dH, dwa;
     const std::vector<float> ones(A, 1.0f);
-    upload(dH, H, st);
-    upload(dwa, ones, st);
+    TableSet tb(D);
+    const TabRef dH = tb.add(H), dwa = tb.add(ones);
+    tb.upload(st);
     const size_t s2 = (size_t)n * n;
     const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * n * 8 + 2 * s2 * 8;
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
@@ -541,8 +590,9 @@ static void lowpass_device(int dev, float* d_tomo, const float* paramf, const in
     const int A = c.A, N = c.N;
     FilterTables& F = filter_tables(D, c, st);
     std::vector<float> ones(A, 1.0f);
-    DevBuf dwa;
-    upload(dwa, ones, st);
+    TableSet tb(D);
+    const TabRef dwa = tb.add(ones);
+    tb.upload(st);
     const int SB = BP_SLICES_F32;
     const size_t per_group = (size_t)A * N * 16;
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
@@ -589,9 +639,9 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
         cv[a] = std::cos((double)angles[a]);
         sv[a] = std::sin((double)angles[a]);
     }
-    DevBuf dc, ds;
-    upload(dc, cv, st);
-    upload(ds, sv, st);
+    TableSet tb(D);
+    const TabRef dc = tb.add(cv), ds = tb.add(sv);
+    tb.upload(st);
     RayGeom g = ray_geom_radon(Nx, Ny, nrays, nangles, (double)ax, (double)ay);
     const size_t img = (size_t)Nx * Ny;
     int B = ((nslices + 3) / 4) * 4;
@@ -666,12 +716,9 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         g.tlo = lo;
         g.width = hi - lo;
     }
-    DevBuf dc, ds, dcA, dcB, dwa;
-    upload(dc, cv, st);
-    upload(ds, sv, st);
-    upload(dcA, g.cA, st);
-    upload(dcB, g.cB, st);
-    upload(dwa, wa, st);
+    TableSet tb(D);
+    const TabRef dc = tb.add(cv), ds = tb.add(sv), dcA = tb.add(g.cA), dcB = tb.add(g.cB), dwa = tb.add(wa);
+    tb.upload(st);
     RayGeom rg = ray_geom_em(R, A, n);
 
     const size_t img = (size_t)n * n;
@@ -845,12 +892,9 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
         g.tlo = lo;
         g.width = hi - lo;
     }
-    DevBuf dc, ds, dcA, dcB, dwa;
-    upload(dc, cv, st);
-    upload(ds, sv, st);
-    upload(dcA, g.cA, st);
-    upload(dcB, g.cB, st);
-    upload(dwa, wa, st);
+    TableSet tb(D);
+    const TabRef dc = tb.add(cv), ds = tb.add(sv), dcA = tb.add(g.cA), dcB = tb.add(g.cB), dwa = tb.add(wa);
+    tb.upload(st);
     RayGeom rg = ray_geom_em(R, A, n);
 
     const size_t img = (size_t)n * n;

@@ -34,6 +34,7 @@ _SIGNATURES = {
     "raft_nn_conv3x3_ex": ([_vp, _i, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp],
                            _i),
     "raft_nn_conv_first": ([_vp, _f, _vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
+    "raft_nn_conv3x3_head": ([_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _i, _vp], _i),
     "raft_nn_conv_weight_flip": ([_vp, _vp, _i, _i, _vp], _i),
     "raft_nn_wgrad_workspace": ([_i, _i, _i, _i, _i], _ll),
     "raft_nn_conv3x3_wgrad": ([_vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp, _ll, _i, _i, _i, _i, _i, _i, _vp], _i),
@@ -238,6 +239,7 @@ class UNet:
 
     # ---------------------------------------------------------------- tuning
     CONV_VARIANTS = (1, 2, 3, 5, 6, 7, 9, 10, 11, 13)
+    HEAD_VARIANTS = (6, 9, 10, 13)  # persistent forms with one 32-channel output tile
 
     def autotune(self, n, h, w, backward=False, repeats=5, stream=None):
         """Pick, per convolution shape of this network at batch n and image h x w,
@@ -318,11 +320,13 @@ class UNet:
         self._conv("dec1b", b["a7"], 0, b["d1"], 0, n, L[0], st)
         return b
 
-    def _forward_fused(self, img, n, h, w, st):
+    def _forward_fused(self, img, n, h, w, st, out, floor):
         """Inference forward with the input conversion, pooling, upsampling and
-        skip concatenations folded into the convolutions' input loads: the
-        same values as _forward (pooling / upsampling are bit-exact; the first
-        layer sums its 9 exact products in a different order than the MFMA)."""
+        skip concatenations folded into the convolutions' input loads, and the
+        head into the last convolution's epilogue: the same values as _forward
+        followed by the head (pooling / upsampling / head are bit-exact; the
+        first layer sums its 9 exact products in a different order than the
+        MFMA).  Writes the prediction to out."""
         b = self._fused_buffers(n, h, w)
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
         w0, b0 = self.offsets["enc1a"][0], self.offsets["enc1a"][1]
@@ -340,8 +344,14 @@ class UNet:
         self._conv_ex("dec2a", b["e2"], 0, b["d3"], b["a6"], n, L[1], st)
         self._conv_ex("dec2b", b["a6"], 0, None, b["d2"], n, L[1], st)
         self._conv_ex("dec1a", b["e1"], 0, b["d2"], b["a7"], n, L[0], st)
-        self._conv_ex("dec1b", b["a7"], 0, None, b["d1"], n, L[0], st)
-        return b
+        co, ci = self.offsets["dec1b"][2], self.offsets["dec1b"][3]
+        w1, b1 = self.offsets["dec1b"][0], self.offsets["dec1b"][1]
+        h0, hb = self.offsets["head"][0], self.offsets["head"][1]
+        v = self.tuned.get((n, h, w, ci, co), 0)
+        nn_call("raft_nn_conv3x3_head", b["a7"].ptr(), b["a7"].ch, 0, self.params_bf.data_ptr() + 2 * w1,
+                self.params.data_ptr() + 4 * b1, n, h, w, ci, 1, v if v in self.HEAD_VARIANTS else 0,
+                self.params.data_ptr() + 4 * h0, self.params.data_ptr() + 4 * hb, img.data_ptr(), out.data_ptr(),
+                float(floor if floor is not None else 0.0), int(floor is not None), st)
 
     def _fused_buffers(self, n, h, w):
         if self._fshape == (n, h, w):
@@ -352,7 +362,7 @@ class UNet:
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
         spec = {"a1": (0, c1), "e1": (0, c1), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
                 "a4": (3, c4), "e4": (3, c4), "a5": (2, c3), "d3": (2, c3), "a6": (1, c2), "d2": (1, c2),
-                "a7": (0, c1), "d1": (0, c1)}
+                "a7": (0, c1)}
         self._fbufs = None
         self._fbufs = {k: _Act(self.torch, n, L[lv][0], L[lv][1], ch, self.dev) for k, (lv, ch) in spec.items()}
         self._fshape = (n, h, w)
@@ -373,14 +383,21 @@ class UNet:
 
         floor: clamp the prediction from below (an eEM start must be positive).
         fused: fold input conversion, pooling, upsampling and concatenations
-        into the convolutions (False: the training path's separate kernels).
+        into the convolutions and the head into the last one (False: the
+        training path's separate kernels).
         """
         img = self._check_img(img)
         n, h, w = img.shape
         st = _stream(stream)
-        b = self._forward_fused(img, n, h, w, st) if fused else self._forward(img, n, h, w, st)
         if out is None:
             out = self.torch.empty_like(img)
+        elif not (isinstance(out, self.torch.Tensor) and out.is_cuda and out.dtype == self.torch.float32
+                  and out.is_contiguous() and out.numel() == img.numel()):
+            raise ValueError("out must be a contiguous float32 CUDA tensor with the input's size")
+        if fused:
+            self._forward_fused(img, n, h, w, st, out, floor)
+            return out
+        b = self._forward(img, n, h, w, st)
         h0, hb = self.offsets["head"][0], self.offsets["head"][1]
         nn_call("raft_nn_head", b["d1"].ptr(), b["d1"].ch, 0, self.params.data_ptr() + 4 * h0,
                 self.params.data_ptr() + 4 * hb, img.data_ptr(), out.data_ptr(), None, None, 0, 0, None, None, None,

@@ -148,24 +148,65 @@ def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
     assert torch.equal(z1, z2), "skip concatenation"
 
 
-def test_conv_first_layer(gpu):
+@pytest.mark.parametrize("co", [32, 8, 64])
+def test_conv_first_layer(gpu, co):
     """raft_nn_conv_first (FP32 image, one input channel) against the float64
-    convolution of the bf16-rounded image with input channel 0 of the weights."""
+    convolution of the bf16-rounded image with input channel 0 of the weights;
+    channels outside [cy0, cy0 + Cout) of the output rows stay untouched."""
     from sscRaft.nn import nn_call
 
-    rng = np.random.default_rng(8)
-    n, h, w, co = 2, 17, 40, 32
+    rng = np.random.default_rng(8 + co)
+    n, h, w = 2, 17, 40
+    ldy = co + 16
     img = rng.random((n, h, w)).astype(np.float32) * 3.0
     _, wt, b = conv_case(rng, n, h, w, 32, co)
     imd, wd, bd = torch.from_numpy(img).cuda(), dev_bits(wt), torch.from_numpy(b).cuda()
-    y = torch.zeros(n * h * w * 40, dtype=torch.int16, device="cuda")
-    nn_call("raft_nn_conv_first", imd.data_ptr(), 0.5, wd.data_ptr(), 32, bd.data_ptr(), y.data_ptr(), 40, 8,
+    y = torch.zeros(n * h * w * ldy, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv_first", imd.data_ptr(), 0.5, wd.data_ptr(), 32, bd.data_ptr(), y.data_ptr(), ldy, 8,
             n, h, w, co, 1, stream())
-    got = host_val(y).reshape(n, h, w, 40)
+    got = host_val(y).reshape(n, h, w, ldy)
     x1 = val(bits(img * np.float32(0.5)))[..., None]
     ref = torch_conv(x1, wt[:, :, :1], b, relu=True)
     assert_bf16_close(got[..., 8:8 + co], ref, "first layer")
     assert np.all(got[..., :8] == 0) and np.all(got[..., 8 + co:] == 0)
+
+
+@pytest.mark.parametrize("variant", [0, 6, 9, 10, 13])
+@pytest.mark.parametrize("n,h,w,ci,clamp,res", [(2, 11, 45, 32, 1, True), (1, 20, 64, 64, 0, False),
+                                                (3, 9, 8, 96, 1, True)])
+def test_conv3x3_head_equals_conv_then_head(gpu, variant, n, h, w, ci, clamp, res):
+    """raft_nn_conv3x3_head (head in the last convolution's epilogue) is bit for
+    bit raft_nn_conv3x3 with the same tile form followed by raft_nn_head."""
+    from sscRaft.nn import nn_call
+
+    rng = np.random.default_rng(variant + ci + h)
+    x, wt, b = conv_case(rng, n, h, w, ci, 32)
+    xd, wd, bd = dev_bits(x), dev_bits(wt), torch.from_numpy(b).cuda()
+    hw = torch.from_numpy((rng.standard_normal(32) * 0.2).astype(np.float32)).cuda()
+    hb = torch.tensor([0.3], dtype=torch.float32, device="cuda")
+    r = torch.from_numpy(rng.standard_normal((n, h, w)).astype(np.float32)).cuda() if res else None
+    P = n * h * w
+    y = torch.zeros(P * 32, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv3x3", xd.data_ptr(), ci, 0, wd.data_ptr(), bd.data_ptr(), y.data_ptr(), 32, 0, n, h, w, ci,
+            32, 1, variant or 13, stream())
+    p1 = torch.empty(P, dtype=torch.float32, device="cuda")
+    nn_call("raft_nn_head", y.data_ptr(), 32, 0, hw.data_ptr(), hb.data_ptr(), None if r is None else r.data_ptr(),
+            p1.data_ptr(), None, None, 0, 0, None, None, None, None, P, 0.05, clamp, 0, stream())
+    p2 = torch.full((P,), float("nan"), dtype=torch.float32, device="cuda")
+    nn_call("raft_nn_conv3x3_head", xd.data_ptr(), ci, 0, wd.data_ptr(), bd.data_ptr(), n, h, w, ci, 1, variant,
+            hw.data_ptr(), hb.data_ptr(), None if r is None else r.data_ptr(), p2.data_ptr(), 0.05, clamp, stream())
+    torch.cuda.synchronize()
+    assert torch.equal(p1, p2), f"max diff {(p1 - p2).abs().max().item():.3e}"
+
+
+def test_conv3x3_head_rejects_other_tile_forms(gpu):
+    from sscRaft.nn import nn_call
+
+    x = torch.zeros(8 * 8 * 32, dtype=torch.int16, device="cuda")
+    p = torch.zeros(64, dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        nn_call("raft_nn_conv3x3_head", x.data_ptr(), 32, 0, x.data_ptr(), None, 1, 8, 8, 32, 1, 7, p.data_ptr(),
+                p.data_ptr(), None, p.data_ptr(), 0.0, 0, stream())
 
 
 def test_conv3x3_backward_data(gpu):
