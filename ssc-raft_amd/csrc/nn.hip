@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace raft {
 
@@ -1036,8 +1037,22 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     if (total > (1LL << 30)) return RAFT_EINVAL;
     pa.total_tiles = (int)total;
     const int ny = a.Cout / K::CO_T;
-    long long per = (total * ny + 1023) / 1024;
-    pa.tiles_per_block = (int)std::max(1LL, std::min(per, 16LL));
+    // one resident round: as many blocks as the device holds at once (CUs x
+    // blocks per CU for this kernel's registers and LDS), tiles split evenly
+    static int slots[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return RAFT_EHIP;
+    if (slots[dev] == 0) {
+        int occ = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(
+                                                                   &k_conv3x3p<WR, WC, UNROLL_DX, HEAD>),
+                                                         256, K::LDS) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return RAFT_EHIP;
+        slots[dev] = std::max(1, occ) * std::max(1, cus);
+    }
+    const long long per = (total * ny + slots[dev] - 1) / slots[dev];
+    pa.tiles_per_block = (int)std::max(1LL, per);
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
     hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
     return launched();
