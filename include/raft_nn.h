@@ -33,7 +33,8 @@ int raft_nn_mfma_probe(const unsigned short* A16, const unsigned short* B16, flo
  * variant 0 = automatic; 1..5 select a one-tile-per-block shape (rows x output
  * channels per block: 8x64, 16x32, 4x32, 16x64, 8x32), 6..9 the persistent
  * form that prefetches the next tile while computing (16x32, 8x64, 16x64,
- * 8x32).  Any H, W >= 1 (partial tiles are masked). */
+ * 8x32), 10..13 the same four with the tap-column loop not unrolled.  Any
+ * H, W >= 1 (partial tiles are masked). */
 int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b,
                     unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
                     void* stream);
@@ -56,6 +57,17 @@ int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, cons
 int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b, int N,
                          int H, int W, int Cin, int relu, int variant, const float* hw, const float* hb,
                          const float* res, float* pred, float floor_value, int clamp, void* stream);
+
+/* First two 3x3 convolutions of the network in one pass (32 channels each):
+ * y = act(conv3x3(x1; w, b)) with x1 = act0(conv3x3(bf16(img * scale); w0, b0))
+ * computed per tile from the FP32 image inside the halo load, never written.
+ * w0 is the first layer's [32][9][ldw0] bf16 weights of which input channel 0
+ * is used; w the second layer's [32][9][32].  The first layer's 9 exact
+ * products are summed by an MFMA (raft_nn_conv_first sums them in tap order,
+ * so a bf16 rounding of x1 may differ).  variant: 0 or 6, 9, 10, 13. */
+int raft_nn_conv_first2(const float* img, float scale, const unsigned short* w0, int ldw0, const float* b0, int relu0,
+                        const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0, int N, int H,
+                        int W, int relu, int variant, void* stream);
 
 /* First layer from the FP32 image [N][H][W] (one input channel):
  * y[p][cy0 + co] = act(b[co] + sum_tap bf16(img[p + off(tap)] * scale) * w[(co * 9 + tap) * ldw]),

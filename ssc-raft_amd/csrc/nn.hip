@@ -140,6 +140,15 @@ struct ConvArgs {
     float* pred = nullptr;
     float floor_value = 0.0f;
     int clamp = 0;
+    // first layer folded into the halo load (persistent form, Cin = 32 = its
+    // Cout): the halo's 32 channels are act0(b0 + sum_tap bf16(img0 * scale0)
+    // w0[co][tap]) computed from the FP32 image, zero outside the image
+    const float* img0 = nullptr;
+    float scale0 = 1.0f;
+    const u16* w0 = nullptr;
+    int ldw0 = 0;
+    const float* b0 = nullptr;
+    int relu0 = 0;
 };
 
 // 16-byte unit (8 input channels c .. c+7) of the convolution input at pixel
@@ -283,6 +292,10 @@ struct ConvP {
     static constexpr int HU = (HALO_UNITS + 255) / 256;
     static constexpr int WU = (W_UNITS + 255) / 256;
     static constexpr int LDS = (HH * HW + 9 * CO_T) * CONV_PS * 2;
+    // first-layer mode: the FP32 image patch under the halo's 3x3 taps
+    static constexpr int PW = HW + 2, PH = HH + 2, PATCH = PH * PW, PU = (PATCH + 255) / 256;
+    static constexpr int HALO_GROUPS = (HH * HW + 31) / 32;
+    static constexpr int LDS_FIRST = LDS + PATCH * 4;
 };
 
 struct ConvPArgs {
@@ -290,7 +303,7 @@ struct ConvPArgs {
     int tiles_x, tiles_y, total_tiles, tiles_per_block;
 };
 
-template <int WR, int WC, bool UNROLL_DX, bool HEAD = false>
+template <int WR, int WC, bool UNROLL_DX, bool HEAD = false, bool FIRST = false>
 __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
 {
     using K = ConvP<WR, WC>;
@@ -298,6 +311,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
     extern __shared__ uint4 convp_lds[];
     u16* s_in = reinterpret_cast<u16*>(convp_lds);
     u16* s_w = s_in + K::HH * K::HW * CONV_PS;
+    float* s_patch = reinterpret_cast<float*>(s_w + 9 * K::CO_T * CONV_PS); // first-layer mode
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
     const int co0 = blockIdx.y * K::CO_T;
@@ -315,22 +329,53 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
             bias[c][i] = a.b ? a.b[co] : 0.0f;
         }
 
-    u32x4 hreg[K::HU], wreg[K::WU];
+    // first-layer mode: lane (l32, hi) supplies the weights of first-layer
+    // channel l32 for taps 8hi .. 8hi+7 (zero past tap 8) and receives
+    // channels 8g + 4hi + i of a halo pixel
+    bf16x8 a0;
+    float b0v[16];
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int tap = 8 * hi + j;
+            a0[j] = tap < 9 ? __builtin_bit_cast(__bf16, a.w0[((size_t)l32 * 9 + tap) * a.ldw0]) : (__bf16)0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) b0v[i] = a.b0 ? a.b0[8 * (i / 4) + 4 * hi + (i % 4)] : 0.0f;
+    }
+
+    u32x4 hreg[FIRST ? 1 : K::HU], wreg[K::WU];
+    float preg[FIRST ? K::PU : 1];
     auto load_item = [&](int it, bool weights) __attribute__((always_inline)) {
         const int t = t0 + it / nch, ch = it % nch;
         const int tx0 = (t % pa.tiles_x) * 32;
         const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
         const int n = t / (pa.tiles_x * pa.tiles_y);
+        if constexpr (FIRST) {
+            // image patch rows ty0-2 .. ty0+TH+1, columns tx0-2 .. tx0+33, bf16-rounded
 #pragma unroll
-        for (int k = 0; k < K::HU; ++k) {
-            const int u = tid + 256 * k;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (u < K::HALO_UNITS) {
-                const int pix = u >> 2, q = u & 3;
-                const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
-                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, 32 * ch + 8 * q);
+            for (int k = 0; k < K::PU; ++k) {
+                const int u = tid + 256 * k;
+                float v = 0.0f;
+                if (u < K::PATCH) {
+                    const int gy = ty0 - 2 + u / K::PW, gx = tx0 - 2 + u % K::PW;
+                    if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                        v = bf16_float(bf16_bits(a.img0[((size_t)n * a.H + gy) * a.W + gx] * a.scale0));
+                }
+                preg[k] = v;
             }
-            hreg[k] = as_u32x4(v);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K::HU; ++k) {
+                const int u = tid + 256 * k;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (u < K::HALO_UNITS) {
+                    const int pix = u >> 2, q = u & 3;
+                    const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
+                    if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, 32 * ch + 8 * q);
+                }
+                hreg[k] = as_u32x4(v);
+            }
         }
         if (weights) {
 #pragma unroll
@@ -345,10 +390,18 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
         }
     };
     auto store_item = [&](bool weights) __attribute__((always_inline)) {
+        if constexpr (FIRST) {
 #pragma unroll
-        for (int k = 0; k < K::HU; ++k) {
-            const int u = tid + 256 * k;
-            if (u < K::HALO_UNITS) *reinterpret_cast<u32x4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
+            for (int k = 0; k < K::PU; ++k) {
+                const int u = tid + 256 * k;
+                if (u < K::PATCH) s_patch[u] = preg[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K::HU; ++k) {
+                const int u = tid + 256 * k;
+                if (u < K::HALO_UNITS) *reinterpret_cast<u32x4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
+            }
         }
         if (weights) {
 #pragma unroll
@@ -377,6 +430,46 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
         store_item(it == 0 || nch > 1);
         __syncthreads();
         if (it + 1 < items) load_item(it + 1, nch > 1);
+        if constexpr (FIRST) {
+            // the halo's first-layer channels, one MFMA per 32 halo pixels
+            const int t = t0 + it / nch;
+            const int tx0 = (t % pa.tiles_x) * 32;
+            const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
+            for (int grp = wave; grp < K::HALO_GROUPS; grp += 4) {
+                const int hp = 32 * grp + l32;
+                const bool valid = hp < K::HH * K::HW;
+                const int hy = valid ? hp / K::HW : 0, hx = valid ? hp % K::HW : 0;
+                bf16x8 bv;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int tap = 8 * hi + j;
+                    bv[j] = tap < 9 ? (__bf16)s_patch[(hy + tap / 3) * K::PW + hx + tap % 3] : (__bf16)0.0f;
+                }
+                f32x16 d;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) d[i] = 0.0f;
+                d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, d, 0, 0, 0);
+                const int gy = ty0 - 1 + hy, gx = tx0 - 1 + hx;
+                const bool inside = valid && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W;
+                if (valid) {
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        float o[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            float v = d[4 * g + i] + b0v[4 * g + i];
+                            if (a.relu0) v = fmaxf(v, 0.0f);
+                            o[i] = inside ? v : 0.0f;
+                        }
+                        uint2 pk;
+                        pk.x = bf16_pack2(o[0], o[1]);
+                        pk.y = bf16_pack2(o[2], o[3]);
+                        *reinterpret_cast<uint2*>(s_in + hp * CONV_PS + 8 * g + 4 * hi) = pk;
+                    }
+                }
+            }
+            __syncthreads();
+        }
         // head: the tile's residual values are fetched before its MFMAs
         float resv[WR];
         if constexpr (HEAD && WC == 1) {
@@ -1018,14 +1111,15 @@ int launch_conv(const ConvArgs& a, hipStream_t st)
     return launched();
 }
 
-template <int WR, int WC, bool UNROLL_DX = true, bool HEAD = false>
+template <int WR, int WC, bool UNROLL_DX = true, bool HEAD = false, bool FIRST = false>
 int launch_convp(const ConvArgs& a, hipStream_t st)
 {
     using K = ConvP<WR, WC>;
+    constexpr int lds = FIRST ? K::LDS_FIRST : K::LDS;
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, K::LDS) != hipSuccess)
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
             return RAFT_EHIP;
         attr = true;
     }
@@ -1045,8 +1139,8 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     if (slots[dev] == 0) {
         int occ = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(
-                                                                   &k_conv3x3p<WR, WC, UNROLL_DX, HEAD>),
-                                                         256, K::LDS) != hipSuccess ||
+                                                                   &k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>),
+                                                         256, lds) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return RAFT_EHIP;
         slots[dev] = std::max(1, occ) * std::max(1, cus);
@@ -1054,7 +1148,7 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     const long long per = (total * ny + slots[dev] - 1) / slots[dev];
     pa.tiles_per_block = (int)std::max(1LL, per);
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
-    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD>), dim3((unsigned)nbx, ny), dim3(256), K::LDS, st, pa);
+    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>), dim3((unsigned)nbx, ny), dim3(256), lds, st, pa);
     return launched();
 }
 
@@ -1182,6 +1276,34 @@ int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsign
     case 0:
     case 13:
         return launch_convp<2, 1, false, true>(a, S(stream));
+    default:
+        return RAFT_EINVAL;
+    }
+}
+
+int raft_nn_conv_first2(const float* img, float scale, const unsigned short* w0, int ldw0, const float* b0, int relu0,
+                        const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0, int N, int H,
+                        int W, int relu, int variant, void* stream)
+{
+    if (N < 1 || H < 1 || W < 1 || ldw0 < 1 || ldy % 4 || cy0 % 4 || cy0 + 32 > ldy || !img || !w0 || !w)
+        return RAFT_EINVAL;
+    ConvArgs a{nullptr, w, b, y, N, H, W, 32, 32, 32, 0, ldy, cy0, relu, nullptr, 0, 0, 32, 0};
+    a.img0 = img;
+    a.scale0 = scale;
+    a.w0 = w0;
+    a.ldw0 = ldw0;
+    a.b0 = b0;
+    a.relu0 = relu0 ? 1 : 0;
+    switch (variant) {
+    case 6:
+        return launch_convp<4, 1, true, false, true>(a, S(stream));
+    case 9:
+        return launch_convp<2, 1, true, false, true>(a, S(stream));
+    case 10:
+        return launch_convp<4, 1, false, false, true>(a, S(stream));
+    case 0:
+    case 13:
+        return launch_convp<2, 1, false, false, true>(a, S(stream));
     default:
         return RAFT_EINVAL;
     }

@@ -34,6 +34,7 @@ _SIGNATURES = {
     "raft_nn_conv3x3_ex": ([_vp, _i, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp],
                            _i),
     "raft_nn_conv_first": ([_vp, _f, _vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
+    "raft_nn_conv_first2": ([_vp, _f, _vp, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
     "raft_nn_conv3x3_head": ([_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _i, _vp], _i),
     "raft_nn_conv_weight_flip": ([_vp, _vp, _i, _i, _vp], _i),
     "raft_nn_wgrad_workspace": ([_i, _i, _i, _i, _i], _ll),
@@ -239,7 +240,7 @@ class UNet:
 
     # ---------------------------------------------------------------- tuning
     CONV_VARIANTS = (1, 2, 3, 5, 6, 7, 9, 10, 11, 13)
-    HEAD_VARIANTS = (6, 9, 10, 13)  # persistent forms with one 32-channel output tile
+    HEAD_VARIANTS = (6, 9, 10, 13)  # persistent forms with one 32-channel output tile (head / first-layer modes)
 
     def autotune(self, n, h, w, backward=False, repeats=5, stream=None):
         """Pick, per convolution shape of this network at batch n and image h x w,
@@ -321,18 +322,21 @@ class UNet:
         return b
 
     def _forward_fused(self, img, n, h, w, st, out, floor):
-        """Inference forward with the input conversion, pooling, upsampling and
-        skip concatenations folded into the convolutions' input loads, and the
-        head into the last convolution's epilogue: the same values as _forward
-        followed by the head (pooling / upsampling / head are bit-exact; the
-        first layer sums its 9 exact products in a different order than the
-        MFMA).  Writes the prediction to out."""
+        """Inference forward with the first layer computed inside the second
+        one's halo load, pooling, upsampling and skip concatenations folded
+        into the convolutions' input loads, and the head into the last
+        convolution's epilogue: the same values as _forward followed by the
+        head, up to the order in which the first layer's 9 exact products are
+        summed (pooling / upsampling / head are bit-exact).  Writes the
+        prediction to out."""
         b = self._fused_buffers(n, h, w)
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
         w0, b0 = self.offsets["enc1a"][0], self.offsets["enc1a"][1]
-        nn_call("raft_nn_conv_first", img.data_ptr(), 1.0, self.params_bf.data_ptr() + 2 * w0, 32,
-                self.params.data_ptr() + 4 * b0, b["a1"].ptr(), b["a1"].ch, 0, n, h, w, self.base, 1, st)
-        self._conv_ex("enc1b", b["a1"], 0, None, b["e1"], n, L[0], st)
+        w1, b1 = self.offsets["enc1b"][0], self.offsets["enc1b"][1]
+        v = self.tuned.get((n, h, w, self.base, self.base), 0)
+        nn_call("raft_nn_conv_first2", img.data_ptr(), 1.0, self.params_bf.data_ptr() + 2 * w0, 32,
+                self.params.data_ptr() + 4 * b0, 1, self.params_bf.data_ptr() + 2 * w1, self.params.data_ptr() + 4 * b1,
+                b["e1"].ptr(), b["e1"].ch, 0, n, h, w, 1, v if v in self.HEAD_VARIANTS else 0, st)
         self._conv_ex("enc2a", b["e1"], 1, None, b["a2"], n, L[1], st)
         self._conv_ex("enc2b", b["a2"], 0, None, b["e2"], n, L[1], st)
         self._conv_ex("enc3a", b["e2"], 1, None, b["a3"], n, L[2], st)
@@ -360,7 +364,7 @@ class UNet:
             raise ValueError("image height and width must be multiples of 8 (three 2x2 poolings)")
         c1, c2, c3, c4 = self.base, 2 * self.base, 4 * self.base, 8 * self.base
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
-        spec = {"a1": (0, c1), "e1": (0, c1), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
+        spec = {"e1": (0, c1), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
                 "a4": (3, c4), "e4": (3, c4), "a5": (2, c3), "d3": (2, c3), "a6": (1, c2), "d2": (1, c2),
                 "a7": (0, c1)}
         self._fbufs = None

@@ -172,6 +172,45 @@ def test_conv_first_layer(gpu, co):
 
 
 @pytest.mark.parametrize("variant", [0, 6, 9, 10, 13])
+@pytest.mark.parametrize("n,h,w", [(2, 17, 40), (1, 32, 64), (3, 9, 8)])
+def test_conv_first2_equals_first_then_conv(gpu, variant, n, h, w):
+    """raft_nn_conv_first2 (first layer inside the second one's halo load)
+    against raft_nn_conv_first + raft_nn_conv3x3, and both against the float64
+    composition with the intermediate rounded to bf16."""
+    from sscRaft.nn import nn_call
+
+    rng = np.random.default_rng(variant + h + w)
+    img = (rng.random((n, h, w)) * 3.0).astype(np.float32)
+    _, w0, b0 = conv_case(rng, n, h, w, 32, 32)
+    _, w1, b1 = conv_case(rng, n, h, w, 32, 32)
+    imd = torch.from_numpy(img).cuda()
+    w0d, b0d, w1d, b1d = dev_bits(w0), torch.from_numpy(b0).cuda(), dev_bits(w1), torch.from_numpy(b1).cuda()
+    P = n * h * w
+    x1 = torch.zeros(P * 32, dtype=torch.int16, device="cuda")
+    y1 = torch.zeros(P * 32, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv_first", imd.data_ptr(), 0.5, w0d.data_ptr(), 32, b0d.data_ptr(), x1.data_ptr(), 32, 0, n, h, w,
+            32, 1, stream())
+    nn_call("raft_nn_conv3x3", x1.data_ptr(), 32, 0, w1d.data_ptr(), b1d.data_ptr(), y1.data_ptr(), 32, 0, n, h, w, 32,
+            32, 1, variant or 13, stream())
+    y2 = torch.full((P * 40,), 0x3F80, dtype=torch.int16, device="cuda")  # bf16 1.0 outside [cy0, cy0 + 32)
+    nn_call("raft_nn_conv_first2", imd.data_ptr(), 0.5, w0d.data_ptr(), 32, b0d.data_ptr(), 1, w1d.data_ptr(),
+            b1d.data_ptr(), y2.data_ptr(), 40, 4, n, h, w, 1, variant, stream())
+    sep = host_val(y1).reshape(n, h, w, 32)
+    got = host_val(y2).reshape(n, h, w, 40)
+    assert np.all(got[..., :4] == 1.0) and np.all(got[..., 36:] == 1.0)
+    fused = got[..., 4:36]
+    xin = val(bits(img * np.float32(0.5)))[..., None]
+    mid = val(bits(torch_conv(xin, w0[:, :, :1], b0, relu=True).astype(np.float32)))
+    ref = torch_conv(mid, w1, b1, relu=True)
+    e_sep = np.linalg.norm(sep - ref) / np.linalg.norm(ref)
+    e_fused = np.linalg.norm(fused - ref) / np.linalg.norm(ref)
+    e_pair = np.linalg.norm(fused - sep) / np.linalg.norm(sep)
+    print(f"first2 variant {variant} {n}x{h}x{w}: rel-L2 separate {e_sep:.2e}, fused {e_fused:.2e}, between {e_pair:.2e}")
+    assert e_sep <= 4e-3 and e_fused <= 4e-3 and e_pair <= 4e-3
+    assert np.abs(fused - sep).max() <= 2e-2 * np.abs(sep).max()
+
+
+@pytest.mark.parametrize("variant", [0, 6, 9, 10, 13])
 @pytest.mark.parametrize("n,h,w,ci,clamp,res", [(2, 11, 45, 32, 1, True), (1, 20, 64, 64, 0, False),
                                                 (3, 9, 8, 96, 1, True)])
 def test_conv3x3_head_equals_conv_then_head(gpu, variant, n, h, w, ci, clamp, res):
