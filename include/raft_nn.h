@@ -44,10 +44,14 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
  * at H x W, or, when pool != 0, as the 2x2 average of x at 2H x 2W (the values
  * raft_nn_pool2 would write); channels [split, Cin) come from x2 (ldx2, cx2)
  * at H/2 x W/2 with nearest 2x upsampling (the values raft_nn_up2 would write).
- * split is a multiple of 32 in [32, Cin]; split < Cin needs x2 and even H, W. */
+ * split is a multiple of 32 in [32, Cin]; split < Cin needs x2 and even H, W.
+ * yp (may be null): also write the 2x2 average of y, the values raft_nn_pool2
+ * would compute from it, to yp[(n * H/2 + y) * W/2 + x][cp0 + co] (channel
+ * stride ldp; H, W even; ldp, cp0 multiples of 4; persistent variants 6..13). */
 int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, const unsigned short* x2, int ldx2,
                        int cx2, int split, const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0,
-                       int N, int H, int W, int Cin, int Cout, int relu, int variant, void* stream);
+                       int N, int H, int W, int Cin, int Cout, int relu, int variant, unsigned short* yp, int ldp,
+                       int cp0, void* stream);
 
 /* Last 3x3 convolution (Cout = 32) with the 1x1 output head in its epilogue:
  * pred[p] = res[p] + hb[0] + sum_c hw[c] * bf16(act(conv(x)[p][c])), summed over
@@ -64,10 +68,11 @@ int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsign
  * w0 is the first layer's [32][9][ldw0] bf16 weights of which input channel 0
  * is used; w the second layer's [32][9][32].  The first layer's 9 exact
  * products are summed by an MFMA (raft_nn_conv_first sums them in tap order,
- * so a bf16 rounding of x1 may differ).  variant: 0 or 6, 9, 10, 13. */
+ * so a bf16 rounding of x1 may differ).  variant: 0 or 6, 9, 10, 13.  yp, ldp,
+ * cp0: the 2x2-pooled output as for raft_nn_conv3x3_ex (yp may be null). */
 int raft_nn_conv_first2(const float* img, float scale, const unsigned short* w0, int ldw0, const float* b0, int relu0,
                         const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0, int N, int H,
-                        int W, int relu, int variant, void* stream);
+                        int W, int relu, int variant, unsigned short* yp, int ldp, int cp0, void* stream);
 
 /* First layer from the FP32 image [N][H][W] (one input channel):
  * y[p][cy0 + co] = act(b[co] + sum_tap bf16(img[p + off(tap)] * scale) * w[(co * 9 + tap) * ldw]),

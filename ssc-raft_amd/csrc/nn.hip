@@ -44,6 +44,12 @@ __device__ __forceinline__ unsigned bf16_pack2(float lo, float hi)
     return __builtin_bit_cast(unsigned, v);
 }
 __device__ __forceinline__ u16 bf16_bits(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+
+// value of lane l | 1 in lane l (for even l): a DPP quad permutation [1, 1, 3, 3]
+__device__ __forceinline__ float odd_neighbour(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xF5, 0xF, 0xF, false));
+}
 __device__ __forceinline__ float bf16_float(u16 b) { return __uint_as_float((unsigned)b << 16); }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
@@ -149,6 +155,10 @@ struct ConvArgs {
     int ldw0 = 0;
     const float* b0 = nullptr;
     int relu0 = 0;
+    // also write the 2x2 average of the output (persistent form; H, W even):
+    // ypool[(n * H/2 + y) * W/2 + x][cp0 + co] at channel stride ldp
+    u16* ypool = nullptr;
+    int ldp = 0, cp0 = 0;
 };
 
 // 16-byte unit (8 input channels c .. c+7) of the convolution input at pixel
@@ -303,7 +313,7 @@ struct ConvPArgs {
     int tiles_x, tiles_y, total_tiles, tiles_per_block;
 };
 
-template <int WR, int WC, bool UNROLL_DX, bool HEAD = false, bool FIRST = false>
+template <int WR, int WC, bool UNROLL_DX, bool HEAD = false, bool FIRST = false, bool POOL = false>
 __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
 {
     using K = ConvP<WR, WC>;
@@ -567,6 +577,50 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3p(ConvPArgs pa)
                                 *reinterpret_cast<uint2*>(yp + co) = pk;
                             }
                     }
+                }
+            }
+            if constexpr (POOL && !HEAD) {
+                // 2x2 average of the stored (bf16) outputs of row pairs (2k, 2k+1)
+                // and column pairs (l32, l32 + 1), summed in k_pool2's order
+                // (+0, x00, x01, x10, x11), written by the even-column lanes
+#pragma unroll
+                for (int r = 1; r < WR; r += 2) {
+                    const int row = ty0 + wave * WR + r - 1;
+                    if (row >= a.H) break; // wave-uniform; H even, so row + 1 < H
+                    const bool store = (l32 & 1) == 0 && tx0 + l32 < a.W;
+                    const size_t pp = ((size_t)(t / (pa.tiles_x * pa.tiles_y)) * (a.H >> 1) + (row >> 1)) * (a.W >> 1) +
+                                      ((tx0 + l32) >> 1);
+                    u16* qp = a.ypool + pp * a.ldp + a.cp0;
+#pragma unroll
+                    for (int c = 0; c < WC; ++c)
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            float q[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                float v0 = acc[r - 1][c][4 * g + i] + bias[c][4 * g + i];
+                                float v1 = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                                if (a.relu) {
+                                    v0 = fmaxf(v0, 0.0f);
+                                    v1 = fmaxf(v1, 0.0f);
+                                }
+                                v0 = bf16_float(bf16_bits(v0));
+                                v1 = bf16_float(bf16_bits(v1));
+                                const float n0 = odd_neighbour(v0), n1 = odd_neighbour(v1);
+                                float sum = 0.0f;
+                                sum += v0;
+                                sum += n0;
+                                sum += v1;
+                                sum += n1;
+                                q[i] = sum * 0.25f;
+                            }
+                            if (store) {
+                                uint2 pk;
+                                pk.x = bf16_pack2(q[0], q[1]);
+                                pk.y = bf16_pack2(q[2], q[3]);
+                                *reinterpret_cast<uint2*>(qp + co0 + 32 * c + 8 * g + 4 * hi) = pk;
+                            }
+                        }
                 }
             }
 #pragma unroll
@@ -1111,14 +1165,14 @@ int launch_conv(const ConvArgs& a, hipStream_t st)
     return launched();
 }
 
-template <int WR, int WC, bool UNROLL_DX = true, bool HEAD = false, bool FIRST = false>
+template <int WR, int WC, bool UNROLL_DX = true, bool HEAD = false, bool FIRST = false, bool POOL = false>
 int launch_convp(const ConvArgs& a, hipStream_t st)
 {
     using K = ConvP<WR, WC>;
     constexpr int lds = FIRST ? K::LDS_FIRST : K::LDS;
     static bool attr = false;
     if (!attr) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>),
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
             return RAFT_EHIP;
         attr = true;
@@ -1139,7 +1193,7 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     if (slots[dev] == 0) {
         int occ = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(
-                                                                   &k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>),
+                                                                   &k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>),
                                                          256, lds) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return RAFT_EHIP;
@@ -1148,7 +1202,8 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     const long long per = (total * ny + slots[dev] - 1) / slots[dev];
     pa.tiles_per_block = (int)std::max(1LL, per);
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
-    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST>), dim3((unsigned)nbx, ny), dim3(256), lds, st, pa);
+    hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>), dim3((unsigned)nbx, ny), dim3(256), lds, st,
+                       pa);
     return launched();
 }
 
@@ -1226,6 +1281,38 @@ static int conv_dispatch(const ConvArgs& a, int variant, hipStream_t st)
     }
 }
 
+// persistent forms that also write the 2x2-pooled output (ConvArgs::ypool)
+static int conv_dispatch_pool(const ConvArgs& a, int variant, hipStream_t st)
+{
+    if (variant == 0) variant = (a.Cout % 64 == 0) ? 7 : 13;
+    if ((variant == 7 || variant == 8 || variant == 11 || variant == 12) && a.Cout % 64) return RAFT_EINVAL;
+    switch (variant) {
+    case 6:
+        return launch_convp<4, 1, true, false, false, true>(a, st);
+    case 7:
+        return launch_convp<2, 2, true, false, false, true>(a, st);
+    case 8:
+        return launch_convp<4, 2, true, false, false, true>(a, st);
+    case 9:
+        return launch_convp<2, 1, true, false, false, true>(a, st);
+    case 10:
+        return launch_convp<4, 1, false, false, false, true>(a, st);
+    case 11:
+        return launch_convp<2, 2, false, false, false, true>(a, st);
+    case 12:
+        return launch_convp<4, 2, false, false, false, true>(a, st);
+    case 13:
+        return launch_convp<2, 1, false, false, false, true>(a, st);
+    default:
+        return RAFT_EINVAL;
+    }
+}
+
+static bool pool_out_ok(unsigned short* yp, int ldp, int cp0, int H, int W, int Cout)
+{
+    return !yp || (H % 2 == 0 && W % 2 == 0 && ldp % 4 == 0 && cp0 % 4 == 0 && cp0 + Cout <= ldp);
+}
+
 int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b,
                     unsigned short* y, int ldy, int cy0, int N, int H, int W, int Cin, int Cout, int relu, int variant,
                     void* stream)
@@ -1239,8 +1326,10 @@ int raft_nn_conv3x3(const unsigned short* x, int ldx, int cx0, const unsigned sh
 
 int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, const unsigned short* x2, int ldx2,
                        int cx2, int split, const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0,
-                       int N, int H, int W, int Cin, int Cout, int relu, int variant, void* stream)
+                       int N, int H, int W, int Cin, int Cout, int relu, int variant, unsigned short* yp, int ldp,
+                       int cp0, void* stream)
 {
+    if (!pool_out_ok(yp, ldp, cp0, H, W, Cout)) return RAFT_EINVAL;
     if (N < 1 || H < 1 || W < 1 || Cin < 32 || Cout < 32 || Cin % 32 || Cout % 32 || ldx % 8 || cx0 % 8 ||
         ldy % 4 || cy0 % 4 || cy0 + Cout > ldy || split < 32 || split > Cin || split % 32 || cx0 + split > ldx)
         return RAFT_EINVAL;
@@ -1248,7 +1337,11 @@ int raft_nn_conv3x3_ex(const unsigned short* x, int ldx, int cx0, int pool, cons
         return RAFT_EINVAL;
     ConvArgs a{x, w, b, y, N, H, W, Cin, Cout, ldx, cx0, ldy, cy0, relu, split < Cin ? x2 : nullptr, ldx2, cx2, split,
                pool ? 1 : 0};
-    return conv_dispatch(a, variant, S(stream));
+    if (!yp) return conv_dispatch(a, variant, S(stream));
+    a.ypool = yp;
+    a.ldp = ldp;
+    a.cp0 = cp0;
+    return conv_dispatch_pool(a, variant, S(stream));
 }
 
 int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsigned short* w, const float* b, int N,
@@ -1283,9 +1376,10 @@ int raft_nn_conv3x3_head(const unsigned short* x, int ldx, int cx0, const unsign
 
 int raft_nn_conv_first2(const float* img, float scale, const unsigned short* w0, int ldw0, const float* b0, int relu0,
                         const unsigned short* w, const float* b, unsigned short* y, int ldy, int cy0, int N, int H,
-                        int W, int relu, int variant, void* stream)
+                        int W, int relu, int variant, unsigned short* yp, int ldp, int cp0, void* stream)
 {
-    if (N < 1 || H < 1 || W < 1 || ldw0 < 1 || ldy % 4 || cy0 % 4 || cy0 + 32 > ldy || !img || !w0 || !w)
+    if (N < 1 || H < 1 || W < 1 || ldw0 < 1 || ldy % 4 || cy0 % 4 || cy0 + 32 > ldy || !img || !w0 || !w ||
+        !pool_out_ok(yp, ldp, cp0, H, W, 32))
         return RAFT_EINVAL;
     ConvArgs a{nullptr, w, b, y, N, H, W, 32, 32, 32, 0, ldy, cy0, relu, nullptr, 0, 0, 32, 0};
     a.img0 = img;
@@ -1294,6 +1388,24 @@ int raft_nn_conv_first2(const float* img, float scale, const unsigned short* w0,
     a.ldw0 = ldw0;
     a.b0 = b0;
     a.relu0 = relu0 ? 1 : 0;
+    if (yp) {
+        a.ypool = yp;
+        a.ldp = ldp;
+        a.cp0 = cp0;
+        switch (variant) {
+        case 6:
+            return launch_convp<4, 1, true, false, true, true>(a, S(stream));
+        case 9:
+            return launch_convp<2, 1, true, false, true, true>(a, S(stream));
+        case 10:
+            return launch_convp<4, 1, false, false, true, true>(a, S(stream));
+        case 0:
+        case 13:
+            return launch_convp<2, 1, false, false, true, true>(a, S(stream));
+        default:
+            return RAFT_EINVAL;
+        }
+    }
     switch (variant) {
     case 6:
         return launch_convp<4, 1, true, false, true>(a, S(stream));

@@ -31,10 +31,11 @@ _ll = ctypes.c_longlong
 _SIGNATURES = {
     "raft_nn_mfma_probe": ([_vp] * 7, _i),
     "raft_nn_conv3x3": ([_vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
-    "raft_nn_conv3x3_ex": ([_vp, _i, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp],
-                           _i),
+    "raft_nn_conv3x3_ex": ([_vp, _i, _i, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp,
+                            _i, _i, _vp], _i),
     "raft_nn_conv_first": ([_vp, _f, _vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
-    "raft_nn_conv_first2": ([_vp, _f, _vp, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp], _i),
+    "raft_nn_conv_first2": ([_vp, _f, _vp, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _vp],
+                            _i),
     "raft_nn_conv3x3_head": ([_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _i, _vp], _i),
     "raft_nn_conv_weight_flip": ([_vp, _vp, _i, _i, _vp], _i),
     "raft_nn_wgrad_workspace": ([_i, _i, _i, _i, _i], _ll),
@@ -221,16 +222,22 @@ class UNet:
         nn_call("raft_nn_conv3x3", x.ptr(), x.ch, cx0, self.params_bf.data_ptr() + 2 * w0,
                 self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, cy0, n, hw[0], hw[1], ci, co, int(relu), v, st)
 
-    def _conv_ex(self, name, x, pool, x2, y, n, hw, st, relu=True):
+    POOL_VARIANTS = (6, 7, 8, 9, 10, 11, 12, 13)  # persistent forms (pooled output in the epilogue)
+
+    def _conv_ex(self, name, x, pool, x2, y, n, hw, st, relu=True, yp=None):
         """Fused-input convolution (inference): x pooled 2x2 when `pool`, and
-        input channels past x's own read from x2 upsampled 2x."""
+        input channels past x's own read from x2 upsampled 2x; yp (optional)
+        receives the 2x2 average of the output."""
         co, ci = self.offsets[name][2], self.offsets[name][3]
         w0, b0 = self.offsets[name][0], self.offsets[name][1]
         v = self.tuned.get((n, hw[0], hw[1], ci, co), 0)
+        if yp is not None and v not in self.POOL_VARIANTS:
+            v = 0
         split = ci if x2 is None else x.ch
         nn_call("raft_nn_conv3x3_ex", x.ptr(), x.ch, 0, int(pool), None if x2 is None else x2.ptr(),
                 0 if x2 is None else x2.ch, 0, split, self.params_bf.data_ptr() + 2 * w0,
-                self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, 0, n, hw[0], hw[1], ci, co, int(relu), v, st)
+                self.params.data_ptr() + 4 * b0, y.ptr(), y.ch, 0, n, hw[0], hw[1], ci, co, int(relu), v,
+                None if yp is None else yp.ptr(), 0 if yp is None else yp.ch, 0, st)
 
     def _conv_bwd_data(self, name, g, gx, cgx0, n, hw, st):
         co, ci = self.offsets[name][2], self.offsets[name][3]
@@ -323,8 +330,9 @@ class UNet:
 
     def _forward_fused(self, img, n, h, w, st, out, floor):
         """Inference forward with the first layer computed inside the second
-        one's halo load, pooling, upsampling and skip concatenations folded
-        into the convolutions' input loads, and the head into the last
+        one's halo load, each level's 2x2 average written by its last
+        convolution's epilogue, upsampling and skip concatenations folded into
+        the convolutions' input loads, and the head into the last
         convolution's epilogue: the same values as _forward followed by the
         head, up to the order in which the first layer's 9 exact products are
         summed (pooling / upsampling / head are bit-exact).  Writes the
@@ -336,12 +344,13 @@ class UNet:
         v = self.tuned.get((n, h, w, self.base, self.base), 0)
         nn_call("raft_nn_conv_first2", img.data_ptr(), 1.0, self.params_bf.data_ptr() + 2 * w0, 32,
                 self.params.data_ptr() + 4 * b0, 1, self.params_bf.data_ptr() + 2 * w1, self.params.data_ptr() + 4 * b1,
-                b["e1"].ptr(), b["e1"].ch, 0, n, h, w, 1, v if v in self.HEAD_VARIANTS else 0, st)
-        self._conv_ex("enc2a", b["e1"], 1, None, b["a2"], n, L[1], st)
-        self._conv_ex("enc2b", b["a2"], 0, None, b["e2"], n, L[1], st)
-        self._conv_ex("enc3a", b["e2"], 1, None, b["a3"], n, L[2], st)
-        self._conv_ex("enc3b", b["a3"], 0, None, b["e3"], n, L[2], st)
-        self._conv_ex("bot_a", b["e3"], 1, None, b["a4"], n, L[3], st)
+                b["e1"].ptr(), b["e1"].ch, 0, n, h, w, 1, v if v in self.HEAD_VARIANTS else 0, b["p1"].ptr(),
+                b["p1"].ch, 0, st)
+        self._conv_ex("enc2a", b["p1"], 0, None, b["a2"], n, L[1], st)
+        self._conv_ex("enc2b", b["a2"], 0, None, b["e2"], n, L[1], st, yp=b["p2"])
+        self._conv_ex("enc3a", b["p2"], 0, None, b["a3"], n, L[2], st)
+        self._conv_ex("enc3b", b["a3"], 0, None, b["e3"], n, L[2], st, yp=b["p3"])
+        self._conv_ex("bot_a", b["p3"], 0, None, b["a4"], n, L[3], st)
         self._conv_ex("bot_b", b["a4"], 0, None, b["e4"], n, L[3], st)
         self._conv_ex("dec3a", b["e3"], 0, b["e4"], b["a5"], n, L[2], st)
         self._conv_ex("dec3b", b["a5"], 0, None, b["d3"], n, L[2], st)
@@ -364,7 +373,7 @@ class UNet:
             raise ValueError("image height and width must be multiples of 8 (three 2x2 poolings)")
         c1, c2, c3, c4 = self.base, 2 * self.base, 4 * self.base, 8 * self.base
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
-        spec = {"e1": (0, c1), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
+        spec = {"e1": (0, c1), "p1": (1, c1), "p2": (2, c2), "p3": (3, c3), "a2": (1, c2), "e2": (1, c2), "a3": (2, c3), "e3": (2, c3),
                 "a4": (3, c4), "e4": (3, c4), "a5": (2, c3), "d3": (2, c3), "a6": (1, c2), "d2": (1, c2),
                 "a7": (0, c1)}
         self._fbufs = None

@@ -128,7 +128,7 @@ def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
     nn_call("raft_nn_conv3x3", p.data_ptr(), 32, 0, wd.data_ptr(), bd.data_ptr(), y1.data_ptr(), 64, 0, n, h, w,
             32, 64, 1, variant, stream())
     nn_call("raft_nn_conv3x3_ex", xd.data_ptr(), 48, 16, 1, None, 0, 0, 32, wd.data_ptr(), bd.data_ptr(),
-            y2.data_ptr(), 64, 0, n, h, w, 32, 64, 1, variant, stream())
+            y2.data_ptr(), 64, 0, n, h, w, 32, 64, 1, variant, None, 0, 0, stream())
     torch.cuda.synchronize()
     assert torch.equal(y1, y2), "pooled input"
     # skip concatenation: 32 skip channels + 64 upsampled channels -> 96
@@ -143,9 +143,38 @@ def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
     nn_call("raft_nn_conv3x3", cd.data_ptr(), 96, 0, wd2.data_ptr(), bd2.data_ptr(), z1.data_ptr(), 32, 0, n, h, w,
             96, 32, 1, variant, stream())
     nn_call("raft_nn_conv3x3_ex", sd.data_ptr(), 32, 0, 0, ld.data_ptr(), 64, 0, 32, wd2.data_ptr(), bd2.data_ptr(),
-            z2.data_ptr(), 32, 0, n, h, w, 96, 32, 1, variant, stream())
+            z2.data_ptr(), 32, 0, n, h, w, 96, 32, 1, variant, None, 0, 0, stream())
     torch.cuda.synchronize()
     assert torch.equal(z1, z2), "skip concatenation"
+
+
+@pytest.mark.parametrize("variant", [0, 6, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("n,h,w,co", [(2, 10, 36, 64), (1, 16, 64, 32), (3, 6, 8, 128)])
+def test_conv3x3_pooled_output_equals_pool2(gpu, variant, n, h, w, co):
+    """The 2x2-pooled output written by the convolution's epilogue is bit for
+    bit raft_nn_pool2 of its regular output, which is itself unchanged."""
+    from sscRaft.nn import nn_call
+
+    if co % 64 and variant in (7, 8, 11, 12):
+        pytest.skip("64-channel tile form")
+    rng = np.random.default_rng(70 + variant + co)
+    x, wt, b = conv_case(rng, n, h, w, 64, co)
+    xd, wd, bd = dev_bits(x), dev_bits(wt), torch.from_numpy(b).cuda()
+    P, ldp = n * h * w, co + 16
+    y0 = torch.zeros(P * co, dtype=torch.int16, device="cuda")
+    y1 = torch.zeros(P * co, dtype=torch.int16, device="cuda")
+    q0 = torch.zeros(P // 4 * ldp, dtype=torch.int16, device="cuda")
+    q1 = torch.full((P // 4 * ldp,), 0x3F80, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv3x3_ex", xd.data_ptr(), 64, 0, 0, None, 0, 0, 64, wd.data_ptr(), bd.data_ptr(),
+            y0.data_ptr(), co, 0, n, h, w, 64, co, 1, variant, None, 0, 0, stream())
+    nn_call("raft_nn_pool2", y0.data_ptr(), co, 0, q0.data_ptr(), ldp, 8, n, h // 2, w // 2, co, stream())
+    nn_call("raft_nn_conv3x3_ex", xd.data_ptr(), 64, 0, 0, None, 0, 0, 64, wd.data_ptr(), bd.data_ptr(),
+            y1.data_ptr(), co, 0, n, h, w, 64, co, 1, variant, q1.data_ptr(), ldp, 8, stream())
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1), "regular output"
+    q0, q1 = q0.reshape(-1, ldp), q1.reshape(-1, ldp)
+    assert torch.equal(q0[:, 8:8 + co], q1[:, 8:8 + co]), "pooled output"
+    assert bool(torch.all(q1[:, :8] == 0x3F80)) and bool(torch.all(q1[:, 8 + co:] == 0x3F80))
 
 
 @pytest.mark.parametrize("co", [32, 8, 64])
@@ -192,13 +221,20 @@ def test_conv_first2_equals_first_then_conv(gpu, variant, n, h, w):
             32, 1, stream())
     nn_call("raft_nn_conv3x3", x1.data_ptr(), 32, 0, w1d.data_ptr(), b1d.data_ptr(), y1.data_ptr(), 32, 0, n, h, w, 32,
             32, 1, variant or 13, stream())
-    y2 = torch.full((P * 40,), 0x3F80, dtype=torch.int16, device="cuda")  # bf16 1.0 outside [cy0, cy0 + 32)
+    y2 = torch.full((P * 48,), 0x3F80, dtype=torch.int16, device="cuda")  # bf16 1.0 outside [cy0, cy0 + 32)
+    qp = torch.zeros((n * (h // 2) * (w // 2) * 32,), dtype=torch.int16, device="cuda")
     nn_call("raft_nn_conv_first2", imd.data_ptr(), 0.5, w0d.data_ptr(), 32, b0d.data_ptr(), 1, w1d.data_ptr(),
-            b1d.data_ptr(), y2.data_ptr(), 40, 4, n, h, w, 1, variant, stream())
+            b1d.data_ptr(), y2.data_ptr(), 48, 8, n, h, w, 1, variant, qp.data_ptr() if h % 2 == 0 and w % 2 == 0
+            else None, 32, 0, stream())
+    if h % 2 == 0 and w % 2 == 0:
+        q = torch.zeros_like(qp)
+        nn_call("raft_nn_pool2", y2.data_ptr(), 48, 8, q.data_ptr(), 32, 0, n, h // 2, w // 2, 32, stream())
+        torch.cuda.synchronize()
+        assert torch.equal(q, qp), "pooled output of the fused first layers"
     sep = host_val(y1).reshape(n, h, w, 32)
-    got = host_val(y2).reshape(n, h, w, 40)
-    assert np.all(got[..., :4] == 1.0) and np.all(got[..., 36:] == 1.0)
-    fused = got[..., 4:36]
+    got = host_val(y2).reshape(n, h, w, 48)
+    assert np.all(got[..., :8] == 1.0) and np.all(got[..., 40:] == 1.0)
+    fused = got[..., 8:40]
     xin = val(bits(img * np.float32(0.5)))[..., None]
     mid = val(bits(torch_conv(xin, w0[:, :, :1], b0, relu=True).astype(np.float32)))
     ref = torch_conv(mid, w1, b1, relu=True)
