@@ -133,12 +133,35 @@ __device__ __forceinline__ float ratio(float yv, float v)
 }
 
 // Output of one ray for the 4 slices of group g: Radon line sums, eEM ratio
-// row (+ KL terms), or tEM flat * exp(-v) row.
+// row (+ KL terms), or tEM flat * exp(-v) row.  `part` of the `nsplit` lanes
+// that hold the ray's sum (nsplit in {1, 2, 4}): part 0 writes the row; the
+// FP64 KL terms of slices s with s % nsplit == part are computed by that lane,
+// so each (ray, slice) term is added exactly once.
 template <int EM>
 __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, float4 acc, double* klsum,
-                                        unsigned long long* exc)
+                                        unsigned long long* exc, int part = 0, int nsplit = 1)
 {
     const float v[4] = {acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale};
+    if (EM == 1 && a.kl_part) {
+        // lane `part` takes slices part, part + nsplit, ...: the slice is picked
+        // by selects so that all lanes run one FP64 block per round
+        for (int j = 0; j < 4 / nsplit; ++j) {
+            const int s = part + j * nsplit;
+            const int sl = g * 4 + s;
+            if (sl < a.S) {
+                const float vs = s == 0 ? v[0] : (s == 1 ? v[1] : (s == 2 ? v[2] : v[3]));
+                const double vd = (double)vs, yd = (double)a.y[((size_t)sl * a.nangles + ang) * a.nrays + i];
+                const bool ok = vd > 0.0 && yd >= 0.0;
+                const double t = ok ? vd - yd + (yd > 0.0 ? yd * log(yd / vd) : 0.0) : 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    klsum[q] += q == s ? t : 0.0;
+                    exc[q] += (q == s && !ok) ? 1ull : 0ull;
+                }
+            }
+        }
+    }
+    if (EM == 1 && part > 0) return;
     if (!EM) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -165,11 +188,6 @@ __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, f
             const int sl = g * 4 + s;
             const float yv = (sl < a.S) ? a.y[((size_t)sl * a.nangles + ang) * a.nrays + i] : 0.0f;
             rr[s] = ratio(yv, v[s]) * wv;
-            if (a.kl_part && sl < a.S) {
-                const double vd = (double)v[s], yd = (double)yv;
-                if (vd > 0.0 && yd >= 0.0) klsum[s] += vd - yd + (yd > 0.0 ? yd * log(yd / vd) : 0.0);
-                else exc[s] += 1;
-            }
         }
         reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
             make_float4(rr[0], rr[1], rr[2], rr[3]);
@@ -291,7 +309,9 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
             acc.z += __shfl_xor(acc.z, off);
             acc.w += __shfl_xor(acc.w, off);
         }
-        if (have && kp == 0 && i < a.nrays) ray_out<EM>(a, g, ang, i, acc, klsum, exc);
+        // every lane of the ray holds its sum: up to 4 of them share the KL terms
+        const int nsplit = EM == 1 ? min(Kp, 4) : 1;
+        if (have && kp < nsplit && i < a.nrays) ray_out<EM>(a, g, ang, i, acc, klsum, exc, kp, nsplit);
         __syncthreads(); // keep the waves (angles) of the workgroup on the same ray block
     }
     if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang, lane, klsum, exc);
@@ -498,7 +518,8 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
         acc.y += __shfl_xor(acc.y, 32);
         acc.z += __shfl_xor(acc.z, 32);
         acc.w += __shfl_xor(acc.w, 32);
-        if (ray_ok && kp == 0) ray_out<EM>(a, g, ang, i, acc, klsum, exc);
+        // both phases hold the ray's sum and share its KL terms
+        if (ray_ok && (EM == 1 || kp == 0)) ray_out<EM>(a, g, ang, i, acc, klsum, exc, kp, EM == 1 ? 2 : 1);
     }
     if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc);
 }
