@@ -20,6 +20,7 @@
 #include "../../include/raft_nn.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 
@@ -1152,12 +1153,14 @@ template <int WR, int WC>
 int launch_conv(const ConvArgs& a, hipStream_t st)
 {
     constexpr int lds = conv_lds_bytes<WR, WC>();
-    static bool attr = false;
-    if (!attr) {
+    static std::atomic<bool> attr[64]; // per device: the LDS limit is raised once
+    int attr_dev = 0;
+    if (hipGetDevice(&attr_dev) != hipSuccess || attr_dev < 0 || attr_dev >= 64) return RAFT_EHIP;
+    if (!attr[attr_dev].load(std::memory_order_relaxed)) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3<WR, WC>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
             return RAFT_EHIP;
-        attr = true;
+        attr[attr_dev].store(true, std::memory_order_relaxed);
     }
     dim3 grid((a.W + 31) / 32, ((a.H + 4 * WR - 1) / (4 * WR)) * a.N, a.Cout / (32 * WC));
     if ((long long)((a.H + 4 * WR - 1) / (4 * WR)) * a.N > 65535) return RAFT_EINVAL;
@@ -1170,12 +1173,14 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
 {
     using K = ConvP<WR, WC>;
     constexpr int lds = FIRST ? K::LDS_FIRST : K::LDS;
-    static bool attr = false;
-    if (!attr) {
+    static std::atomic<bool> attr[64]; // per device: the LDS limit is raised once
+    int attr_dev = 0;
+    if (hipGetDevice(&attr_dev) != hipSuccess || attr_dev < 0 || attr_dev >= 64) return RAFT_EHIP;
+    if (!attr[attr_dev].load(std::memory_order_relaxed)) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
             return RAFT_EHIP;
-        attr = true;
+        attr[attr_dev].store(true, std::memory_order_relaxed);
     }
     ConvPArgs pa;
     pa.c = a;
@@ -1187,19 +1192,21 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
     const int ny = a.Cout / K::CO_T;
     // one resident round: as many blocks as the device holds at once (CUs x
     // blocks per CU for this kernel's registers and LDS), tiles split evenly
-    static int slots[64] = {0};
+    static std::atomic<int> slots[64]; // per device, 0 = not yet queried
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return RAFT_EHIP;
-    if (slots[dev] == 0) {
+    int nslots = slots[dev].load(std::memory_order_relaxed);
+    if (nslots == 0) {
         int occ = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(
                                                                    &k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>),
                                                          256, lds) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return RAFT_EHIP;
-        slots[dev] = std::max(1, occ) * std::max(1, cus);
+        nslots = std::max(1, occ) * std::max(1, cus);
+        slots[dev].store(nslots, std::memory_order_relaxed);
     }
-    const long long per = (total * ny + slots[dev] - 1) / slots[dev];
+    const long long per = (total * ny + nslots - 1) / nslots;
     pa.tiles_per_block = (int)std::max(1LL, per);
     const long long nbx = (total + pa.tiles_per_block - 1) / pa.tiles_per_block;
     hipLaunchKernelGGL((k_conv3x3p<WR, WC, UNROLL_DX, HEAD, FIRST, POOL>), dim3((unsigned)nbx, ny), dim3(256), lds, st,
@@ -1458,12 +1465,14 @@ int raft_nn_conv3x3_wgrad(const unsigned short* x, int ldx, int cx0, const unsig
     WgradArgs a{x, g, work, N, H, W, Cin, Cout, ldx, cx0, ldg, cg0, p.per, p.tiles};
     constexpr int lds = (WG_G + 3 * WG_X) * 2;
     static_assert(lds >= 9 * 32 * 32 * 4, "the wave reduction reuses the staging LDS");
-    static bool attr = false;
-    if (!attr) {
+    static std::atomic<bool> attr[64]; // per device: the LDS limit is raised once
+    int attr_dev = 0;
+    if (hipGetDevice(&attr_dev) != hipSuccess || attr_dev < 0 || attr_dev >= 64) return RAFT_EHIP;
+    if (!attr[attr_dev].load(std::memory_order_relaxed)) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3x3_wgrad),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
             return RAFT_EHIP;
-        attr = true;
+        attr[attr_dev].store(true, std::memory_order_relaxed);
     }
     hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(p.nblk, Cout / 32, Cin / 32), dim3(256), lds, S(stream), a);
     if (hipGetLastError() != hipSuccess) return RAFT_EHIP;

@@ -180,7 +180,7 @@ __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, f
         }
         reinterpret_cast<float4*>(a.out)[((size_t)g * a.nangles + ang) * a.W + (i - a.tlo)] =
             make_float4(rr[0], rr[1], rr[2], rr[3]);
-    } else {
+    } else if (a.out) { // null: KL only, no ratio rows
         const float wv = a.wa[ang];
         float rr[4];
 #pragma unroll

@@ -757,15 +757,15 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         b1.n = n; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 0;
         launch_backproject(b1, RAFT_STORE_F32, st);
     }
-    // ratio rows: margins stay zero, the projector writes [0, R)
-    RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)Gb * A * g.width * 16, st));
+    // ratio rows: margins stay zero, the projector writes [0, R) (none for the KL alone)
+    if (!kl_only) RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)Gb * A * g.width * 16, st));
 
     // one iteration: projector + ratio (+ KL partials), KL reduction, backprojection + update
     int* counter = reinterpret_cast<int*>(static_cast<char*>(D.workspace(6, 256)));
     auto iteration = [&](int it, int nb, const float* y, double* klo, unsigned long long* exo, bool graph,
                          hipStream_t s) {
         ProjArgs p;
-        p.img = xil; p.imgT = xT; p.out = rows; p.y = y; p.wa = dwa.as<float>();
+        p.img = xil; p.imgT = xT; p.out = kl_only ? nullptr : rows; p.y = y; p.wa = dwa.as<float>();
         p.kl_part = want_kl ? part : nullptr;
         p.ex_part = want_kl ? expart : nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
