@@ -103,6 +103,14 @@ int raft_radon_device(int device, float* d_proj, const float* d_obj, const float
 int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                     double* kl, long long* kl_excluded, void* stream);
 
+/* raft_eem_device recording the KL objective of the first kl_iterations
+ * iterations only: kl/kl_excluded are HOST arrays [nslices][min(kl_iterations,
+ * iterations)]; the other iterations run without the FP64 KL terms.  No
+ * counterpart in the reference; for callers that test a starting image by its
+ * KL (the learned warm start), which the first iteration computes anyway. */
+int raft_eem_device_kl(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                       int kl_iterations, double* kl, long long* kl_excluded, void* stream);
+
 /* FBP by the Backprojection Slice Theorem (replaces getBSTMultiGPU,
  * cuda/src/geometries/parallel/reconstruction_methods/bst.cu:674-736, bound in
  * rafttypes.py, called from sscRaft/geometries/parallel/fbp_methods.py:200):

@@ -670,8 +670,10 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
 // eEM on device pointers
 // ---------------------------------------------------------------------------
 // kl_only: KL(y || A x) of the given images (one projection, x unchanged), kl[S]
+// kl_iters >= 0: the KL of the first kl_iters iterations only, kl[S][kl_iters]
+// (the others run without the FP64 terms); < 0: of every iteration, kl[S][K]
 static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
-                       double* kl, long long* kl_ex, hipStream_t st, bool kl_only = false)
+                       double* kl, long long* kl_ex, hipStream_t st, bool kl_only = false, int kl_iters = -1)
 {
     if (!parami) fail(RAFT_EINVAL, "null parameter array");
     const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = kl_only ? 1 : parami[8];
@@ -740,12 +742,14 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     double* part = static_cast<double*>(D.workspace(4, (size_t)B * kslots * 16));
     unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * kslots);
     const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
+    // iterations whose KL is recorded (the first Kkl), the row stride of kl
+    const int Kkl = want_kl ? (kl_iters < 0 ? K : std::min(kl_iters, K)) : 0;
     double* dkl = nullptr;
     unsigned long long* dex = nullptr;
-    if (want_kl && K > 0) {
-        double* buf = static_cast<double*>(D.workspace(5, (size_t)S * K * 16));
+    if (Kkl > 0) {
+        double* buf = static_cast<double*>(D.workspace(5, (size_t)S * Kkl * 16));
         dkl = buf;
-        dex = reinterpret_cast<unsigned long long*>(buf + (size_t)S * K);
+        dex = reinterpret_cast<unsigned long long*>(buf + (size_t)S * Kkl);
     }
 
     // backones = B(1): the same for every slice
@@ -763,18 +767,18 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // one iteration: projector + ratio (+ KL partials), KL reduction, backprojection + update
     int* counter = reinterpret_cast<int*>(static_cast<char*>(D.workspace(6, 256)));
     auto iteration = [&](int it, int nb, const float* y, double* klo, unsigned long long* exo, bool graph,
-                         hipStream_t s) {
+                         hipStream_t s, bool with_kl) {
         ProjArgs p;
         p.img = xil; p.imgT = xT; p.out = kl_only ? nullptr : rows; p.y = y; p.wa = dwa.as<float>();
-        p.kl_part = want_kl ? part : nullptr;
-        p.ex_part = want_kl ? expart : nullptr;
+        p.kl_part = with_kl ? part : nullptr;
+        p.ex_part = with_kl ? expart : nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
         p.g = rg; p.S = nb; p.tlo = g.tlo; p.W = g.width; p.lds_rc = prc;
         {
             ProfScope ps(KF_PROJ, s);
             launch_project(p, 1, s);
         }
-        if (want_kl) launch_kl_reduce(part, expart, kslots, nb, klo, exo, K, it, graph ? counter : nullptr, s);
+        if (with_kl) launch_kl_reduce(part, expart, kslots, nb, klo, exo, Kkl, it, graph ? counter : nullptr, s);
         if (kl_only) return;
         BpArgs b;
         b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
@@ -790,16 +794,22 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // (RAFT_EM_GRAPH=0 disables it).  The graph holds kernels only: the counter it
     // advances is cleared by a kernel outside it.
     const char* genv = std::getenv("RAFT_EM_GRAPH");
-    const bool use_graph = !kl_only && K >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
+    // iterations [0, it0) run as direct launches with the KL terms; the rest
+    // [it0, K) with the KL terms when kl_rest, replayed from a graph when there
+    // are at least two of them
+    const bool kl_rest = Kkl > 0 && Kkl == K;
+    const int it0 = kl_rest ? 0 : Kkl;
+    const bool use_graph = !kl_only && K - it0 >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
 
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
         float* x = d_obj + (size_t)s0 * img;
         const float* y = d_tomo + (size_t)s0 * A * R;
-        double* klo = want_kl ? dkl + (size_t)s0 * K : nullptr;
-        unsigned long long* exo = want_kl ? dex + (size_t)s0 * K : nullptr;
+        double* klo = Kkl > 0 ? dkl + (size_t)s0 * Kkl : nullptr;
+        unsigned long long* exo = Kkl > 0 ? dex + (size_t)s0 * Kkl : nullptr;
         launch_to_interleaved(x, xil, nb, n, n, false, st);
         launch_to_interleaved(x, xT, nb, n, n, true, st);
+        for (int it = 0; it < it0; ++it) iteration(it, nb, y, klo, exo, false, st, true);
         if (use_graph) {
             launch_counter(counter, 0, st);
             hipStream_t cs = nullptr;
@@ -809,7 +819,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
             hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
                 try {
-                    iteration(0, nb, y, klo, exo, true, cs);
+                    iteration(0, nb, y, klo, exo, true, cs, kl_rest);
                 } catch (...) {
                     (void)hipStreamEndCapture(cs, &graph);
                     if (graph) (void)hipGraphDestroy(graph);
@@ -820,19 +830,19 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
             }
             if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             if (e == hipSuccess)
-                for (int it = 0; it < K && e == hipSuccess; ++it) e = hipGraphLaunch(exec, st);
+                for (int it = it0; it < K && e == hipSuccess; ++it) e = hipGraphLaunch(exec, st);
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
             (void)hipStreamDestroy(cs);
             RAFT_HIP(e);
         } else {
-            for (int it = 0; it < K; ++it) iteration(it, nb, y, klo, exo, false, st);
+            for (int it = it0; it < K; ++it) iteration(it, nb, y, klo, exo, false, st, kl_rest);
         }
         if (!kl_only) launch_from_interleaved(xil, x, nb, n, n, st);
     }
-    if (want_kl && K > 0) {
-        std::vector<double> hk((size_t)S * K);
-        std::vector<unsigned long long> he((size_t)S * K);
+    if (Kkl > 0) {
+        std::vector<double> hk((size_t)S * Kkl);
+        std::vector<unsigned long long> he((size_t)S * Kkl);
         RAFT_HIP(hipMemcpyAsync(hk.data(), dkl, hk.size() * 8, hipMemcpyDeviceToHost, st));
         RAFT_HIP(hipMemcpyAsync(he.data(), dex, he.size() * 8, hipMemcpyDeviceToHost, st));
         RAFT_HIP(hipStreamSynchronize(st));
@@ -1609,6 +1619,16 @@ int raft_batch_device(int device, int nscans, const int* nrays, const int* nangl
     return guarded([&] {
         batch_device(device, nscans, nrays, nangles, angles, d_tomo, d_y, d_fbp, d_em, paramf, pad, filter, store,
                      iterations, static_cast<hipStream_t>(stream), graph_ms);
+    });
+}
+
+int raft_eem_device_kl(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                       int kl_iterations, double* kl, long long* kl_excluded, void* stream)
+{
+    return guarded([&] {
+        if (kl_iterations < 0) fail(RAFT_EINVAL, "kl_iterations must be >= 0");
+        eem_device(device, d_obj, d_tomo, angles, parami, kl_iterations > 0 ? kl : nullptr,
+                   kl_iterations > 0 ? kl_excluded : nullptr, static_cast<hipStream_t>(stream), false, kl_iterations);
     });
 }
 

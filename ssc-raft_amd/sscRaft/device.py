@@ -68,9 +68,12 @@ def radon_device(img, angles, nrays=None, ax=1.0, ay=1.0, out=None, stream=None)
     return out
 
 
-def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None):
+def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None, kl_iterations=None):
     """eEM of tomo [S, A, R]; obj [S, R, R] is the in/out initial guess (ones if None).
-    Returns (obj, kl [S, K] float64 or None, excluded [S, K] or None)."""
+    Returns (obj, kl [S, K] float64 or None, excluded [S, K] or None).
+    kl_iterations: record the KL of the first kl_iterations iterations only
+    (kl [S, min(kl_iterations, K)], raft_eem_device_kl); the others run
+    without the FP64 KL terms."""
     import torch
 
     _check_tensor(tomo, "tomo")
@@ -80,17 +83,22 @@ def eem_device(tomo, angles, iterations, obj=None, with_kl=True, stream=None):
     _check_tensor(obj, "obj")
     angles = CNICE(numpy.asarray(angles))
     parami = CNICE(numpy.array([R, A, S, obj.shape[-1], 0, 0, 0, 1, iterations, 0]), numpy.int32)
-    kl = numpy.zeros((S, max(iterations, 1)), dtype=numpy.float64) if with_kl else None
-    ex = numpy.zeros((S, max(iterations, 1)), dtype=numpy.int64) if with_kl else None
-    rc = libraft.raft_eem_device(tomo.device.index or 0, ctypes.c_void_p(obj.data_ptr()),
-                                 ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
-                                 parami.ctypes.data_as(ctypes.c_void_p),
-                                 kl.ctypes.data_as(ctypes.c_void_p) if with_kl else None,
-                                 ex.ctypes.data_as(ctypes.c_void_p) if with_kl else None, _stream(tomo, stream))
+    K = iterations if kl_iterations is None else min(kl_iterations, iterations)
+    with_kl = with_kl and K > 0
+    kl = numpy.zeros((S, max(K, 1)), dtype=numpy.float64) if with_kl else None
+    ex = numpy.zeros((S, max(K, 1)), dtype=numpy.int64) if with_kl else None
+    args = (ctypes.c_void_p(obj.data_ptr()), ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
+            parami.ctypes.data_as(ctypes.c_void_p))
+    outs = (kl.ctypes.data_as(ctypes.c_void_p) if with_kl else None,
+            ex.ctypes.data_as(ctypes.c_void_p) if with_kl else None, _stream(tomo, stream))
+    if kl_iterations is None:
+        rc = libraft.raft_eem_device(tomo.device.index or 0, *args, *outs)
+    else:
+        rc = libraft.raft_eem_device_kl(tomo.device.index or 0, *args, int(kl_iterations), *outs)
     if rc:
         raft_check("raft_eem_device")
     if with_kl:
-        return obj, kl[:, :iterations], ex[:, :iterations]
+        return obj, kl[:, :K], ex[:, :K]
     return obj, None, None
 
 

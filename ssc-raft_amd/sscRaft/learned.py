@@ -200,6 +200,35 @@ def warm_start(net, p_hat, y, ang, cfg=WarmStartConfig, batch=64, check=True):
     return x0, accept, xf, (kl_net, kl_fbp)
 
 
+def learned_eem(net, p_hat, y, ang, iterations, cfg=WarmStartConfig, batch=64):
+    """eEM from the learned start, with the acceptance rule of warm_start folded
+    into the first iteration: the network start's KL(y || A x0) is the KL the
+    first iteration computes (eem_device(..., kl_iterations=1)), so only the
+    FBP start needs a projection of its own.  Slices whose network start has a
+    higher KL (or more excluded bins) than their FBP start are run again from
+    the FBP start.  The result equals warm_start followed by eem_device.
+
+    Returns (x [S, N, N], accepted mask)."""
+    import torch
+
+    x0, _, xf, _ = warm_start(net, p_hat, y, ang, cfg, batch=batch, check=False)
+    kl_fbp, ex_fbp = eem_kl_device(y, ang, xf)
+    if iterations == 0:
+        kl_net, ex_net = eem_kl_device(y, ang, x0)
+        x = x0
+    else:
+        x, kl, ex = eem_device(y, ang, iterations, obj=x0, kl_iterations=1)
+        kl_net, ex_net = kl[:, 0], ex[:, 0]
+    accept = (kl_net <= kl_fbp) & (ex_net <= ex_fbp)
+    if not accept.all():
+        idx = torch.from_numpy(np.nonzero(~accept)[0]).to(x.device)
+        xr = xf.index_select(0, idx).contiguous()
+        if iterations > 0:
+            eem_device(y.index_select(0, idx).contiguous(), ang, iterations, obj=xr, with_kl=False)
+        x.index_copy_(0, idx, xr)
+    return x, accept
+
+
 def iterations_to_reach(curve, target):
     """First k (number of updates) with curve[:, k] <= target per slice, -1 if never."""
     hit = curve <= target[:, None]
@@ -256,9 +285,7 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
     # timed runs over all slices: each start runs until its last (reaching) slice is there
     if Kl is not None:
         def learned():
-            x, _, _, _ = warm_start(net, p_hat, y, ang, cfg)
-            if Kl > 0:
-                eem_device(y, ang, Kl, obj=x, with_kl=False)
+            x, _ = learned_eem(net, p_hat, y, ang, Kl, cfg)
             return x
 
         _, t_learned = timed(learned)

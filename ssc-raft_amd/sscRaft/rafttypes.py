@@ -57,6 +57,8 @@ libraft.raft_radon_device.argtypes = [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f,
 libraft.raft_radon_device.restype = _i
 libraft.raft_eem_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_eem_device.restype = _i
+libraft.raft_eem_device_kl.argtypes = [_i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]
+libraft.raft_eem_device_kl.restype = _i
 libraft.getBSTMultiGPU.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i]
 libraft.getBSTMultiGPU.restype = None
 libraft.raft_bst_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp]

@@ -85,3 +85,37 @@ def test_trained_network_gains_over_fbp_and_uniform_starts(gpu):
     assert np.median(kl0_net) < 0.5 * np.median(kl0_fbp)
     assert np.median(k_net[k_net >= 0]) < np.median(k_fbp[k_fbp >= 0]) < np.median(k_plain)
     assert (k_net >= 0).sum() >= 22
+
+
+def test_learned_eem_equals_warm_start_then_eem(gpu, monkeypatch):
+    """learned_eem (acceptance folded into the first iteration's KL) gives the
+    same acceptance and the same image as the warm start's separate KL checks
+    followed by eEM; one slice's network start is spoiled so that it is
+    rejected and rerun from its FBP start."""
+    import torch
+
+    from sscRaft import learned
+    from sscRaft.device import eem_device, eem_kl_device
+    from sscRaft.nn import UNet
+
+    net = UNet.load(learned.default_weights_path())
+    cfg = learned.WarmStartConfig
+    p_hat, y, _, ang = learned.sinograms(8, cfg.heldout_first + 7000, cfg)
+    orig = learned.warm_start
+
+    def spoiled(*args, **kwargs):
+        x0, acc, xf, kls = orig(*args, **kwargs)
+        x0[3].mul_(40.0)  # far worse than the FBP start
+        return x0, acc, xf, kls
+
+    monkeypatch.setattr(learned, "warm_start", spoiled)
+    x0, _, xf, _ = learned.warm_start(net, p_hat, y, ang, cfg, check=False)
+    kn, en = eem_kl_device(y, ang, x0)
+    kf, ef = eem_kl_device(y, ang, xf)
+    acc_ref = (kn <= kf) & (en <= ef)
+    assert not acc_ref[3] and acc_ref.sum() >= 6
+    ref = torch.where(torch.from_numpy(acc_ref).cuda()[:, None, None], x0, xf).contiguous()
+    eem_device(y, ang, 5, obj=ref, with_kl=False)
+    x, acc = learned.learned_eem(net, p_hat, y, ang, 5, cfg)
+    np.testing.assert_array_equal(acc, acc_ref)
+    assert torch.equal(x, ref)

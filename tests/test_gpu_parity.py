@@ -310,6 +310,29 @@ def test_eem_graph_replay_matches_direct_launches(gpu, monkeypatch):
     assert rel(x1, xr) <= 1e-5
 
 
+@pytest.mark.parametrize("kl_iterations", [0, 1, 2, 7])
+def test_eem_kl_of_first_iterations_only(gpu, kl_iterations):
+    """raft_eem_device_kl: the same image as a full run, and the KL of the
+    first kl_iterations iterations equal to the full run's first columns."""
+    import torch
+    from sscRaft import device as dv
+
+    n, a, k = 48, 40, 5
+    img = ph.phantoms(n, 3, config=3, first=11)
+    ang = ph.uniform_angles(a)
+    y = torch.from_numpy(np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=12), 0).astype(np.float32)).cuda()
+    x1, kl1, ex1 = dv.eem_device(y, ang, k)
+    x2, kl2, ex2 = dv.eem_device(y, ang, k, kl_iterations=kl_iterations)
+    assert torch.equal(x1, x2)
+    m = min(kl_iterations, k)
+    if m == 0:
+        assert kl2 is None and ex2 is None
+    else:
+        assert kl2.shape == (3, m)
+        np.testing.assert_array_equal(kl2, kl1[:, :m])
+        np.testing.assert_array_equal(ex2, ex1[:, :m])
+
+
 def test_eem_kl_only_matches_first_iteration(gpu):
     """raft_eem_kl_device: KL of the given image, image unchanged, equal to KL[0] of an eEM run."""
     import torch
