@@ -246,7 +246,7 @@ class UNet:
                 hw[1], co, ci, 0, v, st)
 
     # ---------------------------------------------------------------- tuning
-    CONV_VARIANTS = (1, 2, 3, 5, 6, 7, 9, 10, 11, 13)
+    CONV_VARIANTS = (1, 2, 3, 5, 6, 7, 9, 10, 11, 13)  # the 16x64 forms (8, 12) measured slower on every shape
     HEAD_VARIANTS = (6, 9, 10, 13)  # persistent forms with one 32-channel output tile (head / first-layer modes)
 
     def autotune(self, n, h, w, backward=False, repeats=5, stream=None):
