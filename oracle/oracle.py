@@ -211,7 +211,11 @@ def tem(count, flat, angles, iterations, obj=None, prec="f64"):
         count = count[None]
     S, A, R = count.shape
     flat = np.asarray(flat, dtype=np.float32).reshape(S, R)
-    x = np.ones((S, R, R)) if obj is None else np.array(obj, dtype=np.float64, copy=True).reshape(S, -1, R)
+    if obj is None:
+        x = np.ones((S, R, R))
+    else:
+        x = np.array(obj, dtype=np.float64, copy=True)
+        x = x.reshape(S, x.shape[-2], x.shape[-1])  # n x n, n = objsize (may differ from R)
     n = x.shape[-1]
     for s in range(S):
         bc = em_backproject(count[s], angles, n, prec=prec)

@@ -426,6 +426,28 @@ def test_tem(gpu, n, a, k):
     assert e <= 1e-5 * max(1, k)
 
 
+@pytest.mark.parametrize("nrays,n", [(48, 64), (64, 60)])
+def test_tem_objsize_differs_from_nrays(gpu, nrays, n):
+    """Transmission EM of an n x n image from nrays-bin rows (objsize != nrays)
+    against the oracle; the image side comes from the initial guess."""
+    import torch
+    from sscRaft import device as dv
+
+    a, k = 36, 3
+    rng = np.random.default_rng(nrays + n)
+    img = ph.phantoms(nrays, 2, config=3, first=n) * 0.5
+    ang = ph.uniform_angles(a)
+    count = np.exp(-ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n)).astype(np.float32)
+    flat = rng.uniform(0.9, 1.1, (2, nrays)).astype(np.float32)
+    x0 = rng.uniform(0.5, 1.5, (2, n, n)).astype(np.float32)
+    x = dv.tem_device(torch.from_numpy(count).cuda(), torch.from_numpy(flat).cuda(), ang, k,
+                      obj=torch.from_numpy(x0).cuda()).cpu().numpy()
+    xr = oracle.tem(count, flat, ang, k, obj=x0)
+    e = rel(x, xr)
+    print(f"tem objsize {n} nrays {nrays}: rel-L2 {e:.2e}")
+    assert x.shape == (2, n, n) and e <= 1e-5 * k
+
+
 def test_tem_dispatch_and_device_entry(gpu):
     import torch
     import sscRaft

@@ -416,6 +416,21 @@ def test_eem_objsize_one_update_composes_the_operators():
     assert x1.shape == (1, n, n)
 
 
+def test_tem_objsize_one_update_composes_the_operators():
+    """One tEM update with objsize != nrays equals x * B(flat exp(-A x)) / B(count)
+    built from the separately pinned forward and backprojection."""
+    n, R, A = 11, 8, 6
+    ang = np.sort(RNG.uniform(0.1, 3.0, A)).astype(np.float32)
+    x0 = RNG.uniform(0.5, 1.5, (n, n))
+    count = np.exp(-oracle.em_forward(x0 * 0.7, ang, R)).astype(np.float32)
+    flat = RNG.uniform(0.9, 1.1, R).astype(np.float32)
+    t = flat.astype(np.float64)[None, :] * np.exp(-oracle.em_forward(x0, ang, R))
+    expect = x0 * oracle.em_backproject(t, ang, n) / oracle.em_backproject(count.astype(np.float64), ang, n)
+    x1 = oracle.tem(count[None], flat[None], ang, 1, obj=x0[None])
+    assert x1.shape == (1, n, n)
+    np.testing.assert_allclose(x1[0], expect, rtol=1e-12, atol=1e-12)
+
+
 def test_em_forward_is_radon_geometry():
     """kernel_radonWithDivision samples exactly like Radon_RT with a = 1 and nrays = N."""
     for n in (16, 17):
