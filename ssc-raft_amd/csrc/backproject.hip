@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
             for (int s = 0; s < SB; ++s) tot[q][p][s] = 0.0f;
 
     for (int a0 = 0; a0 < a.A; a0 += BP_CHUNK) {
-        const int nA = min(BP_CHUNK, a.A - a0);
+        const int nA = __builtin_amdgcn_readfirstlane(min(BP_CHUNK, a.A - a0)); // uniform: scalar loop counter
         __syncthreads();
         // ---- per-angle bin windows of this tile (one thread per angle) ----
         if (tid < nA) {
@@ -162,6 +162,8 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
 #pragma unroll
                 for (int s = 0; s < SB; ++s) part[q][p][s] = 0.0f;
         const char* wbase = reinterpret_cast<const char*>(win);
+        // 4 angles per trip: their coefficients come from one LDS base by immediate offsets
+#pragma unroll 4
         for (int al = 0; al < nA; ++al) {
             const double cA = s_cA[al], cB = s_cB[al];
             // byte offset of bin 0 of this angle's window: entry (al, T - tl)
