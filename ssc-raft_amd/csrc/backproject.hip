@@ -154,13 +154,17 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
         }
         __syncthreads();
         // ---- sweep ----
-        float part[2][4][SB];
+        // FP32 rows: per-chunk partial sums added to the totals (two-level
+        // summation, for the 1e-5 bar); FP16 rows sum into the totals directly
+        // (their error is the FP16 storage; 64 fewer registers, occupancy 3)
+        constexpr bool TWO_LEVEL = SB == 4;
+        float part[2][4][TWO_LEVEL ? SB : 1];
 #pragma unroll
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int s = 0; s < SB; ++s) part[q][p][s] = 0.0f;
+                for (int s = 0; s < (TWO_LEVEL ? SB : 1); ++s) part[q][p][s] = 0.0f;
         const char* wbase = reinterpret_cast<const char*>(win);
         // 4 angles per trip: their coefficients come from one LDS base by immediate offsets
 #pragma unroll 4
@@ -184,14 +188,16 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
 #pragma unroll
             for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int p = 0; p < 4; ++p) Acc<QT>::add(part[q][p], v[q][p]);
+                for (int p = 0; p < 4; ++p) Acc<QT>::add(TWO_LEVEL ? part[q][p] : tot[q][p], v[q][p]);
         }
+        if constexpr (TWO_LEVEL) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int p = 0; p < 4; ++p)
+                for (int p = 0; p < 4; ++p)
 #pragma unroll
-                for (int s = 0; s < SB; ++s) tot[q][p][s] += part[q][p][s];
+                    for (int s = 0; s < SB; ++s) tot[q][p][s] += part[q][p][s];
+        }
     }
 
     // ---- epilogue ----
