@@ -481,11 +481,14 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                 __syncthreads();
                 if (have && ks < wkh && ks + PL_K > wkl) {
                     // byte offset of sample (R, C) in the tile: R * pitch16 - off16 + C * 16
-                    // (24-bit multiply-add, then shift-add), clamped to the tile (the box holds
-                    // every sample; the clamp is a guard)
+                    // (24-bit multiply-add, then shift-add).  Every sample lies in its wave's
+                    // patch_box (margins of 1 and 2 rows / columns around the FP64 extremes,
+                    // far beyond their rounding difference), so in the union box: the offset
+                    // is below nr * pitch * 16 <= PL_CAP * 16, the bound proj_plan admitted
+                    // the launch with.  No clamp is needed; the staging clamp of nr keeps the
+                    // tile's writes in bounds
                     const int pitch16 = pitch << 4;
                     const int moff16 = -((u.x * pitch + u.z) << 4);
-                    const unsigned cap16 = (unsigned)(PL_CAP - 1) << 4;
                     const char* tb = reinterpret_cast<const char*>(tile);
                     double kr = (double)(ks + kp) - a.ck;
                     float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f); // this step's sum (two-level summation)
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                         for (int j = 0; j < PL_K / 2; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const unsigned e = min(((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16), cap16);
+                            const unsigned e = ((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16);
                             sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                         for (int j = 0; j < jmax; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const unsigned e = min(((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16), cap16);
+                            const unsigned e = ((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16);
                             sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
