@@ -334,6 +334,18 @@ __device__ __forceinline__ int4 patch_box(double ir0, double kr0, double rU, dou
     return make_int4((int)floor(rlo) - 1, (int)floor(rhi) + 3, (int)floor(clo) - 1, (int)floor(chi) + 3);
 }
 
+// byte offset ((R * pitch + C) * 16 - off16) of a staged sample in two VALU ops:
+// a signed 24-bit multiply-add (R may be negative: the box extends past the image
+// edge; |R|, pitch * 16 < 2^23) and a shift-add (the compiler's form takes three)
+__device__ __forceinline__ unsigned tile_off(int R, int C, int pitch16, int moff16)
+{
+    int rp;
+    unsigned e;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(rp) : "v"(R), "v"(pitch16), "v"(moff16));
+    asm("v_lshl_add_u32 %0, %1, 4, %2" : "=v"(e) : "v"(C), "v"(rp));
+    return e;
+}
+
 template <int EM>
 __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
 {
@@ -498,7 +510,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                         for (int j = 0; j < PL_K / 2; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const unsigned e = ((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16);
+                            const unsigned e = tile_off(R, C, pitch16, moff16);
                             sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
@@ -506,7 +518,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
                         for (int j = 0; j < jmax; ++j) {
                             const int R = (int)fma(kr, rK, rowi);
                             const int C = (int)fma(kr, cK, coli);
-                            const unsigned e = ((unsigned)C << 4) + (unsigned)(__mul24(R, pitch16) + moff16);
+                            const unsigned e = tile_off(R, C, pitch16, moff16);
                             sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
                             kr += 2.0;
                         }
