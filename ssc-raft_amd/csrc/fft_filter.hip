@@ -137,18 +137,23 @@ template <> struct Dft<5> : DftGeneric<5> {};
 // Ns is a power of two for every pass before the first odd radix (radices
 // are ordered 8, 4, 2 first), so jm is a mask there.
 template <int R, int THREADS, int E>
-__device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns)
+__device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns, int twstep, unsigned nsm, int nss)
 {
     constexpr int MAXB = (E + R - 1) / R;
     const int nb = n / R;
-    const int twstep = n / (Ns * R);
     const bool pow2 = (Ns & (Ns - 1)) == 0;
+    // j mod Ns for j < nb: q = (j * nsm) >> nss equals j / Ns exactly (the host
+    // picks nsm = ceil(2^nss / Ns) with 2^nss >= nb * Ns and checks every j),
+    // in 24-bit full-rate multiplies instead of a 32-bit division
+    auto jmod = [&](int j) __attribute__((always_inline)) {
+        return pow2 ? (j & (Ns - 1)) : (int)((unsigned)j - __umul24(__umul24((unsigned)j, nsm) >> nss, (unsigned)Ns));
+    };
     float2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = threadIdx.x + b * THREADS;
         if (j < nb) {
-            const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
+            const int jm = jmod(j);
             // W^(r ts) = (W^ts)^r: one table twiddle per butterfly, the rest by
             // recurrence (at most R - 2 extra roundings)
             const float2 w1 = tw(jm * twstep);
@@ -167,7 +172,7 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns)
         const int j = threadIdx.x + b * THREADS;
         if (j < nb) {
             Dft<R>::run(v[b], tw, n);
-            const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
+            const int jm = jmod(j);
             const int d = (j - jm) * R + jm;
 #pragma unroll
             for (int r = 0; r < R; ++r) z[pix(d + r * Ns)] = v[b][r];
@@ -179,6 +184,10 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns)
 struct FftK {
     int n, nstages;
     int radix[24];
+    // per stage: twiddle step n / (Ns R) and the exact-division pair for Ns
+    int twstep[24];
+    unsigned nsm[24];
+    int nss[24];
 };
 
 template <int THREADS, int E>
@@ -187,12 +196,14 @@ __device__ void fft_run(float2* z, const Tw& tw, const FftK& p)
     int Ns = 1;
     for (int st = 0; st < p.nstages; ++st) {
         const int R = p.radix[st];
+        const int ts = p.twstep[st], ss = p.nss[st];
+        const unsigned sm = p.nsm[st];
         switch (R) {
-        case 8: fft_pass<8, THREADS, E>(z, tw, p.n, Ns); break;
-        case 4: fft_pass<4, THREADS, E>(z, tw, p.n, Ns); break;
-        case 2: fft_pass<2, THREADS, E>(z, tw, p.n, Ns); break;
-        case 3: fft_pass<3, THREADS, E>(z, tw, p.n, Ns); break;
-        default: fft_pass<5, THREADS, E>(z, tw, p.n, Ns); break;
+        case 8: fft_pass<8, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
+        case 4: fft_pass<4, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
+        case 2: fft_pass<2, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
+        case 3: fft_pass<3, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
+        default: fft_pass<5, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
         }
         Ns *= R;
     }
@@ -520,7 +531,34 @@ FftK fftk_of(const FftPlan& plan)
     FftK p;
     p.n = plan.n;
     p.nstages = plan.nstages;
-    for (int i = 0; i < 24; ++i) p.radix[i] = plan.radix[i];
+    for (int i = 0; i < 24; ++i) {
+        p.radix[i] = plan.radix[i];
+        p.twstep[i] = 0;
+        p.nsm[i] = 0;
+        p.nss[i] = 0;
+    }
+    // stage st works on Ns = radix[0] * .. * radix[st-1] and j < nb = n / radix[st]:
+    // nsm = ceil(2^nss / Ns) with 2^nss >= nb * Ns makes (j * nsm) >> nss == j / Ns
+    // for every j < nb (the error j * (nsm Ns - 2^nss) / (Ns 2^nss) stays below the
+    // gap 1/Ns); nsm <= 2 nb + 1 < 2^24 and j * nsm < 2^32, as the 24-bit multiplies need
+    long long Ns = 1;
+    for (int st = 0; st < p.nstages; ++st) {
+        const int R = p.radix[st];
+        const long long nb = p.n / R;
+        p.twstep[st] = (int)(p.n / (Ns * R));
+        int k = 0;
+        while ((1LL << k) < nb * Ns) ++k;
+        const long long m = ((1LL << k) + Ns - 1) / Ns;
+        if (m >= (1LL << 24) || (nb - 1) * m >= (1LL << 32) || nb >= (1LL << 24))
+            fail(RAFT_EUNSUPPORTED, "internal: FFT stage index out of the 24-bit range");
+        if ((Ns & (Ns - 1)) != 0)
+            for (long long j = 0; j < nb; ++j)
+                if ((long long)(((unsigned long long)j * (unsigned long long)m) >> k) != j / Ns)
+                    fail(RAFT_EUNSUPPORTED, "internal: FFT stage division check failed");
+        p.nsm[st] = (unsigned)m;
+        p.nss[st] = k;
+        Ns *= R;
+    }
     return p;
 }
 
@@ -568,10 +606,7 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
     const size_t lds = sizeof(float2) * lds_elems(a.npad);
     const size_t lds_fft = lds + sizeof(float2) * (size_t)(128 + a.npad / 128 + 1);
     if (plan.ok || a.none) {
-        FftK p;
-        p.n = plan.n;
-        p.nstages = plan.nstages;
-        for (int i = 0; i < 24; ++i) p.radix[i] = plan.radix[i];
+        const FftK p = fftk_of(plan);
         const int threads = plan.ok ? plan.threads : 256;
         if (store == RAFT_STORE_F16) launch_fft<__half>(k, p, threads, (int)blocks, lds_fft, st);
         else launch_fft<float>(k, p, threads, (int)blocks, lds_fft, st);
