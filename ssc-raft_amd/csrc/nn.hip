@@ -192,6 +192,27 @@ __device__ __forceinline__ uint4 conv_src(const ConvArgs& a, int n, int gy, int 
     return *reinterpret_cast<const uint4*>(a.x + p * a.ldx + a.cx0 + c);
 }
 
+// x / d by a 24-bit multiply-shift, exact for the ranges used below (checked
+// exhaustively: /34 below 4096, /36 below 3311, /9 below 8192)
+__device__ __forceinline__ int div34(int x) { return (int)(__umul24((unsigned)x, 1928u) >> 16); }
+__device__ __forceinline__ int div36(int x) { return (int)(__umul24((unsigned)x, 1821u) >> 16); }
+__device__ __forceinline__ int div9(int x) { return (int)(__umul24((unsigned)x, 7282u) >> 16); }
+
+// conv_src for the persistent form: the same values, with 32-bit offsets from
+// per-image bases (the host admits only H W < 2^24 and H W ld < 2^31) and
+// 24-bit multiplies; the pooled input keeps conv_src
+__device__ __forceinline__ uint4 conv_src_fast(const ConvArgs& a, const u16* xb, const u16* x2b, int n, int gy, int gx,
+                                               int c)
+{
+    if (c >= a.split) {
+        const unsigned pix = __umul24((unsigned)(gy >> 1), (unsigned)(a.W >> 1)) + (unsigned)(gx >> 1);
+        return *reinterpret_cast<const uint4*>(x2b + __umul24(pix, (unsigned)a.ldx2) + (c - a.split));
+    }
+    if (a.pool) return conv_src(a, n, gy, gx, c);
+    const unsigned pix = __umul24((unsigned)gy, (unsigned)a.W) + (unsigned)gx;
+    return *reinterpret_cast<const uint4*>(xb + __umul24(pix, (unsigned)a.ldx) + c);
+}
+
 template <int WR, int WC>
 constexpr int conv_lds_bytes()
 {
@@ -357,48 +378,79 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
         for (int i = 0; i < 16; ++i) b0v[i] = a.b0 ? a.b0[8 * (i / 4) + 4 * hi + (i % 4)] : 0.0f;
     }
 
+    // Item cursors (uniform): chunk, tile column / row, image.  Items run in
+    // order, so a cursor advances by increments; only the first tile is divided out.
+    struct Cursor {
+        int ch, tx, ty, n;
+    };
+    auto cursor_at = [&](int t) __attribute__((always_inline)) {
+        Cursor c;
+        c.ch = 0;
+        c.tx = t % pa.tiles_x;
+        c.ty = (t / pa.tiles_x) % pa.tiles_y;
+        c.n = t / (pa.tiles_x * pa.tiles_y);
+        return c;
+    };
+    auto advance = [&](Cursor& c) __attribute__((always_inline)) {
+        if (++c.ch == nch) {
+            c.ch = 0;
+            if (++c.tx == pa.tiles_x) {
+                c.tx = 0;
+                if (++c.ty == pa.tiles_y) {
+                    c.ty = 0;
+                    ++c.n;
+                }
+            }
+        }
+    };
+    Cursor lc = cursor_at(t0); // next item to load
+    Cursor cc = lc;            // item being computed
+
     u32x4 hreg[FIRST ? 1 : K::HU], wreg[K::WU];
     float preg[FIRST ? K::PU : 1];
-    auto load_item = [&](int it, bool weights) __attribute__((always_inline)) {
-        const int t = t0 + it / nch, ch = it % nch;
-        const int tx0 = (t % pa.tiles_x) * 32;
-        const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
-        const int n = t / (pa.tiles_x * pa.tiles_y);
+    // weight unit u: row x = u / 4 = co * 9 + tap of [Cout][9][Cin], 8 channels at 8 (u % 4)
+    auto load_item = [&](const Cursor& c, bool weights) __attribute__((always_inline)) {
+        const int ch = c.ch, tx0 = c.tx * 32, ty0 = c.ty * K::TH, n = c.n;
         if constexpr (FIRST) {
             // image patch rows ty0-2 .. ty0+TH+1, columns tx0-2 .. tx0+33, bf16-rounded
+            const float* ib = a.img0 + (size_t)n * a.H * a.W;
 #pragma unroll
             for (int k = 0; k < K::PU; ++k) {
                 const int u = tid + 256 * k;
                 float v = 0.0f;
                 if (u < K::PATCH) {
-                    const int gy = ty0 - 2 + u / K::PW, gx = tx0 - 2 + u % K::PW;
-                    if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                        v = bf16_float(bf16_bits(a.img0[((size_t)n * a.H + gy) * a.W + gx] * a.scale0));
+                    const int py = div36(u);
+                    const int gy = ty0 - 2 + py, gx = tx0 - 2 + (u - py * K::PW);
+                    if ((unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W)
+                        v = bf16_float(bf16_bits(ib[__umul24((unsigned)gy, (unsigned)a.W) + (unsigned)gx] * a.scale0));
                 }
                 preg[k] = v;
             }
         } else {
+            const u16* xb = a.x + (size_t)n * a.H * a.W * a.ldx + a.cx0;
+            const u16* x2b = a.x2 ? a.x2 + (size_t)n * (a.H >> 1) * (a.W >> 1) * a.ldx2 + a.cx2 : nullptr;
 #pragma unroll
             for (int k = 0; k < K::HU; ++k) {
                 const int u = tid + 256 * k;
                 uint4 v = make_uint4(0u, 0u, 0u, 0u);
                 if (u < K::HALO_UNITS) {
                     const int pix = u >> 2, q = u & 3;
-                    const int gy = ty0 + pix / K::HW - 1, gx = tx0 + pix % K::HW - 1;
-                    if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) v = conv_src(a, n, gy, gx, 32 * ch + 8 * q);
+                    const int hy = div34(pix);
+                    const int gy = ty0 + hy - 1, gx = tx0 + (pix - hy * K::HW) - 1;
+                    if ((unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W)
+                        v = conv_src_fast(a, xb, x2b, n, gy, gx, 32 * ch + 8 * q);
                 }
                 hreg[k] = as_u32x4(v);
             }
         }
         if (weights) {
+            const u16* wb = a.w + 32 * ch;
 #pragma unroll
             for (int k = 0; k < K::WU; ++k) {
                 const int u = tid + 256 * k;
-                if (u < K::W_UNITS) {
-                    const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
-                    wreg[k] = *reinterpret_cast<const u32x4*>(a.w + ((size_t)(co0 + co) * 9 + tap) * a.Cin + 32 * ch +
-                                                              8 * q);
-                }
+                if (u < K::W_UNITS)
+                    wreg[k] = *reinterpret_cast<const u32x4*>(
+                        wb + __umul24((unsigned)(co0 * 9 + (u >> 2)), (unsigned)a.Cin) + 8 * (u & 3));
             }
         }
     };
@@ -421,8 +473,8 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             for (int k = 0; k < K::WU; ++k) {
                 const int u = tid + 256 * k;
                 if (u < K::W_UNITS) {
-                    const int q = u & 3, tap = (u >> 2) % 9, co = (u >> 2) / 9;
-                    *reinterpret_cast<u32x4*>(s_w + (tap * K::CO_T + co) * CONV_PS + 8 * q) = wreg[k];
+                    const int x = u >> 2, co = div9(x), tap = x - 9 * co;
+                    *reinterpret_cast<u32x4*>(s_w + (tap * K::CO_T + co) * CONV_PS + 8 * (u & 3)) = wreg[k];
                 }
             }
         }
@@ -436,22 +488,26 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
 
-    if (items > 0) load_item(0, true);
+    if (items > 0) {
+        load_item(lc, true);
+        advance(lc);
+    }
     for (int it = 0; it < items; ++it) {
-        const int ch = it % nch;
+        const int ch = cc.ch;
         __syncthreads();
         store_item(it == 0 || nch > 1);
         __syncthreads();
-        if (it + 1 < items) load_item(it + 1, nch > 1);
+        if (it + 1 < items) {
+            load_item(lc, nch > 1);
+            advance(lc);
+        }
         if constexpr (FIRST) {
             // the halo's first-layer channels, one MFMA per 32 halo pixels
-            const int t = t0 + it / nch;
-            const int tx0 = (t % pa.tiles_x) * 32;
-            const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
+            const int tx0 = cc.tx * 32, ty0 = cc.ty * K::TH;
             for (int grp = wave; grp < K::HALO_GROUPS; grp += 4) {
                 const int hp = 32 * grp + l32;
                 const bool valid = hp < K::HH * K::HW;
-                const int hy = valid ? hp / K::HW : 0, hx = valid ? hp % K::HW : 0;
+                const int hy = valid ? div34(hp) : 0, hx = valid ? hp - hy * K::HW : 0;
                 bf16x8 bv;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
@@ -487,10 +543,8 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
         float resv[WR];
         if constexpr (HEAD && WC == 1) {
             if (ch == nch - 1) {
-                const int t = t0 + it / nch;
-                const int tx0 = (t % pa.tiles_x) * 32;
-                const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
-                const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+                const int tx0 = cc.tx * 32, ty0 = cc.ty * K::TH;
+                const size_t img = (size_t)cc.n * a.H * a.W;
 #pragma unroll
                 for (int r = 0; r < WR; ++r) {
                     const int row = ty0 + wave * WR + r;
@@ -522,10 +576,8 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             }
         }
         if (ch == nch - 1) {
-            const int t = t0 + it / nch;
-            const int tx0 = (t % pa.tiles_x) * 32;
-            const int ty0 = ((t / pa.tiles_x) % pa.tiles_y) * K::TH;
-            const size_t img = (size_t)(t / (pa.tiles_x * pa.tiles_y)) * a.H * a.W;
+            const int tx0 = cc.tx * 32, ty0 = cc.ty * K::TH;
+            const size_t img = (size_t)cc.n * a.H * a.W;
             if constexpr (HEAD && WC == 1) {
                 // lane (l32, hi) holds channels 8g + 4hi + i of pixel column l32;
                 // the hi = 0 lane gathers its partner's 16 and sums all 32 in order
@@ -591,8 +643,7 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
                     const int row = ty0 + wave * WR + r - 1;
                     if (row >= a.H) break; // wave-uniform; H even, so row + 1 < H
                     const bool store = (l32 & 1) == 0 && tx0 + l32 < a.W;
-                    const size_t pp = ((size_t)(t / (pa.tiles_x * pa.tiles_y)) * (a.H >> 1) + (row >> 1)) * (a.W >> 1) +
-                                      ((tx0 + l32) >> 1);
+                    const size_t pp = ((size_t)cc.n * (a.H >> 1) + (row >> 1)) * (a.W >> 1) + ((tx0 + l32) >> 1);
                     u16* qp = a.ypool + pp * a.ldp + a.cp0;
 #pragma unroll
                     for (int c = 0; c < WC; ++c)
@@ -633,6 +684,7 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
 #pragma unroll
                     for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
         }
+        advance(cc);
     }
 }
 
@@ -1184,6 +1236,11 @@ int launch_convp(const ConvArgs& a, hipStream_t st)
             return RAFT_EHIP;
         attr[attr_dev].store(true, std::memory_order_relaxed);
     }
+    // the kernel's 24-bit multiplies and 32-bit per-image offsets need these
+    const long long hw = (long long)a.H * a.W;
+    if (hw >= (1LL << 24) || hw * a.ldx >= (1LL << 31) || (long long)a.Cout * 9 * a.Cin >= (1LL << 31) ||
+        (a.x2 && (long long)(a.H >> 1) * (a.W >> 1) * a.ldx2 >= (1LL << 31)))
+        return RAFT_EINVAL;
     ConvPArgs pa;
     pa.c = a;
     pa.tiles_x = (a.W + 31) / 32;
