@@ -167,7 +167,7 @@ def radon_bench(args, torch, dv, img, ang, out):
     bytes_alg = samples * 4 + float(S) * N * N * 4 + float(S) * A * N * 4
     ks = k_ms * 1e-3
     r = dict(config=f"Radon-RT {S} x 2048^2 -> {A} angles x {N} bins", slices_per_s=S * args.steps / dt,
-             ms_per_step=1e3 * dt / args.steps, kernel="k_project<0>", kernel_ms=k_ms,
+             ms_per_step=1e3 * dt / args.steps, kernel="k_project_lds<0>", kernel_ms=k_ms,
              samples_per_s=samples / ks,
              roofline=dict(bound="hbm", achieved=bytes_alg / ks / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                            frac=bytes_alg / ks / HBM_PEAK, bytes_per_launch=bytes_alg, samples_per_launch=samples,
