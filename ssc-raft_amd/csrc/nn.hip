@@ -166,6 +166,25 @@ struct ConvArgs {
 // (gy, gx) of image n, inside the image.  c and split are multiples of 8 and
 // split a multiple of 32, so the branch is uniform over a 32-channel chunk.
 // The pooled and upsampled values equal k_pool2 / k_up2 outputs bit for bit.
+// 2x2 average of x (at 2H x 2W) for input channels c .. c+7 (c < split) at
+// pixel (gy, gx), as k_pool2 computes it; every load is consumed here
+__device__ __forceinline__ uint4 conv_pooled_src(const ConvArgs& a, int n, int gy, int gx, int c)
+{
+    const int W2 = 2 * a.W;
+    const size_t base = ((size_t)n * 2 * a.H + 2 * gy) * W2 + 2 * gx;
+    const size_t src[4] = {base, base + 1, base + W2, base + W2 + 1};
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        unpack8(ld8(a.x + src[k] * a.ldx + a.cx0 + c), f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] += f[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] *= 0.25f;
+    return pack8(acc);
+}
+
 __device__ __forceinline__ uint4 conv_src(const ConvArgs& a, int n, int gy, int gx, int c)
 {
     if (c >= a.split) {
@@ -173,21 +192,7 @@ __device__ __forceinline__ uint4 conv_src(const ConvArgs& a, int n, int gy, int 
         const size_t p = ((size_t)n * Hl + (gy >> 1)) * Wl + (gx >> 1);
         return *reinterpret_cast<const uint4*>(a.x2 + p * a.ldx2 + a.cx2 + (c - a.split));
     }
-    if (a.pool) {
-        const int W2 = 2 * a.W;
-        const size_t base = ((size_t)n * 2 * a.H + 2 * gy) * W2 + 2 * gx;
-        const size_t src[4] = {base, base + 1, base + W2, base + W2 + 1};
-        float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, f[8];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            unpack8(ld8(a.x + src[k] * a.ldx + a.cx0 + c), f);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) acc[j] += f[j];
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[j] *= 0.25f;
-        return pack8(acc);
-    }
+    if (a.pool) return conv_pooled_src(a, n, gy, gx, c);
     const size_t p = ((size_t)n * a.H + gy) * a.W + gx;
     return *reinterpret_cast<const uint4*>(a.x + p * a.ldx + a.cx0 + c);
 }
@@ -197,21 +202,6 @@ __device__ __forceinline__ uint4 conv_src(const ConvArgs& a, int n, int gy, int 
 __device__ __forceinline__ int div34(int x) { return (int)(__umul24((unsigned)x, 1928u) >> 16); }
 __device__ __forceinline__ int div36(int x) { return (int)(__umul24((unsigned)x, 1821u) >> 16); }
 __device__ __forceinline__ int div9(int x) { return (int)(__umul24((unsigned)x, 7282u) >> 16); }
-
-// conv_src for the persistent form: the same values, with 32-bit offsets from
-// per-image bases (the host admits only H W < 2^24 and H W ld < 2^31) and
-// 24-bit multiplies; the pooled input keeps conv_src
-__device__ __forceinline__ uint4 conv_src_fast(const ConvArgs& a, const u16* xb, const u16* x2b, int n, int gy, int gx,
-                                               int c)
-{
-    if (c >= a.split) {
-        const unsigned pix = __umul24((unsigned)(gy >> 1), (unsigned)(a.W >> 1)) + (unsigned)(gx >> 1);
-        return *reinterpret_cast<const uint4*>(x2b + __umul24(pix, (unsigned)a.ldx2) + (c - a.split));
-    }
-    if (a.pool) return conv_src(a, n, gy, gx, c);
-    const unsigned pix = __umul24((unsigned)gy, (unsigned)a.W) + (unsigned)gx;
-    return *reinterpret_cast<const uint4*>(xb + __umul24(pix, (unsigned)a.ldx) + c);
-}
 
 template <int WR, int WC>
 constexpr int conv_lds_bytes()
@@ -363,6 +353,15 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             bias[c][i] = a.b ? a.b[co] : 0.0f;
         }
 
+    // head: the 1x1 weights and bias, read once (uniform, kept in scalar registers)
+    float hwv[HEAD ? 33 : 1];
+    if constexpr (HEAD) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j)
+            hwv[j] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a.hw[j])));
+        hwv[32] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, a.hb[0])));
+    }
+
     // first-layer mode: lane (l32, hi) supplies the weights of first-layer
     // channel l32 for taps 8hi .. 8hi+7 (zero past tap 8) and receives
     // channels 8g + 4hi + i of a halo pixel
@@ -408,39 +407,66 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
 
     u32x4 hreg[FIRST ? 1 : K::HU], wreg[K::WU];
     float preg[FIRST ? K::PU : 1];
+    unsigned vmask = 0; // bit k: unit k of the prefetched item lies inside the image
+    // Every unit's load is issued unconditionally from a clamped in-image address
+    // and out-of-image units are zeroed when stored to LDS, so the loads of an item
+    // are all in flight together (a load under a per-lane branch is waited for
+    // before the next one is issued).  The source of a 32-channel chunk (x, the
+    // upsampled x2, or the 2x2 average of x) is uniform over the item.
     // weight unit u: row x = u / 4 = co * 9 + tap of [Cout][9][Cin], 8 channels at 8 (u % 4)
     auto load_item = [&](const Cursor& c, bool weights) __attribute__((always_inline)) {
         const int ch = c.ch, tx0 = c.tx * 32, ty0 = c.ty * K::TH, n = c.n;
+        vmask = 0;
         if constexpr (FIRST) {
-            // image patch rows ty0-2 .. ty0+TH+1, columns tx0-2 .. tx0+33, bf16-rounded
+            // image patch rows ty0-2 .. ty0+TH+1, columns tx0-2 .. tx0+33
             const float* ib = a.img0 + (size_t)n * a.H * a.W;
 #pragma unroll
             for (int k = 0; k < K::PU; ++k) {
-                const int u = tid + 256 * k;
-                float v = 0.0f;
-                if (u < K::PATCH) {
-                    const int py = div36(u);
-                    const int gy = ty0 - 2 + py, gx = tx0 - 2 + (u - py * K::PW);
-                    if ((unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W)
-                        v = bf16_float(bf16_bits(ib[__umul24((unsigned)gy, (unsigned)a.W) + (unsigned)gx] * a.scale0));
-                }
-                preg[k] = v;
+                const int u = min(tid + 256 * k, K::PATCH - 1);
+                const int py = div36(u);
+                const int gy = ty0 - 2 + py, gx = tx0 - 2 + (u - py * K::PW);
+                const bool valid = tid + 256 * k < K::PATCH && (unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W;
+                const int cy = min(max(gy, 0), a.H - 1), cx = min(max(gx, 0), a.W - 1);
+                preg[k] = ib[__umul24((unsigned)cy, (unsigned)a.W) + (unsigned)cx];
+                vmask |= (unsigned)valid << k;
             }
         } else {
             const u16* xb = a.x + (size_t)n * a.H * a.W * a.ldx + a.cx0;
-            const u16* x2b = a.x2 ? a.x2 + (size_t)n * (a.H >> 1) * (a.W >> 1) * a.ldx2 + a.cx2 : nullptr;
+            const int c0 = 32 * ch;
+            const bool up = __builtin_amdgcn_readfirstlane(c0 >= a.split ? 1 : 0) != 0;
+            if (!up && a.pool) {
 #pragma unroll
-            for (int k = 0; k < K::HU; ++k) {
-                const int u = tid + 256 * k;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (u < K::HALO_UNITS) {
+                for (int k = 0; k < K::HU; ++k) {
+                    const int u = tid + 256 * k;
+                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                    if (u < K::HALO_UNITS) {
+                        const int pix = u >> 2, q = u & 3;
+                        const int hy = div34(pix);
+                        const int gy = ty0 + hy - 1, gx = tx0 + (pix - hy * K::HW) - 1;
+                        if ((unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W)
+                            v = conv_pooled_src(a, n, gy, gx, c0 + 8 * q);
+                    }
+                    hreg[k] = as_u32x4(v);
+                }
+                vmask = ~0u;
+            } else {
+                // x at H x W, or x2 at H/2 x W/2 read with nearest 2x upsampling
+                const int sh = up ? 1 : 0;
+                const unsigned ws = (unsigned)(a.W >> sh), ld = (unsigned)(up ? a.ldx2 : a.ldx);
+                const u16* base = up ? a.x2 + (size_t)n * (a.H >> 1) * (a.W >> 1) * a.ldx2 + a.cx2 + (c0 - a.split)
+                                     : xb + c0;
+#pragma unroll
+                for (int k = 0; k < K::HU; ++k) {
+                    const int u = tid + 256 * k;
                     const int pix = u >> 2, q = u & 3;
                     const int hy = div34(pix);
                     const int gy = ty0 + hy - 1, gx = tx0 + (pix - hy * K::HW) - 1;
-                    if ((unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W)
-                        v = conv_src_fast(a, xb, x2b, n, gy, gx, 32 * ch + 8 * q);
+                    const bool valid = (unsigned)gy < (unsigned)a.H && (unsigned)gx < (unsigned)a.W;
+                    const int cy = min(max(gy, 0), a.H - 1), cx = min(max(gx, 0), a.W - 1);
+                    const unsigned sp = __umul24((unsigned)(cy >> sh), ws) + (unsigned)(cx >> sh);
+                    hreg[k] = *reinterpret_cast<const u32x4*>(base + __umul24(sp, ld) + 8 * q);
+                    vmask |= (unsigned)valid << k;
                 }
-                hreg[k] = as_u32x4(v);
             }
         }
         if (weights) {
@@ -454,18 +480,35 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             }
         }
     };
+    // Every prefetch register is waited for here, also in lanes whose unit lies
+    // past the end: a load left pending on one path makes the compiler wait for
+    // all outstanding memory operations (the tile's output stores included) at
+    // the next write of that register.
     auto store_item = [&](bool weights) __attribute__((always_inline)) {
+        if constexpr (FIRST) {
+#pragma unroll
+            for (int k = 0; k < K::PU; ++k) asm volatile("" ::"v"(preg[k]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < K::HU; ++k) asm volatile("" ::"v"(hreg[k]));
+        }
+#pragma unroll
+        for (int k = 0; k < K::WU; ++k) asm volatile("" ::"v"(wreg[k]));
         if constexpr (FIRST) {
 #pragma unroll
             for (int k = 0; k < K::PU; ++k) {
                 const int u = tid + 256 * k;
-                if (u < K::PATCH) s_patch[u] = preg[k];
+                // bf16-rounded scaled image, zero outside
+                if (u < K::PATCH)
+                    s_patch[u] = (vmask >> k) & 1u ? bf16_float(bf16_bits(preg[k] * a.scale0)) : 0.0f;
             }
         } else {
 #pragma unroll
             for (int k = 0; k < K::HU; ++k) {
                 const int u = tid + 256 * k;
-                if (u < K::HALO_UNITS) *reinterpret_cast<u32x4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = hreg[k];
+                const u32x4 zero = {0u, 0u, 0u, 0u};
+                if (u < K::HALO_UNITS)
+                    *reinterpret_cast<u32x4*>(s_in + (u >> 2) * CONV_PS + 8 * (u & 3)) = (vmask >> k) & 1u ? hreg[k] : zero;
             }
         }
         if (weights) {
@@ -488,15 +531,137 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
 #pragma unroll
             for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
 
+    float resv[WR]; // head: the tile's residual values
+    // bias, activation and stores of a finished tile (then its accumulators are cleared)
+    auto finish_tile = [&](const Cursor& t) __attribute__((always_inline)) {
+        const int tx0 = t.tx * 32, ty0 = t.ty * K::TH;
+        const size_t img = (size_t)t.n * a.H * a.W;
+        if constexpr (HEAD && WC == 1) {
+            // lane (l32, hi) holds channels 8g + 4hi + i of pixel column l32;
+            // the hi = 0 lane gathers its partner's 16 and sums all 32 in order
+#pragma unroll
+            for (int r = 0; r < WR; ++r) {
+                if (ty0 + wave * WR + r >= a.H) break; // wave-uniform
+                float hv[16], pv[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    float v = acc[r][0][i] + bias[0][i];
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    hv[i] = bf16_float(bf16_bits(v));
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) pv[i] = __shfl_xor(hv[i], 32);
+                if (hi == 0 && tx0 + l32 < a.W) {
+                    float z = hwv[32];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) z = fmaf(hwv[8 * g + i], hv[4 * g + i], z);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) z = fmaf(hwv[8 * g + 4 + i], pv[4 * g + i], z);
+                    }
+                    const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
+                    float pr = resv[r] + z;
+                    if (a.clamp) pr = fmaxf(pr, a.floor_value);
+                    a.pred[p] = pr;
+                }
+            }
+        } else if (tx0 + l32 < a.W) {
+#pragma unroll
+            for (int r = 0; r < WR; ++r) {
+                if (ty0 + wave * WR + r < a.H) {
+                    const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
+                    u16* yp = a.y + p * a.ldy + a.cy0;
+#pragma unroll
+                    for (int c = 0; c < WC; ++c)
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            const int co = co0 + 32 * c + 8 * g + 4 * hi;
+                            float o[4];
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                float v = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                                if (a.relu) v = fmaxf(v, 0.0f);
+                                o[i] = v;
+                            }
+                            uint2 pk;
+                            pk.x = bf16_pack2(o[0], o[1]);
+                            pk.y = bf16_pack2(o[2], o[3]);
+                            *reinterpret_cast<uint2*>(yp + co) = pk;
+                        }
+                }
+            }
+        }
+        if constexpr (POOL && !HEAD) {
+            // 2x2 average of the stored (bf16) outputs of row pairs (2k, 2k+1)
+            // and column pairs (l32, l32 + 1), summed in k_pool2's order
+            // (+0, x00, x01, x10, x11), written by the even-column lanes
+#pragma unroll
+            for (int r = 1; r < WR; r += 2) {
+                const int row = ty0 + wave * WR + r - 1;
+                if (row >= a.H) break; // wave-uniform; H even, so row + 1 < H
+                const bool store = (l32 & 1) == 0 && tx0 + l32 < a.W;
+                const size_t pp = ((size_t)t.n * (a.H >> 1) + (row >> 1)) * (a.W >> 1) + ((tx0 + l32) >> 1);
+                u16* qp = a.ypool + pp * a.ldp + a.cp0;
+#pragma unroll
+                for (int c = 0; c < WC; ++c)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        float q[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            float v0 = acc[r - 1][c][4 * g + i] + bias[c][4 * g + i];
+                            float v1 = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                            if (a.relu) {
+                                v0 = fmaxf(v0, 0.0f);
+                                v1 = fmaxf(v1, 0.0f);
+                            }
+                            v0 = bf16_float(bf16_bits(v0));
+                            v1 = bf16_float(bf16_bits(v1));
+                            const float n0 = odd_neighbour(v0), n1 = odd_neighbour(v1);
+                            float sum = 0.0f;
+                            sum += v0;
+                            sum += n0;
+                            sum += v1;
+                            sum += n1;
+                            q[i] = sum * 0.25f;
+                        }
+                        if (store) {
+                            uint2 pk;
+                            pk.x = bf16_pack2(q[0], q[1]);
+                            pk.y = bf16_pack2(q[2], q[3]);
+                            *reinterpret_cast<uint2*>(qp + co0 + 32 * c + 8 * g + 4 * hi) = pk;
+                        }
+                    }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < WR; ++r)
+#pragma unroll
+            for (int c = 0; c < WC; ++c)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
+    };
+
     if (items > 0) {
         load_item(lc, true);
         advance(lc);
     }
+    // A tile's outputs are written at the start of the next item, before that
+    // item's prefetch loads: global stores share the loads' completion counter,
+    // so stores issued after the loads would be waited for at the next LDS
+    // store; issued first, they complete while the MFMAs run.
+    Cursor pc = cc; // tile whose outputs are pending
+    bool pending = false;
     for (int it = 0; it < items; ++it) {
         const int ch = cc.ch;
         __syncthreads();
         store_item(it == 0 || nch > 1);
         __syncthreads();
+        if (pending) {
+            finish_tile(pc);
+            pending = false;
+        }
         if (it + 1 < items) {
             load_item(lc, nch > 1);
             advance(lc);
@@ -540,7 +705,6 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             __syncthreads();
         }
         // head: the tile's residual values are fetched before its MFMAs
-        float resv[WR];
         if constexpr (HEAD && WC == 1) {
             if (ch == nch - 1) {
                 const int tx0 = cc.tx * 32, ty0 = cc.ty * K::TH;
@@ -576,116 +740,12 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
             }
         }
         if (ch == nch - 1) {
-            const int tx0 = cc.tx * 32, ty0 = cc.ty * K::TH;
-            const size_t img = (size_t)cc.n * a.H * a.W;
-            if constexpr (HEAD && WC == 1) {
-                // lane (l32, hi) holds channels 8g + 4hi + i of pixel column l32;
-                // the hi = 0 lane gathers its partner's 16 and sums all 32 in order
-#pragma unroll
-                for (int r = 0; r < WR; ++r) {
-                    if (ty0 + wave * WR + r >= a.H) break; // wave-uniform
-                    float hv[16], pv[16];
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        float v = acc[r][0][i] + bias[0][i];
-                        if (a.relu) v = fmaxf(v, 0.0f);
-                        hv[i] = bf16_float(bf16_bits(v));
-                    }
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) pv[i] = __shfl_xor(hv[i], 32);
-                    if (hi == 0 && tx0 + l32 < a.W) {
-                        float z = a.hb[0];
-#pragma unroll
-                        for (int g = 0; g < 4; ++g) {
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) z = fmaf(a.hw[8 * g + i], hv[4 * g + i], z);
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) z = fmaf(a.hw[8 * g + 4 + i], pv[4 * g + i], z);
-                        }
-                        const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
-                        float pr = resv[r] + z;
-                        if (a.clamp) pr = fmaxf(pr, a.floor_value);
-                        a.pred[p] = pr;
-                    }
-                }
-            } else if (tx0 + l32 < a.W) {
-#pragma unroll
-                for (int r = 0; r < WR; ++r) {
-                    if (ty0 + wave * WR + r < a.H) {
-                        const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
-                        u16* yp = a.y + p * a.ldy + a.cy0;
-#pragma unroll
-                        for (int c = 0; c < WC; ++c)
-#pragma unroll
-                            for (int g = 0; g < 4; ++g) {
-                                const int co = co0 + 32 * c + 8 * g + 4 * hi;
-                                float o[4];
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    float v = acc[r][c][4 * g + i] + bias[c][4 * g + i];
-                                    if (a.relu) v = fmaxf(v, 0.0f);
-                                    o[i] = v;
-                                }
-                                uint2 pk;
-                                pk.x = bf16_pack2(o[0], o[1]);
-                                pk.y = bf16_pack2(o[2], o[3]);
-                                *reinterpret_cast<uint2*>(yp + co) = pk;
-                            }
-                    }
-                }
-            }
-            if constexpr (POOL && !HEAD) {
-                // 2x2 average of the stored (bf16) outputs of row pairs (2k, 2k+1)
-                // and column pairs (l32, l32 + 1), summed in k_pool2's order
-                // (+0, x00, x01, x10, x11), written by the even-column lanes
-#pragma unroll
-                for (int r = 1; r < WR; r += 2) {
-                    const int row = ty0 + wave * WR + r - 1;
-                    if (row >= a.H) break; // wave-uniform; H even, so row + 1 < H
-                    const bool store = (l32 & 1) == 0 && tx0 + l32 < a.W;
-                    const size_t pp = ((size_t)cc.n * (a.H >> 1) + (row >> 1)) * (a.W >> 1) + ((tx0 + l32) >> 1);
-                    u16* qp = a.ypool + pp * a.ldp + a.cp0;
-#pragma unroll
-                    for (int c = 0; c < WC; ++c)
-#pragma unroll
-                        for (int g = 0; g < 4; ++g) {
-                            float q[4];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                float v0 = acc[r - 1][c][4 * g + i] + bias[c][4 * g + i];
-                                float v1 = acc[r][c][4 * g + i] + bias[c][4 * g + i];
-                                if (a.relu) {
-                                    v0 = fmaxf(v0, 0.0f);
-                                    v1 = fmaxf(v1, 0.0f);
-                                }
-                                v0 = bf16_float(bf16_bits(v0));
-                                v1 = bf16_float(bf16_bits(v1));
-                                const float n0 = odd_neighbour(v0), n1 = odd_neighbour(v1);
-                                float sum = 0.0f;
-                                sum += v0;
-                                sum += n0;
-                                sum += v1;
-                                sum += n1;
-                                q[i] = sum * 0.25f;
-                            }
-                            if (store) {
-                                uint2 pk;
-                                pk.x = bf16_pack2(q[0], q[1]);
-                                pk.y = bf16_pack2(q[2], q[3]);
-                                *reinterpret_cast<uint2*>(qp + co0 + 32 * c + 8 * g + 4 * hi) = pk;
-                            }
-                        }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < WR; ++r)
-#pragma unroll
-                for (int c = 0; c < WC; ++c)
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) acc[r][c][i] = 0.f;
+            pc = cc;
+            pending = true;
         }
         advance(cc);
     }
+    if (pending) finish_tile(pc);
 }
 
 // wt[ci][8 - tap][co] = w[co][tap][ci]: the backward-data weights
