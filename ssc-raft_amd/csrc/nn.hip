@@ -313,7 +313,8 @@ struct ConvP {
     static constexpr int W_UNITS = 9 * CO_T * 4;
     static constexpr int HU = (HALO_UNITS + 255) / 256;
     static constexpr int WU = (W_UNITS + 255) / 256;
-    static constexpr int LDS = (HH * HW + 9 * CO_T) * CONV_PS * 2;
+    // halo, weights, then the tile's CO_T biases (FP32)
+    static constexpr int LDS = (HH * HW + 9 * CO_T) * CONV_PS * 2 + CO_T * 4;
     // first-layer mode: the FP32 image patch under the halo's 3x3 taps
     static constexpr int PW = HW + 2, PH = HH + 2, PATCH = PH * PW, PU = (PATCH + 255) / 256;
     static constexpr int HALO_GROUPS = (HH * HW + 31) / 32;
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
     extern __shared__ uint4 convp_lds[];
     u16* s_in = reinterpret_cast<u16*>(convp_lds);
     u16* s_w = s_in + K::HH * K::HW * CONV_PS;
-    float* s_patch = reinterpret_cast<float*>(s_w + 9 * K::CO_T * CONV_PS); // first-layer mode
+    float* s_bias = reinterpret_cast<float*>(s_w + 9 * K::CO_T * CONV_PS);
+    float* s_patch = s_bias + K::CO_T; // first-layer mode
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
     const int co0 = blockIdx.y * K::CO_T;
@@ -344,14 +346,13 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
     const int t1 = min(t0 + pa.tiles_per_block, pa.total_tiles);
     const int items = (t1 - t0) * nch;
 
-    float bias[WC][16];
-#pragma unroll
-    for (int c = 0; c < WC; ++c)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int co = co0 + 32 * c + 8 * (i / 4) + 4 * hi + (i % 4);
-            bias[c][i] = a.b ? a.b[co] : 0.0f;
-        }
+    // the tile's biases live in LDS (read by the epilogue only), not in registers;
+    // the first barrier of the item loop makes them visible
+    for (int i = tid; i < K::CO_T; i += 256) s_bias[i] = a.b ? a.b[co0 + i] : 0.0f;
+    // bias of accumulator element e (channel 8 (e / 4) + 4 hi + e % 4) of output-channel block c
+    auto bias_of = [&](int c, int e) __attribute__((always_inline)) {
+        return s_bias[32 * c + 8 * (e >> 2) + 4 * hi + (e & 3)];
+    };
 
     // head: the 1x1 weights and bias, read once (uniform, kept in scalar registers)
     float hwv[HEAD ? 33 : 1];
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
                 float hv[16], pv[16];
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    float v = acc[r][0][i] + bias[0][i];
+                    float v = acc[r][0][i] + bias_of(0, i);
                     if (a.relu) v = fmaxf(v, 0.0f);
                     hv[i] = bf16_float(bf16_bits(v));
                 }
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
                             float o[4];
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
-                                float v = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                                float v = acc[r][c][4 * g + i] + bias_of(c, 4 * g + i);
                                 if (a.relu) v = fmaxf(v, 0.0f);
                                 o[i] = v;
                             }
@@ -610,8 +611,8 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPA
                         float q[4];
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
-                            float v0 = acc[r - 1][c][4 * g + i] + bias[c][4 * g + i];
-                            float v1 = acc[r][c][4 * g + i] + bias[c][4 * g + i];
+                            float v0 = acc[r - 1][c][4 * g + i] + bias_of(c, 4 * g + i);
+                            float v1 = acc[r][c][4 * g + i] + bias_of(c, 4 * g + i);
                             if (a.relu) {
                                 v0 = fmaxf(v0, 0.0f);
                                 v1 = fmaxf(v1, 0.0f);
