@@ -80,7 +80,7 @@ def torch_conv(x, wt, b, relu):
     return y.permute(0, 2, 3, 1).numpy()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 @pytest.mark.parametrize("n,h,w,ci,co,ldx,cx0,ldy,cy0", [
     (2, 13, 37, 64, 128, 96, 32, 132, 4),   # ragged tiles, channel offsets on both sides
     (1, 32, 64, 32, 64, 32, 0, 64, 0),
@@ -89,7 +89,7 @@ def torch_conv(x, wt, b, relu):
 def test_conv3x3_forward(gpu, variant, n, h, w, ci, co, ldx, cx0, ldy, cy0):
     from sscRaft.nn import nn_call
 
-    if variant in (1, 4, 7, 8) and co % 64:
+    if variant in (1, 4, 7, 8, 11, 12) and co % 64:
         pytest.skip("variant needs Cout % 64 == 0")
     rng = np.random.default_rng(variant + 10 * h)
     x, wt, b = conv_case(rng, n, h, w, ci, co)
@@ -108,7 +108,7 @@ def test_conv3x3_forward(gpu, variant, n, h, w, ci, co, ldx, cx0, ldy, cy0):
     assert np.all(outside == 1.0), "wrote outside its channel window"
 
 
-@pytest.mark.parametrize("variant", [2, 5, 6, 9, 10])
+@pytest.mark.parametrize("variant", [2, 5, 6, 7, 9, 10, 11, 13])
 def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
     """raft_nn_conv3x3_ex reading a 2x2-pooled input, or a skip concatenation
     [x | up2(x2)] assembled in its input load, gives bit for bit what
@@ -132,18 +132,20 @@ def test_conv3x3_ex_equals_separate_pool_and_upsample(gpu, variant):
     torch.cuda.synchronize()
     assert torch.equal(y1, y2), "pooled input"
     # skip concatenation: 32 skip channels + 64 upsampled channels -> 96
+    # (64 output channels for the forms with 64-channel tiles)
+    co2 = 64 if variant in (7, 11) else 32
     skip = val(bits(rng.standard_normal((n, h, w, 32)).astype(np.float32)))
     low = val(bits(rng.standard_normal((n, h // 2, w // 2, 64)).astype(np.float32)))
-    _, wt2, b2 = conv_case(rng, n, h, w, 96, 32)
+    _, wt2, b2 = conv_case(rng, n, h, w, 96, co2)
     cat = np.concatenate([skip, low.repeat(2, axis=1).repeat(2, axis=2)], axis=-1)
     sd, ld, cd = dev_bits(skip), dev_bits(low), dev_bits(cat)
     wd2, bd2 = dev_bits(wt2), torch.from_numpy(b2).cuda()
-    z1 = torch.zeros(n * h * w * 32, dtype=torch.int16, device="cuda")
-    z2 = torch.ones(n * h * w * 32, dtype=torch.int16, device="cuda")
-    nn_call("raft_nn_conv3x3", cd.data_ptr(), 96, 0, wd2.data_ptr(), bd2.data_ptr(), z1.data_ptr(), 32, 0, n, h, w,
-            96, 32, 1, variant, stream())
+    z1 = torch.zeros(n * h * w * co2, dtype=torch.int16, device="cuda")
+    z2 = torch.ones(n * h * w * co2, dtype=torch.int16, device="cuda")
+    nn_call("raft_nn_conv3x3", cd.data_ptr(), 96, 0, wd2.data_ptr(), bd2.data_ptr(), z1.data_ptr(), co2, 0, n, h, w,
+            96, co2, 1, variant, stream())
     nn_call("raft_nn_conv3x3_ex", sd.data_ptr(), 32, 0, 0, ld.data_ptr(), 64, 0, 32, wd2.data_ptr(), bd2.data_ptr(),
-            z2.data_ptr(), 32, 0, n, h, w, 96, 32, 1, variant, None, 0, 0, stream())
+            z2.data_ptr(), co2, 0, n, h, w, 96, co2, 1, variant, None, 0, 0, stream())
     torch.cuda.synchronize()
     assert torch.equal(z1, z2), "skip concatenation"
 
