@@ -329,7 +329,7 @@ struct ConvPArgs {
 // the 16x64 tile (WR = 4, WC = 2) holds one block per CU in LDS (95 KB) anyway,
 // so it may use a whole SIMD's registers (accumulators in AGPRs, no spills)
 template <int WR, int WC, bool UNROLL_DX, bool HEAD = false, bool FIRST = false, bool POOL = false>
-__global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : 2) void k_conv3x3p(ConvPArgs pa)
+__global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : (WR * WC <= 2 ? 3 : 2)) void k_conv3x3p(ConvPArgs pa)
 {
     using K = ConvP<WR, WC>;
     const ConvArgs& a = pa.c;
