@@ -573,23 +573,67 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : (WR * WC <= 2 ? 3 : 2)) v
                 if (ty0 + wave * WR + r < a.H) {
                     const size_t p = img + (size_t)(ty0 + wave * WR + r) * a.W + tx0 + l32;
                     u16* yp = a.y + p * a.ldy + a.cy0;
+                    if constexpr (WR * WC > 2) {
+                        // 16-byte stores when the output rows allow them (uniform; the
+                        // 32x32-tile forms keep 8-byte stores and their register budget
+                        // for three blocks per CU)
+                        const bool st16 =
+                            ((reinterpret_cast<size_t>(a.y) & 15) | (size_t)(a.ldy & 7) | (size_t)(a.cy0 & 7)) == 0;
 #pragma unroll
-                    for (int c = 0; c < WC; ++c)
+                        for (int c = 0; c < WC; ++c) {
+                            uint2 pk[4]; // channels 8 g + 4 hi + (0..3), packed
 #pragma unroll
-                        for (int g = 0; g < 4; ++g) {
-                            const int co = co0 + 32 * c + 8 * g + 4 * hi;
-                            float o[4];
+                            for (int g = 0; g < 4; ++g) {
+                                float o[4];
 #pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                float v = acc[r][c][4 * g + i] + bias_of(c, 4 * g + i);
-                                if (a.relu) v = fmaxf(v, 0.0f);
-                                o[i] = v;
+                                for (int i = 0; i < 4; ++i) {
+                                    float v = acc[r][c][4 * g + i] + bias_of(c, 4 * g + i);
+                                    if (a.relu) v = fmaxf(v, 0.0f);
+                                    o[i] = v;
+                                }
+                                pk[g].x = bf16_pack2(o[0], o[1]);
+                                pk[g].y = bf16_pack2(o[2], o[3]);
                             }
-                            uint2 pk;
-                            pk.x = bf16_pack2(o[0], o[1]);
-                            pk.y = bf16_pack2(o[2], o[3]);
-                            *reinterpret_cast<uint2*>(yp + co) = pk;
+                            if (st16) {
+                                // the lane pair (l32, l32 + 32) trades halves so that lane hi
+                                // holds channels 8 (2k + hi) .. +7 of group pair k: one
+                                // 16-byte store per pair instead of two 8-byte ones
+#pragma unroll
+                                for (int k = 0; k < 2; ++k) {
+                                    const uint2 keep = hi ? pk[2 * k + 1] : pk[2 * k];
+                                    const uint2 give = hi ? pk[2 * k] : pk[2 * k + 1];
+                                    const unsigned gx = (unsigned)__shfl_xor((int)give.x, 32);
+                                    const unsigned gy = (unsigned)__shfl_xor((int)give.y, 32);
+                                    const uint4 v =
+                                        hi ? make_uint4(gx, gy, keep.x, keep.y) : make_uint4(keep.x, keep.y, gx, gy);
+                                    *reinterpret_cast<uint4*>(yp + co0 + 32 * c + 8 * (2 * k + hi)) = v;
+                                }
+                            } else {
+#pragma unroll
+                                for (int g = 0; g < 4; ++g)
+                                    *reinterpret_cast<uint2*>(yp + co0 + 32 * c + 8 * g + 4 * hi) = pk[g];
+                            }
                         }
+                    } else {
+                        // 32x32 tiles: each 8-byte group stored as soon as it is packed
+                        // (fewer live registers)
+#pragma unroll
+                        for (int c = 0; c < WC; ++c)
+#pragma unroll
+                            for (int g = 0; g < 4; ++g) {
+                                float o[4];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    float v = acc[r][c][4 * g + i] + bias_of(c, 4 * g + i);
+                                    if (a.relu) v = fmaxf(v, 0.0f);
+                                    o[i] = v;
+                                }
+                                uint2 pk;
+                                pk.x = bf16_pack2(o[0], o[1]);
+                                pk.y = bf16_pack2(o[2], o[3]);
+                                *reinterpret_cast<uint2*>(yp + co0 + 32 * c + 8 * g + 4 * hi) = pk;
+                            }
+                    }
                 }
             }
         }
