@@ -411,9 +411,11 @@ __global__ __launch_bounds__(256, (WR * WC >= 8) ? 1 : (WR * WC <= 2 ? 3 : 2)) v
     unsigned vmask = 0; // bit k: unit k of the prefetched item lies inside the image
     // Every unit's load is issued unconditionally from a clamped in-image address
     // and out-of-image units are zeroed when stored to LDS, so the loads of an item
-    // are all in flight together (a load under a per-lane branch is waited for
-    // before the next one is issued).  The source of a 32-channel chunk (x, the
-    // upsampled x2, or the 2x2 average of x) is uniform over the item.
+    // are all in flight together (with the loads under per-lane branches, the
+    // compiler waited for each one before reusing registers on the other side of
+    // the branch).  The source of a 32-channel chunk (x, the upsampled x2, or the
+    // 2x2 average of x) is uniform over the item; the 2x2 average keeps per-lane
+    // branches and consumes its four loads at once.
     // weight unit u: row x = u / 4 = co * 9 + tap of [Cout][9][Cin], 8 channels at 8 (u % 4)
     auto load_item = [&](const Cursor& c, bool weights) __attribute__((always_inline)) {
         const int ch = c.ch, tx0 = c.tx * 32, ty0 = c.ty * K::TH, n = c.n;
