@@ -485,7 +485,8 @@ def unet_bench(args, torch):
     r = {}
     tuned = net.autotune(32, 512, 512)
     net.autotune(8, 512, 512, backward=True)
-    r["tuned_variants_b32"] = {f"{k[3]}->{k[4]}@{k[1]}": v[0] for k, v in tuned.items()}
+    r["tuned_variants_b32"] = {(f"{k[0]}@{k[2]}" if isinstance(k[0], str) else f"{k[3]}->{k[4]}@{k[1]}"): v[0]
+                               for k, v in tuned.items()}
     for B in (8, 32):
         x = torch.rand((B, 512, 512), device="cuda")
         out = torch.empty_like(x)

@@ -253,7 +253,9 @@ class UNet:
         """Pick, per convolution shape of this network at batch n and image h x w,
         the fastest tile variant of raft_nn_conv3x3 by timing each candidate
         (HIP events on the launch stream); the choice is kept in self.tuned and
-        used by every later call with that shape.  Returns {shape: (variant, us)}."""
+        used by every later call with that shape.  The fused first two layers and
+        the fused last layer + head are timed on their own (keys ("first2", n, h, w)
+        and ("head", n, h, w)).  Returns {key: (variant, us)}."""
         t = self.torch
         st = _stream(stream)
         cur = t.cuda.current_stream() if stream is None else stream
@@ -287,6 +289,38 @@ class UNet:
                     best = (v, us)
             self.tuned[(nb, hh, ww, ci, co)] = best[0]
             result[(nb, hh, ww, ci, co)] = best
+        if h % 2 == 0 and w % 2 == 0:
+            # the fused first two layers and the fused last layer + head, timed on
+            # their own (their variants are not the plain convolution's best)
+            c = self.base
+            img = t.zeros((n, h, w), dtype=t.float32, device=self.dev)
+            wt = t.zeros(c * 9 * c, dtype=t.int16, device=self.dev)
+            y = t.zeros(n * h * w * c, dtype=t.int16, device=self.dev)
+            yp = t.zeros(n * (h // 2) * (w // 2) * c, dtype=t.int16, device=self.dev)
+            hw_ = t.zeros(c + 1, dtype=t.float32, device=self.dev)
+            pred = t.zeros((n, h, w), dtype=t.float32, device=self.dev)
+            calls = {
+                "first2": lambda v: nn_call("raft_nn_conv_first2", img.data_ptr(), 1.0, wt.data_ptr(), c, None, 1,
+                                            wt.data_ptr(), None, y.data_ptr(), c, 0, n, h, w, 1, v, yp.data_ptr(), c,
+                                            0, st),
+                "head": lambda v: nn_call("raft_nn_conv3x3_head", y.data_ptr(), c, 0, wt.data_ptr(), None, n, h, w, c,
+                                          1, v, hw_.data_ptr(), hw_.data_ptr() + 4 * c, img.data_ptr(),
+                                          pred.data_ptr(), 0.0, 1, st)}
+            for kind, call in calls.items():
+                best = None
+                for v in self.HEAD_VARIANTS:
+                    call(v)
+                    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                    e0.record(cur)
+                    for _ in range(repeats):
+                        call(v)
+                    e1.record(cur)
+                    e1.synchronize()
+                    us = 1e3 * e0.elapsed_time(e1) / repeats
+                    if best is None or us < best[1]:
+                        best = (v, us)
+                self.tuned[(kind, n, h, w)] = best[0]
+                result[(kind, n, h, w)] = best
         return result
 
     def _wgrad(self, name, x, cx0, g, n, hw, st):
@@ -341,7 +375,7 @@ class UNet:
         L = [(h, w), (h // 2, w // 2), (h // 4, w // 4), (h // 8, w // 8)]
         w0, b0 = self.offsets["enc1a"][0], self.offsets["enc1a"][1]
         w1, b1 = self.offsets["enc1b"][0], self.offsets["enc1b"][1]
-        v = self.tuned.get((n, h, w, self.base, self.base), 0)
+        v = self.tuned.get(("first2", n, h, w), self.tuned.get((n, h, w, self.base, self.base), 0))
         nn_call("raft_nn_conv_first2", img.data_ptr(), 1.0, self.params_bf.data_ptr() + 2 * w0, 32,
                 self.params.data_ptr() + 4 * b0, 1, self.params_bf.data_ptr() + 2 * w1, self.params.data_ptr() + 4 * b1,
                 b["e1"].ptr(), b["e1"].ch, 0, n, h, w, 1, v if v in self.HEAD_VARIANTS else 0, b["p1"].ptr(),
@@ -360,7 +394,7 @@ class UNet:
         co, ci = self.offsets["dec1b"][2], self.offsets["dec1b"][3]
         w1, b1 = self.offsets["dec1b"][0], self.offsets["dec1b"][1]
         h0, hb = self.offsets["head"][0], self.offsets["head"][1]
-        v = self.tuned.get((n, h, w, ci, co), 0)
+        v = self.tuned.get(("head", n, h, w), self.tuned.get((n, h, w, ci, co), 0))
         nn_call("raft_nn_conv3x3_head", b["a7"].ptr(), b["a7"].ch, 0, self.params_bf.data_ptr() + 2 * w1,
                 self.params.data_ptr() + 4 * b1, n, h, w, ci, 1, v if v in self.HEAD_VARIANTS else 0,
                 self.params.data_ptr() + 4 * h0, self.params.data_ptr() + 4 * hb, img.data_ptr(), out.data_ptr(),
