@@ -7,9 +7,12 @@ sinogram (Poisson, I0 = 1e4) already resident in HBM.  One step = one
 raft_fbp_device call over the 64 slices (pad + FFT filter + backprojection).
 
 Also reported on the same JSON line:
-  roofline      backprojection kernel: gathered-sample bytes (4 B per sample,
-                FP32 rows) + staged-row and image bytes, over its average
-                launch time from HIP events on the launch stream;
+  roofline      backprojection kernel (k_backproject<float,0>): bound "valu",
+                2 FLOP per gathered sample over its average launch time (HIP
+                events on the launch stream) against the 157.3 TFLOP/s FP32
+                vector peak; beside it the effective gather bandwidth, the
+                measured HBM traffic and its fraction of 8 TB/s, and the
+                counter VALU issue rate (profiles/);
   cpu_baseline  the FP32 restatement in oracle/ timed on this host's cores on
                 a bounded sample (whole slices of the same workload);
   fp16_rows     the same FBP with FP16-stored filtered rows;
@@ -65,19 +68,46 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def read_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+def read_profile_value(name, key):
+    """A `key = value` figure from a committed rocprofv3 summary under
+    profiles/ (HBM bytes per launch, VALU issue rate), or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         vals = {}
         with open(path) as fh:
             for line in fh:
-                if "=" in line and not line.lstrip().startswith("#"):
+                if "=" in line and not line.lstrip().startswith(("#", "|")):
                     k, v = line.split("=", 1)
                     vals[k.strip()] = v.strip()
-        return float(vals["hbm_bytes_per_dispatch"])
+        return float(vals[key])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, or None."""
+    return read_profile_value(name, "hbm_bytes_per_dispatch")
+
+
+VALU_PEAK = 157.3e12  # FLOP/s, MI355X FP32 vector
+
+
+def valu_roofline(kernel, samples, launch_s, bytes_gathered, traffic=None, valu_issue=None):
+    """Roofline block of a gather-accumulate kernel (backprojection, projector).
+
+    These kernels do O(N^3) gathers on O(N^2) data served from LDS, so the
+    bound is VALU issue, not HBM: `achieved` is 2 FLOP per sample (coordinate
+    step + accumulate) over the measured launch time, against the FP32 vector
+    peak.  Beside it: the effective gather bandwidth (gathered bytes / time,
+    which may exceed HBM bandwidth because LDS serves the gathers), the HBM
+    fraction of the measured traffic, and the counter VALU issue rate (wave
+    instructions per clock per CU, 1.0 = the issue limit) from profiles/."""
+    ach = 2.0 * samples / launch_s
+    r = dict(bound="valu", achieved=ach / 1e12, peak=VALU_PEAK / 1e12, unit="TFLOP/s", frac=ach / VALU_PEAK,
+             traffic=traffic, kernel=kernel, launch_ms=launch_s * 1e3, samples_per_launch=samples,
+             samples_per_s=samples / launch_s, effective_gather_GBps=bytes_gathered / launch_s / 1e9,
+             hbm_frac=(traffic / launch_s / HBM_PEAK) if traffic else None, valu_issue_per_clk_per_cu=valu_issue)
+    return r
 
 
 def fbp_dic(N):
@@ -169,9 +199,9 @@ def radon_bench(args, torch, dv, img, ang, out):
     r = dict(config=f"Radon-RT {S} x 2048^2 -> {A} angles x {N} bins", slices_per_s=S * args.steps / dt,
              ms_per_step=1e3 * dt / args.steps, kernel="k_project_lds<0>", kernel_ms=k_ms,
              samples_per_s=samples / ks,
-             roofline=dict(bound="hbm", achieved=bytes_alg / ks / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                           frac=bytes_alg / ks / HBM_PEAK, bytes_per_launch=bytes_alg, samples_per_launch=samples,
-                           valu_frac=(2.0 * samples / ks) / 157.3e12))
+             roofline=valu_roofline("k_project_lds<0>", samples, ks, bytes_alg,
+                                    valu_issue=read_profile_value("backproject_projector_counters.txt",
+                                                                  "valu_issue.k_project_lds<0>")))
     log(f"Radon: {r['slices_per_s']:.1f} slices/s, projector {k_ms:.3f} ms/launch")
     return r
 
@@ -251,12 +281,11 @@ def fbp_bench(args, torch, dv, ph):
     samples = float(S) * A * N * N
     bytes_alg = samples * 4 + float(S) * A * width * 4 + float(S) * N * N * 4
     bp_s = res["f32"]["bp_ms"] * 1e-3
-    achieved = bytes_alg / bp_s
-    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=read_traffic("fbp_backproject_2048_traffic.txt"),
-                kernel="k_backproject<float,0>", bytes_per_launch=bytes_alg, samples_per_launch=samples,
-                launch_ms=res["f32"]["bp_ms"], valu_frac=(2.0 * samples / bp_s) / 157.3e12,
-                staged_row_width=width, fft_path=bool(fft_path))
+    roof = valu_roofline("k_backproject<float,0>", samples, bp_s, bytes_alg,
+                         traffic=read_traffic("fbp_backproject_2048_traffic.txt"),
+                         valu_issue=read_profile_value("backproject_projector_counters.txt",
+                                                       "valu_issue.k_backproject<float,0>"))
+    roof.update(gathered_bytes_per_launch=bytes_alg, staged_row_width=width, fft_path=bool(fft_path))
     return res, roof, cpu, radon, pcie
 
 
@@ -358,11 +387,23 @@ def em_bench(args, torch, dv, ph):
     embp_ms = dv.last_kernel_ms(dv.KF_EMBP)
     dv.set_profiling(False)
     samples_it = 2.0 * S * A * N * N
+    half = samples_it / 2.0  # samples of each of the two kernels per iteration
+    rows_bytes = float(S) * A * N * 4
+    img_bytes = float(S) * N * N * 4
+    roof = dict(
+        projector=valu_roofline("k_project_lds<1>", half, proj_ms * 1e-3, half * 4 + img_bytes + rows_bytes,
+                                valu_issue=read_profile_value("backproject_projector_counters.txt",
+                                                              "valu_issue.k_project_lds<1>")),
+        backproject_update=valu_roofline("k_backproject<float,1>", half, embp_ms * 1e-3,
+                                         half * 4 + rows_bytes + 3 * img_bytes,
+                                         valu_issue=read_profile_value("backproject_projector_counters.txt",
+                                                                       "valu_issue.k_backproject<float,1>")))
     r = dict(config="16 x 1024^2, 1024 angles x 1024 bins, I0=1e4, uniform init", iterations=K, seconds=dt,
              ms_per_iteration=1e3 * dt / K, slice_iterations_per_s=S * K / dt,
              samples_per_s=samples_it * K / dt, kl_first=float(kl[:, 0].sum()), kl_last=float(kl[:, -1].sum()),
              kl_min=float(kl.min(axis=1).sum()), kl_argmin_median=float(np.median(kl.argmin(axis=1))),
-             kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms)
+             kl_excluded_last=int(ex[:, -1].sum()), projector_ms=proj_ms, backproject_update_ms=embp_ms,
+             roofline=roof)
     log(f"EM plain: {K} iterations in {dt:.2f} s ({r['ms_per_iteration']:.2f} ms/it), KL {r['kl_first']:.4e} -> "
         f"{r['kl_last']:.4e}; projector {proj_ms:.3f} ms, backprojection+update {embp_ms:.3f} ms")
     if not args.no_cpu:

@@ -197,6 +197,17 @@ int raft_eem_kl_device(int device, const float* d_obj, const float* d_tomo, cons
 int raft_fbp_plan_info(const float* angles, const float* paramf, const int* parami, int* tlo, int* width,
                        int* fft_path);
 
+/* Host-only planning query of the forward projector (no device work): for
+ * the Radon geometry (em = 0: an Ny x Nx image, half extents ax, ay) or the
+ * eEM one (em = 1: an Nx x Nx image, Ny = Nx, ax and ay ignored), the number
+ * of ray chunks *ray_chunks of the staged (LDS) form, or 0 when the direct
+ * form runs (a chunk's staged box would not fit the tile, or RAFT_PROJ_LDS=0).
+ * tile_consts[4] (if non-null) receives the staged form's constants: rays per
+ * patch, samples per step, angles per workgroup, staged 16-byte entries.
+ * Returns 0 or an error code. */
+int raft_proj_plan_info(const float* angles, int nangles, int Nx, int Ny, int nrays, int nslices, float ax, float ay,
+                        int em, int* ray_chunks, int* tile_consts);
+
 /* Average device time (ms) of the last launch of each kernel family on this
  * thread's last call, for roofline reporting: which = 0 filter, 1 FBP
  * backprojection, 2 projector, 3 EM backprojection/update.  Returns -1 if

@@ -625,11 +625,12 @@ static void em_bp_coeffs(int nangles, int nrays, int n, const double* cs, const 
     }
 }
 
-static void em_backproject(const em_ctx* E, const real* q, real* out, int ones)
+/* rows [jlo, jhi) of B(q) (or of B(1) when ones) */
+static void em_backproject_rows(const em_ctx* E, const real* q, real* out, int ones, long jlo, long jhi)
 {
     int n = E->n, A = E->nangles, R = E->nrays;
     double c = (double)(n - 1) / 2.0;
-    for (int j = 0; j < n; ++j) {
+    for (long j = jlo; j < jhi; ++j) {
         for (int i = 0; i < n; ++i) {
             real acc = 0;
             for (int a = 0; a < A; ++a) {
@@ -642,40 +643,86 @@ static void em_backproject(const em_ctx* E, const real* q, real* out, int ones)
     }
 }
 
-static void em_slices(void* vctx, long lo, long hi)
+static void em_backproject(const em_ctx* E, const real* q, real* out, int ones)
 {
-    em_ctx* E = (em_ctx*)vctx;
+    em_backproject_rows(E, q, out, ones, 0, E->n);
+}
+
+/* angles [alo, ahi) of the forward projection v = dt * A x and the clamped
+ * ratio rr = y SIGN(v) / sqrt(e^2 + v^2) */
+static void em_forward_ratio(const em_ctx* E, const real* x, const real* y, real* v, real* rr, long alo, long ahi)
+{
+    int R = E->nrays;
+    const double e = (double)0.0001f;
+    for (long a = alo; a < ahi; ++a)
+        for (int r = 0; r < R; ++r) {
+            real vv = ray_sum(&E->g, x, E->cs[a], E->sn[a], r) * (real)E->dt;
+            v[(size_t)a * R + r] = vv;
+            real sg = (vv > 0) ? (real)1 : ((vv < 0) ? (real)-1 : (real)0);
+            real q = y[(size_t)a * R + r] * (sg / (real)sqrt(e * e + (double)vv * (double)vv));
+            rr[(size_t)a * R + r] = (q < 0) ? (real)0 : q;
+        }
+}
+
+/* one slice's work split over threads: every element is computed exactly as
+ * in the serial loops (the split changes no arithmetic) */
+typedef struct {
+    const em_ctx* E;
+    const real *x, *y, *q;
+    real *v, *rr, *out;
+    int ones;
+} em_part_ctx;
+
+static void em_forward_part(void* vctx, long lo, long hi)
+{
+    em_part_ctx* P = (em_part_ctx*)vctx;
+    em_forward_ratio(P->E, P->x, P->y, P->v, P->rr, lo, hi);
+}
+
+static void em_backproject_part(void* vctx, long lo, long hi)
+{
+    em_part_ctx* P = (em_part_ctx*)vctx;
+    em_backproject_rows(P->E, P->q, P->out, P->ones, lo, hi);
+}
+
+static void em_run_slice(const em_ctx* E, long s, real* back1, real* bq, real* v, real* y, real* rr, int nthreads)
+{
+    int n = E->n, A = E->nangles, R = E->nrays;
+    real* x = E->obj + (size_t)s * n * n;
+    const float* ys = E->tomo + (size_t)s * A * R;
+    for (long b = 0; b < (long)A * R; ++b) y[b] = (real)ys[b];
+    em_part_ctx P = {E, x, y, NULL, v, rr, back1, 1};
+    parallel_for(n, nthreads, em_backproject_part, &P);
+    for (int it = 0; it < E->niter; ++it) {
+        parallel_for(A, nthreads, em_forward_part, &P);
+        if (E->kl) {
+            long ex = 0;
+            E->kl[(size_t)s * E->niter + it] = SFX(oracle_kl)(y, v, (long)A * R, &ex);
+            if (E->kl_excluded) E->kl_excluded[(size_t)s * E->niter + it] = ex;
+        }
+        em_part_ctx B = {E, x, y, rr, v, rr, bq, 0};
+        parallel_for(n, nthreads, em_backproject_part, &B);
+        for (long p = 0; p < (long)n * n; ++p) x[p] = x[p] * bq[p] / back1[p];
+    }
+}
+
+static int em_slices_run(em_ctx* E, long lo, long hi, int inner_threads)
+{
     int n = E->n, A = E->nangles, R = E->nrays;
     real* back1 = (real*)malloc(sizeof(real) * (size_t)n * n);
     real* bq = (real*)malloc(sizeof(real) * (size_t)n * n);
     real* v = (real*)malloc(sizeof(real) * (size_t)A * R);
     real* y = (real*)malloc(sizeof(real) * (size_t)A * R);
     real* rr = (real*)malloc(sizeof(real) * (size_t)A * R);
-    const double e = (double)0.0001f;
-    for (long s = lo; s < hi; ++s) {
-        real* x = E->obj + (size_t)s * n * n;
-        const float* ys = E->tomo + (size_t)s * A * R;
-        for (long b = 0; b < (long)A * R; ++b) y[b] = (real)ys[b];
-        em_backproject(E, NULL, back1, 1);
-        for (int it = 0; it < E->niter; ++it) {
-            for (int a = 0; a < A; ++a)
-                for (int r = 0; r < R; ++r) {
-                    real vv = ray_sum(&E->g, x, E->cs[a], E->sn[a], r) * (real)E->dt;
-                    v[(size_t)a * R + r] = vv;
-                    real sg = (vv > 0) ? (real)1 : ((vv < 0) ? (real)-1 : (real)0);
-                    real q = y[(size_t)a * R + r] * (sg / (real)sqrt(e * e + (double)vv * (double)vv));
-                    rr[(size_t)a * R + r] = (q < 0) ? (real)0 : q;
-                }
-            if (E->kl) {
-                long ex = 0;
-                E->kl[(size_t)s * E->niter + it] = SFX(oracle_kl)(y, v, (long)A * R, &ex);
-                if (E->kl_excluded) E->kl_excluded[(size_t)s * E->niter + it] = ex;
-            }
-            em_backproject(E, rr, bq, 0);
-            for (long p = 0; p < (long)n * n; ++p) x[p] = x[p] * bq[p] / back1[p];
-        }
-    }
+    int rc = (back1 && bq && v && y && rr) ? 0 : -2;
+    for (long s = lo; s < hi && rc == 0; ++s) em_run_slice(E, s, back1, bq, v, y, rr, inner_threads);
     free(back1); free(bq); free(v); free(y); free(rr);
+    return rc;
+}
+
+static void em_slices(void* vctx, long lo, long hi)
+{
+    (void)em_slices_run((em_ctx*)vctx, lo, hi, 1);
 }
 
 /* get_eEM_RT_MultiGPU(..., obj (in/out initial guess), tomogram, angles, paramf, parami)
@@ -706,9 +753,13 @@ int SFX(oracle_eem)(real* obj, const float* tomo, const float* angles, const int
     E.cs = tab; E.sn = tab + A; E.dth = tab + 2 * A;
     em_bp_coeffs(A, E.nrays, E.n, tab, tab + A, tab + 3 * A, tab + 4 * A, tab + 5 * A);
     E.bi = tab + 3 * A; E.bj = tab + 4 * A; E.a0 = tab + 5 * A;
-    parallel_for(nslices, nthreads, em_slices, &E);
+    /* threads over slices when there are enough of them, else over the angles
+     * and image rows of one slice at a time */
+    int rc = 0;
+    if (nslices >= nthreads) parallel_for(nslices, nthreads, em_slices, &E);
+    else rc = em_slices_run(&E, 0, nslices, nthreads);
     free(tab);
-    return 0;
+    return rc;
 }
 
 /* B(q) alone and the forward operator alone, for stage-level checks. */

@@ -610,45 +610,55 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     RAFT_HIP(hipGetLastError());
 }
 
+void proj_tile_consts(int* c4)
+{
+    c4[0] = PL_RAYS;
+    c4[1] = PL_K;
+    c4[2] = PROJ_ANGLES;
+    c4[3] = PL_CAP;
+}
+
 int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool have_T)
 {
     const char* env = std::getenv("RAFT_PROJ_LDS");
     if (env && env[0] == '0') return 0;
     const int A = g.nangles;
     if (A <= 0 || g.nrays <= 0 || S <= 0) return 0;
-    // worst union box of a chunk: the patch extent plus the spread of the
-    // patch position over the chunk's angles; the spread (max - min of affine
-    // functions) is convex in the position, so its maximum is at a corner of
-    // the (ray, sample) domain
-    const double irs[2] = {-g.ci, (double)(g.nrays - 1) - g.ci};
+    // Worst union box of a chunk.  A wave's box at patch origin (ir0, kr0) is
+    // [lo, hi) with lo = f(ir0, kr0) + min(0, 31 rI) + min(0, 31 rK) and
+    // hi = f + max(0, 31 rI) + max(0, 31 rK), f affine (patch_box: the extremes
+    // of the patch corners).  The union's width max_a hi_a - min_a lo_a is a
+    // maximum of affine functions minus a minimum of affine functions, so it
+    // is convex in the origin and largest at a corner of the origins' domain:
+    // ray blocks ib in [0, last block], steps ks in [0, nk - 1].  The kernel's
+    // box [floor(lo) - 1, floor(hi) + 3) spans at most floor(hi - lo) + 5; the
+    // 1e-9 covers the rounding of the kernel's own FP64 evaluation.
+    const double irs[2] = {-g.ci, (double)(((g.nrays - 1) / PL_RAYS) * PL_RAYS) - g.ci};
     const double krs[2] = {-g.ck, (double)(g.nk - 1) - g.ck};
     for (int a0 = 0; a0 < A; a0 += PROJ_ANGLES) {
         const int a1 = std::min(A, a0 + PROJ_ANGLES);
         const int am = std::min(a0 + PROJ_ANGLES / 2, A - 1);
         const bool steep = have_T && std::fabs(s[am]) > std::fabs(c[am]);
-        double ext_r = 0.0, ext_c = 0.0, spr_r = 0.0, spr_c = 0.0;
-        for (int a = a0; a < a1; ++a) {
-            const double bxk = -s[a] * g.ry, byi = s[a] * g.rx;
-            const double rI = steep ? c[a] : byi, rK = steep ? bxk : c[a];
-            const double cI = steep ? byi : c[a], cK = steep ? c[a] : bxk;
-            ext_r = std::max(ext_r, (PL_RAYS - 1) * std::fabs(rI) + (PL_K - 1) * std::fabs(rK));
-            ext_c = std::max(ext_c, (PL_RAYS - 1) * std::fabs(cI) + (PL_K - 1) * std::fabs(cK));
-        }
+        double wr = 0.0, wc = 0.0;
         for (double ir : irs)
             for (double kr : krs) {
-                double rmin = 1e300, rmax = -1e300, cmin = 1e300, cmax = -1e300;
+                double rlo = 1e300, rhi = -1e300, clo = 1e300, chi = -1e300;
                 for (int a = a0; a < a1; ++a) {
                     const double bxk = -s[a] * g.ry, byi = s[a] * g.rx;
-                    const double fr = steep ? ir * c[a] + kr * bxk : ir * byi + kr * c[a];
-                    const double fc = steep ? ir * byi + kr * c[a] : ir * c[a] + kr * bxk;
-                    rmin = std::min(rmin, fr); rmax = std::max(rmax, fr);
-                    cmin = std::min(cmin, fc); cmax = std::max(cmax, fc);
+                    const double rI = steep ? c[a] : byi, rK = steep ? bxk : c[a];
+                    const double cI = steep ? byi : c[a], cK = steep ? c[a] : bxk;
+                    const double fr = ir * rI + kr * rK, fc = ir * cI + kr * cK;
+                    const double er = (PL_RAYS - 1) * rI, ek = (PL_K - 1) * rK;
+                    const double ec = (PL_RAYS - 1) * cI, el = (PL_K - 1) * cK;
+                    rlo = std::min(rlo, fr + std::min(0.0, er) + std::min(0.0, ek));
+                    rhi = std::max(rhi, fr + std::max(0.0, er) + std::max(0.0, ek));
+                    clo = std::min(clo, fc + std::min(0.0, ec) + std::min(0.0, el));
+                    chi = std::max(chi, fc + std::max(0.0, ec) + std::max(0.0, el));
                 }
-                spr_r = std::max(spr_r, rmax - rmin);
-                spr_c = std::max(spr_c, cmax - cmin);
+                wr = std::max(wr, rhi - rlo);
+                wc = std::max(wc, chi - clo);
             }
-        // kernel box: [floor(lo) - 1, floor(hi) + 3) -> at most floor(hi - lo) + 5
-        const double rows = std::floor(spr_r + ext_r) + 5.0, cols = std::floor(spr_c + ext_c) + 5.0;
+        const double rows = std::floor(wr + 1e-9) + 5.0, cols = std::floor(wc + 1e-9) + 5.0;
         if (!(rows < 4096.0 && cols < 4096.0)) return 0;
         const int ic = (int)cols;
         const long long pitch = ic + ((1 - ic) & 3);

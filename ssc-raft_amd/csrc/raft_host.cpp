@@ -33,6 +33,12 @@ static thread_local std::string g_msg;
 
 void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
+void record_error(int code, const std::string& msg)
+{
+    g_err = code;
+    g_msg = msg;
+}
+
 void hip_check(hipError_t e, const char* what)
 {
     if (e != hipSuccess) fail(RAFT_EHIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -256,14 +262,15 @@ struct Device {
     std::map<std::string, std::unique_ptr<FilterTables>> filters;
     DevBuf ws[kSlots];
     DevBuf tab; // per-call tables (see TableSet)
-    void* workspace(int slot, size_t bytes)
+    // the slot is a template argument so that every index is checked at compile time
+    template <int SLOT> void* workspace(size_t bytes)
     {
-        if (slot < 0 || slot >= kSlots) fail(RAFT_EINVAL, "internal: workspace slot out of range");
-        if (ws[slot].bytes < bytes) {
-            ws[slot].release();
-            ws[slot].alloc(bytes);
+        static_assert(SLOT >= 0 && SLOT < kSlots, "workspace slot out of range");
+        if (ws[SLOT].bytes < bytes) {
+            ws[SLOT].release();
+            ws[SLOT].alloc(bytes);
         }
-        return ws[slot].p;
+        return ws[SLOT].p;
     }
     char* tables(size_t bytes)
     {
@@ -471,7 +478,7 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
     B = std::min(c.S, ((B + SB - 1) / SB) * SB);
     const size_t budget = block_budget();
     while (B > SB && (size_t)((B + SB - 1) / SB) * per_group > budget) B = (((B + 1) / 2 + SB - 1) / SB) * SB;
-    void* q = D.workspace(0, (size_t)((B + SB - 1) / SB) * per_group);
+    void* q = D.workspace<0>((size_t)((B + SB - 1) / SB) * per_group);
 
     for (int s0 = 0; s0 < c.S; s0 += B) {
         const int nb = std::min(B, c.S - s0);
@@ -545,10 +552,10 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
     B = ((B + 3) / 4) * 4;
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
-    float* q = static_cast<float*>(D.workspace(0, (size_t)B * A * n * 4));
-    float2* polar = static_cast<float2*>(D.workspace(1, (size_t)B * 2 * A * n * 8));
-    float2* cart = static_cast<float2*>(D.workspace(2, (size_t)B * s2 * 8));
-    float2* cartT = static_cast<float2*>(D.workspace(3, (size_t)B * s2 * 8));
+    float* q = static_cast<float*>(D.workspace<0>((size_t)B * A * n * 4));
+    float2* polar = static_cast<float2*>(D.workspace<1>((size_t)B * 2 * A * n * 8));
+    float2* cart = static_cast<float2*>(D.workspace<2>((size_t)B * s2 * 8));
+    float2* cartT = static_cast<float2*>(D.workspace<3>((size_t)B * s2 * 8));
     for (int s0 = 0; s0 < c.S; s0 += B) {
         const int nb = std::min(B, c.S - s0);
         FilterArgs fa;
@@ -599,7 +606,7 @@ static void lowpass_device(int dev, float* d_tomo, const float* paramf, const in
     B = std::min(c.S, ((B + SB - 1) / SB) * SB);
     const size_t budget = block_budget();
     while (B > SB && (size_t)((B + SB - 1) / SB) * per_group > budget) B = (((B + 1) / 2 + SB - 1) / SB) * SB;
-    float* q = static_cast<float*>(D.workspace(0, (size_t)((B + SB - 1) / SB) * per_group));
+    float* q = static_cast<float*>(D.workspace<0>((size_t)((B + SB - 1) / SB) * per_group));
     for (int s0 = 0; s0 < c.S; s0 += B) {
         const int nb = std::min(B, c.S - s0);
         FilterArgs fa;
@@ -647,8 +654,8 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
     int B = ((nslices + 3) / 4) * 4;
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * img * 8 > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
-    float* il = static_cast<float*>(D.workspace(1, (size_t)B * img * 4));
-    float* ilT = static_cast<float*>(D.workspace(2, (size_t)B * img * 4));
+    float* il = static_cast<float*>(D.workspace<1>((size_t)B * img * 4));
+    float* ilT = static_cast<float*>(D.workspace<2>((size_t)B * img * 4));
     const int prc = proj_plan(g, cv.data(), sv.data(), B, true);
     for (int s0 = 0; s0 < nslices; s0 += B) {
         const int nb = std::min(B, nslices - s0);
@@ -732,14 +739,14 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
     const int Gb = B / 4;
 
-    float* rows = static_cast<float*>(D.workspace(0, (size_t)Gb * A * g.width * 16));
-    float* xil = static_cast<float*>(D.workspace(1, (size_t)Gb * img * 16));
-    float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
-    float* ones = static_cast<float*>(D.workspace(3, img * 4 + (size_t)A * g.width * 16));
+    float* rows = static_cast<float*>(D.workspace<0>((size_t)Gb * A * g.width * 16));
+    float* xil = static_cast<float*>(D.workspace<1>((size_t)Gb * img * 16));
+    float* xT = static_cast<float*>(D.workspace<2>((size_t)Gb * img * 16));
+    float* ones = static_cast<float*>(D.workspace<3>(img * 4 + (size_t)A * g.width * 16));
     float* onerows = ones + img;
     const int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
     const int kslots = proj_kl_slots(A, prc);
-    double* part = static_cast<double*>(D.workspace(4, (size_t)B * kslots * 16));
+    double* part = static_cast<double*>(D.workspace<4>((size_t)B * kslots * 16));
     unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * kslots);
     const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
     // iterations whose KL is recorded (the first Kkl), the row stride of kl
@@ -747,7 +754,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     double* dkl = nullptr;
     unsigned long long* dex = nullptr;
     if (Kkl > 0) {
-        double* buf = static_cast<double*>(D.workspace(5, (size_t)S * Kkl * 16));
+        double* buf = static_cast<double*>(D.workspace<5>((size_t)S * Kkl * 16));
         dkl = buf;
         dex = reinterpret_cast<unsigned long long*>(buf + (size_t)S * Kkl);
     }
@@ -765,7 +772,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     if (!kl_only) RAFT_HIP(hipMemsetAsync(rows, 0, (size_t)Gb * A * g.width * 16, st));
 
     // one iteration: projector + ratio (+ KL partials), KL reduction, backprojection + update
-    int* counter = reinterpret_cast<int*>(static_cast<char*>(D.workspace(6, 256)));
+    int* counter = reinterpret_cast<int*>(static_cast<char*>(D.workspace<6>(256)));
     auto iteration = [&](int it, int nb, const float* y, double* klo, unsigned long long* exo, bool graph,
                          hipStream_t s, bool with_kl) {
         ProjArgs p;
@@ -914,11 +921,11 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
     const int Gb = B / 4;
-    float* rows = static_cast<float*>(D.workspace(0, (size_t)Gb * A * g.width * 16));
-    float* xil = static_cast<float*>(D.workspace(1, (size_t)Gb * img * 16));
-    float* xT = static_cast<float*>(D.workspace(2, (size_t)Gb * img * 16));
-    float* normil = static_cast<float*>(D.workspace(3, (size_t)Gb * img * 16));
-    float* tmp = static_cast<float*>(D.workspace(5, (size_t)B * img * 4));
+    float* rows = static_cast<float*>(D.workspace<0>((size_t)Gb * A * g.width * 16));
+    float* xil = static_cast<float*>(D.workspace<1>((size_t)Gb * img * 16));
+    float* xT = static_cast<float*>(D.workspace<2>((size_t)Gb * img * 16));
+    float* normil = static_cast<float*>(D.workspace<3>((size_t)Gb * img * 16));
+    float* tmp = static_cast<float*>(D.workspace<5>((size_t)B * img * 4));
     const int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
 
     for (int s0 = 0; s0 < S; s0 += B) {
@@ -1085,8 +1092,6 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             b->y.alloc((size_t)S * sino * 4);
             b->x.alloc((size_t)S * img * 4);
         }
-        RAFT_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
-        RAFT_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     }
     RAFT_HIP(hipStreamSynchronize(st));
 
@@ -1176,6 +1181,12 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
         if (cs) (void)hipStreamDestroy(cs);
     };
     try {
+        // streams and events only once every bucket is validated and set up,
+        // and inside the block whose cleanup destroys them
+        for (BatchBucket* b : buckets) {
+            RAFT_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+            RAFT_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+        }
         RAFT_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         RAFT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         RAFT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -1653,6 +1664,27 @@ int raft_fbp_plan_info(const float* angles, const float* paramf, const int* para
         if (tlo) *tlo = g.tlo;
         if (width) *width = g.width;
         if (fft_path) *fft_path = fft_plan(c.npad).ok ? 1 : 0;
+    });
+}
+
+int raft_proj_plan_info(const float* angles, int nangles, int Nx, int Ny, int nrays, int nslices, float ax, float ay,
+                        int em, int* ray_chunks, int* tile_consts)
+{
+    return guarded([&] {
+        if (!angles) fail(RAFT_EINVAL, "null angles");
+        if (nangles < 1 || nrays < 2 || nslices < 1) fail(RAFT_EINVAL, "invalid projection sizes");
+        if (Nx < 2 || Ny < 2) fail(RAFT_EINVAL, "image sizes must be >= 2");
+        if (em && Nx != Ny) fail(RAFT_EINVAL, "the eEM image is square");
+        if (!em && (!(ax != 0.0f) || !(ay != 0.0f))) fail(RAFT_EINVAL, "ax, ay must be non-zero");
+        std::vector<double> cv(nangles), sv(nangles);
+        for (int a = 0; a < nangles; ++a) {
+            cv[a] = std::cos((double)angles[a]);
+            sv[a] = std::sin((double)angles[a]);
+        }
+        const RayGeom g = em ? ray_geom_em(nrays, nangles, Nx)
+                             : ray_geom_radon(Nx, Ny, nrays, nangles, (double)ax, (double)ay);
+        if (ray_chunks) *ray_chunks = proj_plan(g, cv.data(), sv.data(), nslices, true);
+        if (tile_consts) proj_tile_consts(tile_consts);
     });
 }
 

@@ -22,6 +22,8 @@ struct Error {
 };
 
 [[noreturn]] void fail(int code, const std::string& msg);
+// sets this thread's error state (raft_last_error) without throwing
+void record_error(int code, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
 #define RAFT_HIP(call) ::raft::hip_check((call), #call)
 
@@ -176,6 +178,9 @@ void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
 // chunks, or 0 when a chunk of 8 angles cannot be staged (sparse angles) or
 // RAFT_PROJ_LDS=0.  have_T: the transposed image will be passed as imgT.
 int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool have_T);
+// the staged form's tile constants: rays per patch, samples per step, angles
+// per workgroup, staged 16-byte entries per workgroup
+void proj_tile_consts(int* c4);
 inline int proj_kl_slots(int nangles, int lds_rc) { return nangles * (lds_rc > 0 ? lds_rc : 1); }
 
 // planar [S][rows][cols] -> interleaved [G][rows][cols][4] (or its transpose [G][cols][rows][4])

@@ -6,7 +6,7 @@ import numpy
 from .rafttypes import CNICE, libraft, raft_check, ctypes
 
 __all__ = ["fbp_device", "radon_device", "eem_device", "eem_kl_device", "batch_device", "lowpass_device",
-           "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info",
+           "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info", "proj_plan_info",
            "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
 
 KF_FILTER, KF_BP, KF_PROJ, KF_EMBP = 0, 1, 2, 3
@@ -242,3 +242,21 @@ def fbp_plan_info(angles, paramf, parami):
     if rc:
         raft_check("raft_fbp_plan_info")
     return tlo.value, width.value, fft.value
+
+
+def proj_plan_info(angles, Nx, Ny=None, nrays=None, nslices=4, ax=1.0, ay=1.0, em=False):
+    """(ray_chunks, (rays per patch, samples per step, angles per workgroup,
+    staged entries)) of the forward projector's plan (host only): ray_chunks
+    > 0 selects the staged (LDS) form, 0 the direct form.  em=True plans the
+    eEM projector of an Nx x Nx image from nrays-bin rows."""
+    angles = CNICE(numpy.asarray(angles))
+    Ny = Nx if Ny is None else Ny
+    nrays = Nx if nrays is None else nrays
+    rc = ctypes.c_int(0)
+    consts = numpy.zeros(4, numpy.int32)
+    err = libraft.raft_proj_plan_info(angles.ctypes.data_as(ctypes.c_void_p), angles.shape[0], Nx, Ny, nrays,
+                                      nslices, ax, ay, 1 if em else 0, ctypes.byref(rc),
+                                      consts.ctypes.data_as(ctypes.c_void_p))
+    if err:
+        raft_check("raft_proj_plan_info")
+    return rc.value, tuple(int(c) for c in consts)

@@ -7,6 +7,17 @@ from ...rafttypes import CNICE, libraft, raft_check, numpy, ctypes
 __all__ = ["eEMRT_GPU_", "eEMRT_GPU_KL", "tEMRT_GPU_"]
 
 
+def _check_sizes(obj, angles, nslices, objsize, nangles):
+    """The C entry points copy nslices * objsize^2 values to and from `obj` and
+    read nangles angles: refuse arrays of any other size instead of letting
+    the copies run past their ends."""
+    if obj.size != nslices * objsize * objsize:
+        raise ValueError(f"obj must hold nslices * objsize * objsize = {nslices} * {objsize}^2 values, "
+                         f"got shape {obj.shape}")
+    if angles.ndim != 1 or angles.shape[0] != nangles:
+        raise ValueError(f"angles must hold {nangles} values (one per projection), got shape {angles.shape}")
+
+
 def _prepare(tomo, angles, iterations, gpus, blocksize, obj):
     nslices = 1 if len(tomo.shape) == 2 else tomo.shape[0]
     nangles = tomo.shape[-2]
@@ -19,6 +30,7 @@ def _prepare(tomo, angles, iterations, gpus, blocksize, obj):
         obj = CNICE(numpy.ones([nslices, objsize, objsize], dtype=numpy.float32))
     tomo = CNICE(tomo)
     angles = CNICE(numpy.array(angles))
+    _check_sizes(obj, angles, nslices, objsize, nangles)
     parami = CNICE(numpy.array([nrays, nangles, nslices, objsize, 0, 0, 0, 1, iterations, blocksize]), numpy.int32)
     paramf = CNICE(numpy.array([0]), numpy.float32)
     return gpus, obj, tomo, angles, paramf, parami, nslices
@@ -71,6 +83,7 @@ def tEMRT_GPU_(counts, flat, angles, iterations, gpus, blocksize, obj=None):
     else:
         obj = CNICE(numpy.ones([nslices, objsize, objsize], dtype=numpy.float32))
     angles = CNICE(numpy.array(angles))
+    _check_sizes(obj, angles, nslices, objsize, nangles)
     parami = CNICE(numpy.array([nrays, nangles, nslices, objsize, 0, 0, 0, nflats, iterations, blocksize]),
                    numpy.int32)
     paramf = CNICE(numpy.array([0]), numpy.float32)

@@ -71,6 +71,8 @@ def bstGPU(tomogram, angles, gpus, dic, obj=None, nstreams=0):
         obj = numpy.zeros([nslices, nrays, nrays], dtype=numpy.float32)
     obj = CNICE(obj)
     angles = CNICE(numpy.array(angles))
+    if obj.size != nslices * nrays * nrays:
+        raise ValueError("obj must hold nslices * nrays * nrays values")
     libraft.getBSTMultiGPU(gpus.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(len(gpus)),
                            obj.ctypes.data_as(ctypes.c_void_p), tomogram.ctypes.data_as(ctypes.c_void_p),
                            angles.ctypes.data_as(ctypes.c_void_p), paramf.ctypes.data_as(ctypes.c_void_p),

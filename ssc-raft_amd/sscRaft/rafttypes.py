@@ -81,6 +81,8 @@ libraft.raft_eem_kl_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_eem_kl_device.restype = _i
 libraft.raft_fbp_plan_info.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_fbp_plan_info.restype = _i
+libraft.raft_proj_plan_info.argtypes = [_vp, _i, _i, _i, _i, _i, _f, _f, _i, _vp, _vp]
+libraft.raft_proj_plan_info.restype = _i
 libraft.raft_set_profiling.argtypes = [_i]
 libraft.raft_set_profiling.restype = None
 libraft.raft_last_kernel_ms.argtypes = [_i]

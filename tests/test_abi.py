@@ -16,7 +16,7 @@ import re
 
 import numpy as np
 import pytest
-from hypothesis import given, settings
+from hypothesis import example, given, settings
 from hypothesis import strategies as st
 
 import oracle
@@ -202,3 +202,206 @@ def test_fbp_front_end_validation():
         sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0], "method": "tEMFQ"})
     with pytest.raises(ValueError, match="Invalid EM method"):
         sscRaft.em(y, angles=np.zeros(8), dic={"gpu": [0], "method": "xEM"})
+
+
+def test_em_and_bst_front_ends_refuse_wrong_sizes():
+    """eEMRT_GPU_ / eEMRT_GPU_KL / tEMRT_GPU_ / bstGPU check `obj` and `angles`
+    against the data shape before the C call: the C side copies nslices *
+    objsize^2 values to and from `obj` and reads nangles angles, so a wrong
+    size would run past the ends of the caller's arrays."""
+    import sscRaft
+    from sscRaft.geometries.parallel.fbp_methods import bstGPU
+
+    y = np.ones((2, 8, 16), np.float32)
+    ang = np.linspace(0, np.pi, 8, endpoint=False).astype(np.float32)
+    small = np.ones((2, 15, 15), np.float32)
+    for fn in (sscRaft.eEMRT_GPU_, sscRaft.eEMRT_GPU_KL):
+        with pytest.raises(ValueError, match="obj"):
+            fn(y, ang, 2, [0], 0, obj=small)
+        with pytest.raises(ValueError, match="obj"):
+            fn(y, ang, 2, [0], 0, obj=np.ones((1, 16, 16), np.float32))
+        with pytest.raises(ValueError, match="angles"):
+            fn(y, ang[:7], 2, [0], 0)
+        with pytest.raises(ValueError, match="angles"):
+            fn(y, np.zeros((2, 8), np.float32), 2, [0], 0)
+    flat = np.ones((2, 1, 16), np.float32)
+    with pytest.raises(ValueError, match="obj"):
+        sscRaft.tEMRT_GPU_(y, flat, ang, 2, [0], 0, obj=small)
+    with pytest.raises(ValueError, match="angles"):
+        sscRaft.tEMRT_GPU_(y, flat, ang[:5], 2, [0], 0)
+    dic = {"filter": "ramp", "beta/delta": 0.0, "regularization": 1.0, "rotation axis offset": 0.0,
+           "blocksize": 0, "energy[eV]": 1.0, "z2[m]": 0.0, "detectorPixel[m]": 1.0, "padding": 2}
+    with pytest.raises(ValueError, match="obj"):
+        bstGPU(y, ang, [0], dic, obj=small)
+
+
+REFERENCE_RAFTTYPES = "/root/reference/sscRaft/rafttypes.py"
+
+
+def _rafttypes_symbols():
+    import json
+
+    with open(os.path.join(ROOT, "tests", "golden", "rafttypes_symbols.json")) as fh:
+        return json.load(fh)
+
+
+def test_rafttypes_symbol_fixture_matches_the_reference_loader():
+    """The committed list equals a fresh parse of the reference's rafttypes.py
+    (where the reference is present, as in the build container)."""
+    if not os.path.exists(REFERENCE_RAFTTYPES):
+        pytest.skip("reference sources not present on this machine")
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "list_rafttypes_symbols", os.path.join(ROOT, "tests", "golden", "list_rafttypes_symbols.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    with open(REFERENCE_RAFTTYPES) as fh:
+        assert mod.parse(fh.read()) == _rafttypes_symbols()
+
+
+class _LabFDK(ctypes.Structure):  # struct RaftStubLabFDK (include/raft_stubs.h)
+    _fields_ = ([(n, ctypes.c_float) for n in ("x", "y", "z", "dx", "dy", "dz")]
+                + [(n, ctypes.c_int) for n in ("nx", "ny", "nz")]
+                + [(n, ctypes.c_float) for n in ("h", "v", "dh", "dv")]
+                + [(n, ctypes.c_int) for n in ("nh", "nv")]
+                + [(n, ctypes.c_float) for n in ("D", "Dsd", "beta_max", "dbeta")]
+                + [(n, ctypes.c_int) for n in ("nbeta", "fourier", "filter_type")]
+                + [("reg", ctypes.c_float)]
+                + [(n, ctypes.c_int) for n in ("is_slice", "s0", "s1", "s2", "s3", "nph", "padh")]
+                + [("energy", ctypes.c_float)]
+                + [(n, ctypes.c_int) for n in ("rotation_axis_offset", "blocksize")])
+
+
+class _LabRay(ctypes.Structure):  # struct RaftStubLabEM / RaftStubLabCB (include/raft_stubs.h)
+    _fields_ = ([(n, ctypes.c_float) for n in ("a", "b", "c", "x0", "y0", "z0")]
+                + [(n, ctypes.c_int) for n in ("nx", "ny", "nz")]
+                + [(n, ctypes.c_float) for n in ("sx", "sy", "sz")]
+                + [(n, ctypes.c_int) for n in ("nbeta", "ndet", "npts")])
+
+
+def test_every_reference_binding_resolves_and_stubs_report_not_implemented():
+    """Every function the reference's loader binds is exported by libraft.so,
+    and binding it as the loader does works (argtypes / restype set from the
+    fixture).  The ones outside the hot path record RAFT_EUNSUPPORTED and
+    touch nothing; no GPU is needed to call them."""
+    import sscRaft
+
+    lib = ctypes.CDLL(sscRaft.LIB_PATH)
+    structs = {"Lab": _LabFDK, "Lab_EM": _LabRay, "Lab_CB": _LabRay}
+    built = {"getFBPMultiGPU", "get_eEM_RT_MultiGPU", "getRadonRTMultiGPU", "get_tEM_RT_MultiGPU",
+             "getFilterLowPassMultiGPU", "getBSTMultiGPU", "getBackgroundCorrectionMultiGPU"}
+    syms = _rafttypes_symbols()
+    assert len(syms) == 23 and built <= set(syms)
+    lib.raft_last_error.restype = ctypes.c_int
+    lib.raft_last_error_string.restype = ctypes.c_char_p
+    for name, sig in syms.items():
+        fn = getattr(lib, name)  # AttributeError if not exported
+        types = [structs.get(t) or getattr(ctypes, t) for t in sig["argtypes"]]
+        fn.argtypes = types
+        fn.restype = None if sig["restype"] == "None" else getattr(ctypes, sig["restype"])
+        if name in built:
+            continue
+        buf = np.full(8, 3.0, np.float32)
+        args = [t() if t in structs.values() else (buf.ctypes.data_as(ctypes.c_void_p) if t is ctypes.c_void_p
+                                                   else t(1)) for t in types]
+        lib.raft_clear_error()
+        ret = fn(*args)
+        assert lib.raft_last_error() == 4, name  # RAFT_EUNSUPPORTED
+        assert b"not implemented: " + name.encode() in lib.raft_last_error_string()
+        assert np.all(buf == 3.0)
+        if sig["restype"] == "c_int":
+            assert ret == -1
+        elif sig["restype"] == "c_float":
+            assert np.isnan(ret)
+    lib.raft_clear_error()
+
+
+def _proj_geom(Nx, Ny, nrays, ax, ay, em):
+    """Sampling constants of the forward projector (ray_geom_radon / ray_geom_em)."""
+    if em:
+        c = (nrays - 1) / 2.0
+        return dict(ci=c, ck=c, ucx=c, ucy=c, ry=1.0, rx=1.0, nk=nrays)
+    return dict(ci=(Nx - 1) / 2.0, ck=(Ny - 1) / 2.0, ucx=(Nx - 1) / (2.0 * ax), ucy=(Ny - 1) / (2.0 * ay),
+                ry=(ay * (Nx - 1)) / (ax * (Ny - 1)), rx=(ax * (Ny - 1)) / (ay * (Nx - 1)), nk=Ny)
+
+
+@settings(max_examples=30, deadline=None)
+@example(em=True, nx=64, ny=64, nrays=64, a=64, kind="uniform", seed=0, ax=1.0, ay=1.0)
+@example(em=False, nx=96, ny=80, nrays=100, a=100, kind="uniform", seed=0, ax=1.0, ay=1.0)
+@example(em=True, nx=120, ny=120, nrays=120, a=160, kind="uniform", seed=3, ax=1.0, ay=1.0)
+@example(em=False, nx=8, ny=8, nrays=8, a=8, kind="uniform", seed=0, ax=1.0, ay=1.0)
+@given(em=st.booleans(), nx=st.integers(8, 120), ny=st.integers(8, 120), nrays=st.integers(8, 130),
+       a=st.integers(8, 160), kind=st.sampled_from(["uniform", "random", "cluster"]), seed=st.integers(0, 10 ** 6),
+       ax=st.sampled_from([1.0, 0.7, 1.6]), ay=st.sampled_from([1.0, 1.3, 0.8]))
+def test_projector_plan_keeps_every_sample_in_the_staged_box(em, nx, ny, nrays, a, kind, seed, ax, ay):
+    """The staged projector gathers from LDS with no clamp: it relies on every
+    sample's cell lying in its wave's patch box, and on the union of the 8
+    boxes of a workgroup fitting the tile whenever raft_proj_plan_info admits
+    the staged form.  Brute force over every chunk of 8 angles, every 32-ray
+    block and a spread of step offsets (the kernel's steps start anywhere in
+    [0, nk)): the patch box (FP64 extremes over the patch corners, margins -1
+    and +3) contains each of the 32 x 32 samples, and the union of all 8 boxes
+    (a superset of the kernel's union, which skips waves with no samples in
+    the step) fits PL_CAP entries at the kernel's odd pitch."""
+    from sscRaft import device as dv
+
+    rng = np.random.default_rng(seed)
+    if em:
+        ny, ax, ay = nx, 1.0, 1.0
+        nrays = min(nrays, nx + nx // 16)  # what the eEM entry point admits (backprojection window)
+    if kind == "uniform":
+        ang = (np.arange(a) * (np.pi / a)).astype(np.float32)
+    elif kind == "random":
+        ang = np.sort(rng.uniform(-np.pi, 2 * np.pi, a)).astype(np.float32)
+    else:  # dense clusters straddling the axes and the diagonal
+        ang = np.sort(rng.choice([0.0, np.pi / 4, np.pi / 2, np.pi], a) + rng.normal(0, 0.05, a)).astype(np.float32)
+    rc, (PR, PK, PA, CAP) = dv.proj_plan_info(ang, nx, ny, nrays, nslices=4, ax=ax, ay=ay, em=em)
+    if (em, nx, a, kind) in ((True, 64, 64, "uniform"), (False, 96, 100, "uniform"), (True, 120, 160, "uniform")):
+        assert rc > 0  # dense sets take the staged form
+    if rc == 0:
+        return
+    g = _proj_geom(nx, ny, nrays, ax, ay, em)
+    nk = g["nk"]
+    th = ang.astype(np.float64)
+    cs_all, sn_all = np.cos(th), np.sin(th)
+    r = np.arange(PR, dtype=np.float64)
+    m = np.arange(PK, dtype=np.float64)
+    ks_all = np.unique(np.concatenate([np.arange(0, nk, 7), [0, nk - 1], np.arange(0, nk, PK)]))
+    for a0 in range(0, a, PA):
+        am = min(a0 + PA // 2, a - 1)
+        steep = abs(sn_all[am]) > abs(cs_all[am])
+        cs, sn = cs_all[a0:a0 + PA], sn_all[a0:a0 + PA]
+        bxk, byi = -sn * g["ry"], sn * g["rx"]
+        rI, rK = (cs, bxk) if steep else (byi, cs)
+        cI, cK = (byi, cs) if steep else (cs, bxk)
+        rU, cU = (g["ucx"], g["ucy"]) if steep else (g["ucy"], g["ucx"])
+        W = len(cs)
+        for ib in range(0, nrays, PR):
+            ir0 = ib - g["ci"]
+            kr0 = ks_all - g["ck"]  # [K]
+            ir1, kr1 = ir0 + (PR - 1), kr0 + (PK - 1)
+
+            def ext(U, I, K, f):
+                return U + f(ir0 * I, ir1 * I)[:, None] + f(kr0[None, :] * K[:, None], kr1[None, :] * K[:, None])
+
+            lo_r = np.floor(ext(rU, rI, rK, np.minimum)) - 1  # [W, K]
+            hi_r = np.floor(ext(rU, rI, rK, np.maximum)) + 3
+            lo_c = np.floor(ext(cU, cI, cK, np.minimum)) - 1
+            hi_c = np.floor(ext(cU, cI, cK, np.maximum)) + 3
+            # samples: ray ib + r, sample ks + m (m < nk - ks)
+            ir = ir0 + r  # [R]
+            kr = kr0[:, None] + m[None, :]  # [K, M]
+            row = np.trunc(kr[None, :, None, :] * rK[:, None, None, None]
+                           + (ir[None, None, :, None] * rI[:, None, None, None] + rU))  # [W, K, R, M]
+            col = np.trunc(kr[None, :, None, :] * cK[:, None, None, None]
+                           + (ir[None, None, :, None] * cI[:, None, None, None] + cU))
+            valid = (ks_all[:, None] + m[None, :] < nk)[None, :, None, :]
+            inside = ((row >= lo_r[:, :, None, None]) & (row < hi_r[:, :, None, None])
+                      & (col >= lo_c[:, :, None, None]) & (col < hi_c[:, :, None, None]))
+            assert np.all(inside | ~valid), (a0, ib)
+            rows = hi_r.max(axis=0) - lo_r.min(axis=0)
+            w = (hi_c.max(axis=0) - lo_c.min(axis=0)).astype(np.int64)
+            pitch = w + ((1 - w) & 3)
+            assert np.all(rows * pitch <= CAP), (a0, ib, int((rows * pitch).max()), CAP)
+    assert W >= 1

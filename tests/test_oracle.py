@@ -592,3 +592,127 @@ def test_bst_restatement_reconstructs_the_phantom():
     pi0[7] = 0
     b0 = oracle_np.bst(p, pf, pi0)[0][8:-8, 8:-8]
     assert (b0 * b).sum() > 0
+
+
+def _bst_literal(tomo, paramf, parami, plant=None):
+    """getBST restated element by element from the reference kernels, in
+    float64 with explicit DFT sums (pure Python; tiny sizes only):
+      pad        opt::paddR2R (padding.cu:140-178), edge replicate at L = pad N / 2
+      filter     BSTFilter (filters.cu:369-390): SetX, forward C2C,
+                 BandFilterC2C (:295-307: w = min(k, n-k)/n, apply(w),
+                 exp(-2 pi i center k / n), center = (int) offset), inverse
+                 C2C unnormalised, GetX with scale 1
+      sino2p     bst.cu:22-48 (nrays = n, pad0 = 2, csino = 0)
+      C2C        forward, rows of n = n * pad0 / 2 (bst.cu:256-266)
+      convBST    bst.cu:50-61
+      p2c        polar2cartesian_fourier, bst.cu:63-98 (sizeimage = n)
+      2-D C2C    inverse, unnormalised
+      GetX       Re / (n * pixel * 4) (bst.cu:296-301), then the crop
+                 remove_paddR2R (padding.cu:245-268)
+    plant: a deliberate slip for the sensitivity checks ("tfrac" flips the
+    angular weight to angle - iarc, "sign" drops the checkerboard factor,
+    "half" keeps the full weight at the first sample, "mirror" reads the
+    mirrored half-row one bin off)."""
+    import cmath
+    import math
+
+    import oracle_np
+
+    c = oracle_np.decode_fbp(paramf, parami)
+    tomo = np.asarray(tomo, np.float32)
+    S, A, N = tomo.shape
+    n, L, ftype = c["npad"], c["L"], c["ftype"]
+    csino = int(c["offset"])
+    F = [[cmath.exp(-2j * math.pi * k * x / n) for x in range(n)] for k in range(n)]
+    out = np.zeros((S, N, N))
+    for s in range(S):
+        rows = []
+        for a in range(A):
+            u = [float(tomo[s, a, min(max(m - L, 0), N - 1)]) for m in range(n)]
+            if ftype != 0:
+                U = [sum(u[x] * F[k][x] for x in range(n)) for k in range(n)]
+                for k in range(n):
+                    w = min(k, n - k) / n
+                    U[k] *= cmath.exp(-2j * math.pi / n * csino * k) * float(oracle_np.filter_apply(
+                        ftype, w, c["reg"], c["pag"]))
+                u = [sum(U[k] * F[k][x].conjugate() for k in range(n)).real for x in range(n)]
+            rows.append(u)
+        pad0 = 2
+        center = n // 2
+        NR, NA = n * pad0 // 2, 2 * A
+        real = [[0j] * NR for _ in range(NA)]
+        for fory in range(A):
+            for idx in range(n // 2):
+                w = 1.0
+                if idx == 0 and plant != "half":
+                    w *= 0.5
+                w *= pad0 * pad0
+                lo = center - idx if plant == "mirror" else center - 1 - idx
+                real[fory][idx] = w * (rows[fory][lo] if lo >= 0 else rows[fory][0])
+                real[fory + A][idx] = w * (rows[fory][center + idx] if center + idx >= 0 else rows[fory][0])
+        FR = [[cmath.exp(-2j * math.pi * k * x / NR) for x in range(NR)] for k in range(NR)]
+        polar = [[sum(real[r][x] * FR[k][x] for x in range(NR)) for k in range(NR)] for r in range(NA)]
+        for r in range(NA):
+            for tx in range(NR):
+                polar[r][tx] *= 2.0 / (NR * max(min(tx, NR - tx), 0.5))
+        si = n
+        cart = [[0j] * si for _ in range(si)]
+        for ty in range(si):
+            for tx in range(si):
+                posx, posy = tx - si // 2, ty - si // 2
+                rho = NR * math.hypot(posx, posy) / si
+                angle = NA * (0.5 * math.atan2(posy, posx) / math.pi + 0.5)
+                irho, iarc = int(rho), int(angle)
+                val = 0j
+                if irho < NR // 2 - 1:
+                    pfrac = rho - irho
+                    tfrac = (angle - iarc) if plant == "tfrac" else (iarc - angle)
+                    iarc = iarc % NA
+                    uarc = (iarc + 1) % NA
+                    i0 = polar[iarc][irho] * (1.0 - pfrac) + polar[iarc][irho + 1] * pfrac
+                    i1 = polar[uarc][irho] * (1.0 - pfrac) + polar[uarc][irho + 1] * pfrac
+                    val = i0 * tfrac + i1 * (1.0 - tfrac)
+                sgn = 1.0 if plant == "sign" else 4 * (tx % 2 - 0.5) * (ty % 2 - 0.5)
+                cart[(ty + si // 2) % si][(tx + si // 2) % si] = val * sgn
+        Fs = [[cmath.exp(2j * math.pi * k * x / si) for x in range(si)] for k in range(si)]
+        tmp = [[sum(cart[ky][kx] * Fs[kx][x] for kx in range(si)) for x in range(si)] for ky in range(si)]
+        img = [[sum(tmp[ky][x] * Fs[ky][y] for ky in range(si)).real for x in range(si)] for y in range(si)]
+        scale = n * c["pixel"] * 4.0
+        for j in range(N):
+            for i in range(N):
+                out[s, j, i] = img[j + L][i + L] / scale
+    return out
+
+
+@pytest.mark.parametrize("N,A,pad,ftype,offset", [(6, 5, 1, 7, 0.0), (5, 4, 2, 5, 1.0), (8, 6, 1, 0, 0.0),
+                                                  (6, 7, 1, 1, -2.0)])
+def test_bst_restatement_matches_a_literal_brute_force(N, A, pad, ftype, offset):
+    """oracle_np.bst (vectorised, pocketfft) equals the element-by-element
+    restatement of bst.cu's kernels with explicit DFT sums to 1e-10, on tiny
+    sizes with every filter path (ramp, hann with an axis shift, none,
+    gaussian with a negative shift) and odd and even padded sizes."""
+    import oracle_np
+
+    rng = np.random.default_rng(N * 100 + A)
+    tomo = rng.standard_normal((2, A, N)).astype(np.float32)
+    pf, pi = oracle.fbp_params(N, A, 2, pad=pad, filter_type=ftype, pixel=0.7, offset=offset, reg=0.4)
+    lit = _bst_literal(tomo, pf, pi)
+    vec = oracle_np.bst(tomo, pf, pi)
+    e = np.linalg.norm(vec - lit) / np.linalg.norm(lit)
+    assert e < 1e-10, e
+
+
+@pytest.mark.parametrize("plant", ["tfrac", "sign", "half", "mirror"])
+def test_bst_literal_pin_catches_planted_slips(plant):
+    """The pin above is tight enough to fail the slips a restatement could
+    make: the 'corrected' angular weight angle - iarc (the reference's
+    iarc - angle extrapolates), a dropped checkerboard sign, the first
+    sample's half weight, a mirrored half-row read one bin off.  Each plant
+    moves the result far beyond 1e-10."""
+    import oracle_np
+
+    rng = np.random.default_rng(3)
+    tomo = rng.standard_normal((1, 5, 6)).astype(np.float32)
+    pf, pi = oracle.fbp_params(6, 5, 1, pad=1, filter_type=7, pixel=0.7)
+    e = np.linalg.norm(oracle_np.bst(tomo, pf, pi) - _bst_literal(tomo, pf, pi, plant=plant))
+    assert e / np.linalg.norm(oracle_np.bst(tomo, pf, pi)) > 1e-3
