@@ -505,11 +505,25 @@ def batch_bench(args, torch, dv, ph):
                          iterations=args.em_iters)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    branches_ms = None
+    if not args.no_batch_compare:
+        # the same batch with every stage per bucket inside its graph branch
+        ems2 = [torch.ones_like(e) for e in ems]
+        os.environ["RAFT_BATCH_QUEUE"] = "0"
+        try:
+            branches_ms = dv.batch_device(tomos, angs, pfs, pad=2, filter_type=7, store="f16", fbp_out=fouts, ys=ys,
+                                          em=ems2, iterations=args.em_iters)
+        finally:
+            del os.environ["RAFT_BATCH_QUEUE"]
+        same = all(bool(torch.equal(a, b)) for a, b in zip(ems, ems2))
+        log(f"C4 per-bucket branches: graph {branches_ms / 1e3:.2f} s (eEM outputs identical: {same})")
     n3 = float(np.sum(sizes.astype(np.float64) ** 3))
     r = {"config": f"{len(sizes)} scans, N in {{256,384,512,768,1024}}, angles = bins = N, I0 log-uniform "
                    f"[1e3, 1e5], FBP pad 2 ramp FP16 rows + {args.em_iters} eEM iterations, one graph launch",
          "scans": int(len(sizes)), "sizes_count": {int(v): int((sizes == v).sum()) for v in np.unique(sizes)},
+         "mode": "persistent work queue: one launch per stage and iteration over all scans (graph-captured)",
          "graph_seconds": ms / 1e3, "wall_seconds": wall, "scans_per_s": len(sizes) / wall,
+         "per_bucket_branches_graph_seconds": None if branches_ms is None else branches_ms / 1e3,
          "em_samples_per_s": 2.0 * n3 * args.em_iters / (ms / 1e3),
          "finite": bool(all(bool(torch.isfinite(f).all()) for f in fouts))}
     log(f"C4: {len(sizes)} scans in {wall:.2f} s (graph {ms / 1e3:.2f} s), eEM "
@@ -587,6 +601,8 @@ def main():
     ap.add_argument("--learned-slices", type=int, default=512)
     ap.add_argument("--batch-scans", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-batch-compare", action="store_true",
+                    help="skip the second C4 run with every stage per bucket (graph branches)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -663,6 +679,7 @@ def main():
         "next_rows": nxt,
         "unet": unet,
         "batch": batch,
+        "autotune": dv.autotune_choices(),
     }
     print(json.dumps(line), flush=True)
 

@@ -111,6 +111,16 @@ int raft_eem_device(int device, float* d_obj, const float* d_tomo, const float* 
 int raft_eem_device_kl(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
                        int kl_iterations, double* kl, long long* kl_excluded, void* stream);
 
+/* raft_eem_device with an iteration count per slice: iterations[s] (HOST
+ * array [nslices], non-increasing -- sort the slices by it) replaces
+ * parami[8]; iteration t updates only the slices with iterations[s] > t, and
+ * its launches cover only their slice groups, so slices that are done cost
+ * nothing.  No KL is recorded.  No counterpart in the reference (which runs
+ * every slice for the same count); for batches whose slices need different
+ * counts (warm starts, early stopping).  Synchronises before returning. */
+int raft_eem_device_sched(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                          const int* iterations, void* stream);
+
 /* FBP by the Backprojection Slice Theorem (replaces getBSTMultiGPU,
  * cuda/src/geometries/parallel/reconstruction_methods/bst.cu:674-736, bound in
  * rafttypes.py, called from sscRaft/geometries/parallel/fbp_methods.py:200):
@@ -207,6 +217,18 @@ int raft_fbp_plan_info(const float* angles, const float* paramf, const int* para
  * Returns 0 or an error code. */
 int raft_proj_plan_info(const float* angles, int nangles, int Nx, int Ny, int nrays, int nslices, float ax, float ay,
                         int em, int* ray_chunks, int* tile_consts);
+
+/* The autotuner's choices (tile variants per kernel family and geometry) as
+ * JSON text, written into buf (NUL-terminated, truncated to len); returns the
+ * length the whole text needs, NUL included.  The choices are made by the
+ * first call that needs them (each candidate timed on the call's own data)
+ * and kept for the process; RAFT_AUTOTUNE_CACHE=<file> keeps them across
+ * processes in that JSON file, RAFT_AUTOTUNE=0 turns tuning off.  No
+ * counterpart in the reference (fixed 16x16x4 blocks, configs.hpp:22-24). */
+int raft_autotune_dump(char* buf, int len);
+/* Forgets the choices held in memory (the next call re-reads the cache file,
+ * if any, and tunes what it does not hold). */
+void raft_autotune_clear(void);
 
 /* Average device time (ms) of the last launch of each kernel family on this
  * thread's last call, for roofline reporting: which = 0 filter, 1 FBP

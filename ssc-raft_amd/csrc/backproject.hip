@@ -26,6 +26,9 @@
 // em_RT_kernel.cu:249-299; kernel_backprojectionOfOnes, em_RT_kernel.cu:303-350.
 #include "raft_internal.hpp"
 
+#include <algorithm>
+#include <cstring>
+
 namespace raft {
 
 namespace {
@@ -85,19 +88,30 @@ template <> struct Acc<__half> {
     }
 };
 
-template <typename QT, int MODE>
-__global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
+// LDS of one workgroup: the staged windows of CHUNK angles and their coefficients
+template <int CHUNK> struct BpSmem {
+    uint4 win[CHUNK * BP_SLOT];
+    double cA[CHUNK], cB[CHUNK];
+    int tl[CHUNK], w[CHUNK], boff[CHUNK];
+};
+
+// One 64 x 32 pixel tile of slice group g.  Starts with a barrier (the LDS
+// may still be read by the previous tile of a persistent workgroup) and
+// touches LDS only through sm.
+template <typename QT, int MODE, int CHUNK>
+__device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSmem<CHUNK>& sm)
 {
     constexpr int SB = Acc<QT>::SB;
     constexpr int TX = BP_TILE_X, TY = BP_TILE_Y;
-    __shared__ uint4 win[BP_CHUNK * BP_SLOT];
-    __shared__ double s_cA[BP_CHUNK], s_cB[BP_CHUNK];
-    __shared__ int s_tl[BP_CHUNK], s_w[BP_CHUNK], s_boff[BP_CHUNK];
+    uint4* win = sm.win;
+    double* s_cA = sm.cA;
+    double* s_cB = sm.cB;
+    int* s_tl = sm.tl;
+    int* s_w = sm.w;
+    int* s_boff = sm.boff;
 
     const int tid = threadIdx.x;
     const int tx = tid & 15, ty = tid >> 4;
-    const int tile = blockIdx.x;
-    const int g = blockIdx.y;
     const int i0 = (tile % a.tiles_x) * TX;
     const int j0 = (tile / a.tiles_x) * TY;
 
@@ -118,8 +132,8 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
 #pragma unroll
             for (int s = 0; s < SB; ++s) tot[q][p][s] = 0.0f;
 
-    for (int a0 = 0; a0 < a.A; a0 += BP_CHUNK) {
-        const int nA = __builtin_amdgcn_readfirstlane(min(BP_CHUNK, a.A - a0)); // uniform: scalar loop counter
+    for (int a0 = 0; a0 < a.A; a0 += CHUNK) {
+        const int nA = __builtin_amdgcn_readfirstlane(min(CHUNK, a.A - a0)); // uniform: scalar loop counter
         __syncthreads();
         // ---- per-angle bin windows of this tile (one thread per angle) ----
         if (tid < nA) {
@@ -154,49 +168,56 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
         }
         __syncthreads();
         // ---- sweep ----
-        // FP32 rows: per-chunk partial sums added to the totals (two-level
-        // summation, for the 1e-5 bar); FP16 rows sum into the totals directly
-        // (their error is the FP16 storage; 64 fewer registers, occupancy 3)
+        // FP32 rows: partial sums over groups of BP_GROUP angles added to the
+        // totals (two-level summation, for the 1e-5 bar); the grouping does not
+        // depend on CHUNK (a multiple of BP_GROUP), so every CHUNK variant
+        // gives bit-identical sums.  FP16 rows sum into the totals directly
+        // (their error is the FP16 storage; 64 fewer registers, occupancy 3).
         constexpr bool TWO_LEVEL = SB == 4;
-        float part[2][4][TWO_LEVEL ? SB : 1];
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int s = 0; s < (TWO_LEVEL ? SB : 1); ++s) part[q][p][s] = 0.0f;
+        static_assert(CHUNK % BP_GROUP == 0, "the angle chunk must hold whole summation groups");
         const char* wbase = reinterpret_cast<const char*>(win);
-        // 4 angles per trip: their coefficients come from one LDS base by immediate offsets
+#pragma unroll 1
+        for (int ag = 0; ag < nA; ag += BP_GROUP) {
+            const int ng = __builtin_amdgcn_readfirstlane(min(BP_GROUP, nA - ag));
+            float part[2][4][TWO_LEVEL ? SB : 1];
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int p = 0; p < 4; ++p)
+#pragma unroll
+                    for (int s = 0; s < (TWO_LEVEL ? SB : 1); ++s) part[q][p][s] = 0.0f;
+            // 4 angles per trip: their coefficients come from one LDS base by immediate offsets
 #pragma unroll 4
-        for (int al = 0; al < nA; ++al) {
-            const double cA = s_cA[al], cB = s_cB[al];
-            // byte offset of bin 0 of this angle's window: entry (al, T - tl)
-            const unsigned boff = (unsigned)s_boff[al];
-            int T[2][4];
+            for (int al = ag; al < ag + ng; ++al) {
+                const double cA = s_cA[al], cB = s_cB[al];
+                // byte offset of bin 0 of this angle's window: entry (al, T - tl)
+                const unsigned boff = (unsigned)s_boff[al];
+                int T[2][4];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const double rt = fma(Y[q], cB, a.c0);
+                for (int q = 0; q < 2; ++q) {
+                    const double rt = fma(Y[q], cB, a.c0);
 #pragma unroll
-                for (int p = 0; p < 4; ++p) T[q][p] = (int)fma(X[p], cA, rt);
+                    for (int p = 0; p < 4; ++p) T[q][p] = (int)fma(X[p], cA, rt);
+                }
+                uint4 v[2][4];
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+                        v[q][p] = *reinterpret_cast<const uint4*>(wbase + (((unsigned)T[q][p] << 4) + boff));
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) Acc<QT>::add(TWO_LEVEL ? part[q][p] : tot[q][p], v[q][p]);
             }
-            uint4 v[2][4];
+            if constexpr (TWO_LEVEL) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+                for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int p = 0; p < 4; ++p)
-                    v[q][p] = *reinterpret_cast<const uint4*>(wbase + (((unsigned)T[q][p] << 4) + boff));
+                    for (int p = 0; p < 4; ++p)
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int p = 0; p < 4; ++p) Acc<QT>::add(TWO_LEVEL ? part[q][p] : tot[q][p], v[q][p]);
-        }
-        if constexpr (TWO_LEVEL) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int p = 0; p < 4; ++p)
-#pragma unroll
-                    for (int s = 0; s < SB; ++s) tot[q][p][s] += part[q][p][s];
+                        for (int s = 0; s < SB; ++s) tot[q][p][s] += part[q][p][s];
+            }
         }
     }
 
@@ -220,11 +241,14 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
                     const float n1 = a.norm[(size_t)j * a.n + i];
                     nv = make_float4(n1, n1, n1, n1);
                 }
+                // slices at or past S (the padding of the last group, or slices
+                // whose iterations are done in a scheduled run) keep their values
                 float4 xv = *reinterpret_cast<const float4*>(a.x + pix);
-                xv.x = xv.x * tot[q][p][0] / nv.x;
-                xv.y = xv.y * tot[q][p][1] / nv.y;
-                xv.z = xv.z * tot[q][p][2] / nv.z;
-                xv.w = xv.w * tot[q][p][3] / nv.w;
+                const int s0 = g * 4;
+                xv.x = s0 + 0 < a.S ? xv.x * tot[q][p][0] / nv.x : xv.x;
+                xv.y = s0 + 1 < a.S ? xv.y * tot[q][p][1] / nv.y : xv.y;
+                xv.z = s0 + 2 < a.S ? xv.z * tot[q][p][2] / nv.z : xv.z;
+                xv.w = s0 + 3 < a.S ? xv.w * tot[q][p][3] / nv.w : xv.w;
                 *reinterpret_cast<float4*>(a.out + pix) = xv;
                 if (a.xt) *reinterpret_cast<float4*>(a.xt + ((size_t)g * nn + (size_t)i * a.n + j) * 4) = xv;
             } else {
@@ -234,6 +258,48 @@ __global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
                     if (sl < a.S) a.out[(size_t)sl * nn + (size_t)j * a.n + i] = tot[q][p][s];
                 }
             }
+        }
+    }
+}
+
+template <typename QT, int MODE, int CHUNK>
+__global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
+{
+    __shared__ BpSmem<CHUNK> sm;
+    bp_tile<QT, MODE, CHUNK>(a, blockIdx.x, blockIdx.y, sm);
+}
+
+// Persistent work-queue form: a resident grid takes items from a device
+// counter; item i of launch segment b (items [first[b], first[b + 1])) is
+// tile (i - first[b]) % tiles of slice group (i - first[b]) / tiles of the
+// volume described by descs[b].  The host orders the segments by cost per
+// item (angles), largest first.  The last workgroup to leave clears the
+// counters for the next launch.
+template <typename QT, int MODE, int CHUNK>
+__global__ __launch_bounds__(256) void k_backproject_queue(const BpKArgs* __restrict__ descs, BpQueue qu)
+{
+    __shared__ BpSmem<CHUNK> sm;
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads(); // every thread has read the previous item
+        if (threadIdx.x == 0)
+            s_item = __hip_atomic_fetch_add(qu.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= qu.first[qu.nseg]) break;
+        int b = 0;
+        while (b + 1 < qu.nseg && item >= qu.first[b + 1]) ++b;
+        b = __builtin_amdgcn_readfirstlane(b);
+        const int local = item - qu.first[b];
+        const int tiles = qu.tiles[b];
+        bp_tile<QT, MODE, CHUNK>(descs[b], local % tiles, local / tiles, sm);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = __hip_atomic_fetch_add(qu.counter + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == (int)gridDim.x - 1) {
+            __hip_atomic_store(qu.counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(qu.counter + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -331,30 +397,98 @@ __global__ __launch_bounds__(256) void k_layout(const float* in, float* out, int
 
 } // namespace
 
-void launch_backproject(const BpArgs& b, int store, hipStream_t st)
+// the staged rows a launch over b.S slices reads must lie in the buffer behind q
+static void bp_check_rows(const BpArgs& b, int store)
 {
-    if (b.S <= 0 || b.n <= 0) return;
+    const int sb = (store == RAFT_STORE_F16 && b.mode == 0) ? 8 : 4;
+    const size_t need = (size_t)((b.S + sb - 1) / sb) * (size_t)b.A * (size_t)b.W * 16;
+    if (b.q_bytes && need > b.q_bytes)
+        fail(RAFT_EINVAL, "internal: backprojection would read " + std::to_string(need) + " bytes of staged rows from a " +
+                              std::to_string(b.q_bytes) + "-byte buffer");
+}
+
+static BpKArgs bp_kargs(const BpArgs& b)
+{
     BpKArgs k;
     k.q = static_cast<const uint4*>(b.q);
     k.out = b.out; k.x = b.x; k.norm = b.norm; k.xt = b.xt; k.norm_il = b.norm_il;
     k.cA = b.cA; k.cB = b.cB; k.xc = b.xc; k.c0 = b.c0;
     k.n = b.n; k.A = b.A; k.tlo = b.tlo; k.W = b.W; k.S = b.S;
     k.tiles_x = (b.n + BP_TILE_X - 1) / BP_TILE_X;
-    const int tiles_y = (b.n + BP_TILE_Y - 1) / BP_TILE_Y;
+    return k;
+}
+
+int bp_tiles(int n) { return ((n + BP_TILE_X - 1) / BP_TILE_X) * ((n + BP_TILE_Y - 1) / BP_TILE_Y); }
+
+size_t bp_desc_size() { return sizeof(BpKArgs); }
+
+void bp_desc_fill(const BpArgs& b, int store, void* dst)
+{
+    bp_check_rows(b, store);
+    const BpKArgs k = bp_kargs(b);
+    std::memcpy(dst, &k, sizeof k);
+}
+
+// kernel variant: (FP32 rows, store) x (FBP sums, eEM update) x angles per LDS round
+template <int CHUNK> static const void* bp_kernel(int store, int mode, bool queue)
+{
+    if (mode == 1)
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK>)
+                     : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK>);
+    if (store == RAFT_STORE_F16)
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK>)
+                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK>);
+    return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK>)
+                 : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK>);
+}
+
+static const void* bp_kernel_any(int store, int mode, int chunk, bool queue)
+{
+    if (mode == 1 && store != RAFT_STORE_F32) fail(RAFT_EINVAL, "eEM update runs on FP32 rows");
+    switch (chunk) {
+    case 16: return bp_kernel<16>(store, mode, queue);
+    case 48: return bp_kernel<48>(store, mode, queue);
+    case 0:
+    case 32: return bp_kernel<32>(store, mode, queue);
+    default: fail(RAFT_EINVAL, "internal: unsupported backprojection angle chunk " + std::to_string(chunk));
+    }
+}
+
+bool bp_chunk_supported(int chunk) { return chunk == 16 || chunk == 32 || chunk == 48; }
+
+void launch_backproject(const BpArgs& b, int store, hipStream_t st)
+{
+    if (b.S <= 0 || b.n <= 0) return;
+    bp_check_rows(b, store);
+    BpKArgs k = bp_kargs(b);
     const int sb = (store == RAFT_STORE_F16) ? 8 : 4;
     const int groups = (b.S + sb - 1) / sb;
     if (groups > 65535) fail(RAFT_EUNSUPPORTED, "too many slice groups for one backprojection launch");
-    dim3 grid(k.tiles_x * tiles_y, groups);
-    dim3 block(256);
-    if (b.mode == 1) {
-        if (store != RAFT_STORE_F32) fail(RAFT_EINVAL, "eEM update runs on FP32 rows");
-        hipLaunchKernelGGL((k_backproject<float, 1>), grid, block, 0, st, k);
-    } else if (store == RAFT_STORE_F16) {
-        hipLaunchKernelGGL((k_backproject<__half, 0>), grid, block, 0, st, k);
-    } else {
-        hipLaunchKernelGGL((k_backproject<float, 0>), grid, block, 0, st, k);
-    }
-    RAFT_HIP(hipGetLastError());
+    const void* fn = bp_kernel_any(store, b.mode, b.chunk, false);
+    void* args[] = {&k};
+    RAFT_HIP(hipLaunchKernel(fn, dim3(bp_tiles(b.n), groups), dim3(256), args, 0, st));
+}
+
+int bp_queue_grid(int store, int mode, int chunk)
+{
+    const void* fn = bp_kernel_any(store, mode, chunk, true);
+    int dev = 0, cus = 0, per = 0;
+    RAFT_HIP(hipGetDevice(&dev));
+    RAFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RAFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0));
+    return std::max(1, cus * std::max(1, per));
+}
+
+void launch_backproject_queue(const void* d_descs, const BpQueue& q, int store, int mode, int chunk, int grid,
+                              hipStream_t st)
+{
+    if (q.nseg <= 0 || q.first[q.nseg] <= 0) return;
+    if (q.nseg > QUEUE_MAXSEG) fail(RAFT_EINVAL, "internal: too many queue segments");
+    const void* fn = bp_kernel_any(store, mode, chunk, true);
+    BpQueue qq = q;
+    const BpKArgs* descs = static_cast<const BpKArgs*>(d_descs);
+    void* args[] = {&descs, &qq};
+    RAFT_HIP(hipLaunchKernel(fn, dim3(std::max(1, std::min(grid, q.first[q.nseg]))), dim3(256), args, 0, st));
 }
 
 void launch_rows_from_planar(float* q, const float* data, const float* wa, int S, int A, int W, int tlo, int nbins,

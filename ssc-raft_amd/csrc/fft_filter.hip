@@ -22,6 +22,8 @@
 // Filter::apply (filters.cu:395-438).
 #include "raft_internal.hpp"
 
+#include <cstdlib>
+
 namespace raft {
 
 namespace {
@@ -307,6 +309,75 @@ __global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
     store_pair<OT, THREADS>(a, ix, z, -1.0f);
 }
 
+// Whole-entry form: one workgroup filters every slice pair of one (slice
+// group, angle) in turn through the same LDS and stores full 16-byte entries
+// of the interleaved rows (SB slices per bin), so no 64-byte line is written
+// in halves by two workgroups.  The filtered window of the earlier pairs
+// waits in registers: J window bins per thread (the host picks this form
+// when W <= J * THREADS).  The arithmetic of each pair is that of
+// k_filter_fft, so both forms store the same values.
+template <typename OT, int THREADS, int E, int J>
+__global__ __launch_bounds__(THREADS) void k_filter_fft_entry(FilterK a, FftK p)
+{
+    constexpr int SB = sizeof(OT) == 4 ? 4 : 8;
+    constexpr int PAIRS = SB / 2;
+    extern __shared__ float2 z[];
+    float2* tws = z + lds_elems(a.npad);
+    const int ntw = 128 + a.npad / 128 + 1;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
+    const Tw tw{tws, tws + 128};
+    const int ang = blockIdx.x % a.A, g = blockIdx.x / a.A;
+    const float wv = a.wa[ang];
+    const float sign1 = a.none ? 1.0f : -1.0f;
+    float2 hold[PAIRS][J];
+#pragma unroll
+    for (int pr = 0; pr < PAIRS; ++pr) {
+        const PairIdx ix{g, ang, g * SB + 2 * pr};
+        __syncthreads(); // the twiddles are in; the previous pair's window reads are done
+        load_pair<THREADS>(a, ix, z);
+        __syncthreads();
+        if (!a.none) {
+            fft_run<THREADS, E>(z, tw, p);
+            for (int k = threadIdx.x; k < a.npad; k += THREADS) {
+                const float2 v = cmul(z[pix(k)], a.H[k]);
+                z[pix(k)] = make_float2(v.x, -v.y);
+            }
+            __syncthreads();
+            fft_run<THREADS, E>(z, tw, p);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int T = a.tlo + (int)threadIdx.x + j * THREADS;
+            float2 v = make_float2(0.0f, 0.0f);
+            if ((int)threadIdx.x + j * THREADS < a.W && T >= 0 && T < a.npad) {
+                const float2 zv = z[pix(T)];
+                v = make_float2(zv.x * wv, sign1 * zv.y * wv);
+            }
+            hold[pr][j] = v;
+        }
+    }
+    uint4* dst = reinterpret_cast<uint4*>(static_cast<OT*>(a.q)) + ((size_t)g * a.A + ang) * a.W;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int w = (int)threadIdx.x + j * THREADS;
+        if (w >= a.W) continue;
+        uint4 e;
+        if constexpr (SB == 4) {
+            e = make_uint4(__float_as_uint(hold[0][j].x), __float_as_uint(hold[0][j].y), __float_as_uint(hold[1][j].x),
+                           __float_as_uint(hold[1][j].y));
+        } else {
+            unsigned u[PAIRS];
+#pragma unroll
+            for (int q = 0; q < PAIRS; ++q) {
+                const __half2 h = __floats2half2_rn(hold[q][j].x, hold[q][j].y);
+                u[q] = *reinterpret_cast<const unsigned*>(&h);
+            }
+            e = make_uint4(u[0], u[1], u[2], u[3]);
+        }
+        dst[w] = e;
+    }
+}
+
 // direct circular convolution for sizes the LDS FFT does not cover:
 // q[m] = sum_j g[(m - j) mod n] u[j]
 template <typename OT>
@@ -347,6 +418,15 @@ __global__ __launch_bounds__(256) void k_filter_direct(FilterK a)
 template <typename OT, int THREADS, int E>
 void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
 {
+    // whole-entry form when the window fits 4 bins per thread (every padding
+    // of 2 or more at the benchmark sizes); blocks = (groups x angles) x pairs
+    if (k.W <= 4 * THREADS && !std::getenv("RAFT_FILTER_PAIRS")) {
+        const void* fn = reinterpret_cast<const void*>(&k_filter_fft_entry<OT, THREADS, E, 4>);
+        RAFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_filter_fft_entry<OT, THREADS, E, 4>), dim3(blocks / (k.SB / 2)), dim3(THREADS), lds, st,
+                           k, p);
+        return;
+    }
     RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_filter_fft<OT, THREADS, E>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_filter_fft<OT, THREADS, E>), dim3(blocks), dim3(THREADS), lds, st, k, p);

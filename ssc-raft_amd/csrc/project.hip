@@ -58,6 +58,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 namespace raft {
 
@@ -346,22 +347,31 @@ __device__ __forceinline__ unsigned tile_off(int R, int C, int pitch16, int moff
     return e;
 }
 
+// LDS of one staged-form workgroup
+struct ProjSmem {
+    float4 tile[PL_CAP];
+    int4 ubox[PL_BATCH];
+    int kr[PROJ_ANGLES][2];
+};
+
+// One workgroup's work of the staged form: angle chunk ab (angles
+// [8 ab, 8 ab + 8)), ray chunk rc (ray blocks rc, rc + RC, ...), slice group
+// g.  Every LDS write is preceded by a barrier, so a persistent workgroup can
+// run items back to back.
 template <int EM>
-__global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
+__device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, int g, ProjSmem& sm)
 {
-    __shared__ float4 tile[PL_CAP];
-    __shared__ int4 s_ubox[PL_BATCH];
-    __shared__ int s_kr[PROJ_ANGLES][2];
+    float4* tile = sm.tile;
+    int4* s_ubox = sm.ubox;
+    int(*s_kr)[2] = sm.kr;
     // per-wave boxes of a batch of steps, computed while no step is staged
     int4* s_wbox = reinterpret_cast<int4*>(tile);
     static_assert(PROJ_ANGLES * PL_BATCH * sizeof(int4) <= sizeof(float4) * PL_CAP, "box scratch exceeds the tile");
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int a0 = blockIdx.x * PROJ_ANGLES;
+    const int a0 = ab * PROJ_ANGLES;
     const int ang = a0 + wave;
     const bool have = ang < a.nangles;
-    const int rc = blockIdx.y;
-    const int g = blockIdx.z;
     // staged orientation, chosen once per workgroup from its middle angle:
     // rows = Y, cols = X (image [Y][X]), or rows = X, cols = Y (transpose)
     const int amid = min(a0 + PROJ_ANGLES / 2, a.nangles - 1);
@@ -539,6 +549,46 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
     if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc);
 }
 
+template <int EM>
+__global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
+{
+    __shared__ ProjSmem sm;
+    proj_lds_item<EM>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+}
+
+// Persistent work-queue form of the staged projector over several volumes:
+// item i of segment b is (angle chunk, ray chunk, slice group) =
+// (l % nab, (l / nab) % RC, l / (nab RC)), l = i - first[b], of descs[b].
+template <int EM>
+__global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds_queue(const ProjK* __restrict__ descs,
+                                                                            ProjQueue qu)
+{
+    __shared__ ProjSmem sm;
+    __shared__ int s_item;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_item = __hip_atomic_fetch_add(qu.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= qu.first[qu.nseg]) break;
+        int b = 0;
+        while (b + 1 < qu.nseg && item >= qu.first[b + 1]) ++b;
+        b = __builtin_amdgcn_readfirstlane(b);
+        const int local = item - qu.first[b];
+        const int nab = qu.nab[b], RC = descs[b].RC;
+        proj_lds_item<EM>(descs[b], local % nab, (local / nab) % RC, local / (nab * RC), sm);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = __hip_atomic_fetch_add(qu.counter + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == (int)gridDim.x - 1) {
+            __hip_atomic_store(qu.counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(qu.counter + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // kl[s*K + iter] = sum of the slice's partials, fixed order
 __global__ __launch_bounds__(256) void k_kl_reduce(const double* part, const unsigned long long* expart, int nb,
                                                    double* kl, unsigned long long* exo, int K, int iter,
@@ -578,9 +628,8 @@ __global__ void k_counter(int* c, int op)
 
 } // namespace
 
-void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
+static ProjK proj_kargs(const ProjArgs& p)
 {
-    if (p.S <= 0 || p.g.nrays <= 0 || p.g.nangles <= 0) return;
     ProjK k;
     k.img = reinterpret_cast<const float4*>(p.img);
     k.imgT = reinterpret_cast<const float4*>(p.imgT);
@@ -592,6 +641,51 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     k.tlo = p.tlo; k.W = p.W;
     k.RC = p.lds_rc > 0 ? p.lds_rc : 1;
     k.kl_stride = p.g.nangles * k.RC;
+    return k;
+}
+
+size_t proj_desc_size() { return sizeof(ProjK); }
+
+void proj_desc_fill(const ProjArgs& p, void* dst)
+{
+    if (p.lds_rc <= 0) fail(RAFT_EINVAL, "internal: the projector queue runs the staged form only");
+    const ProjK k = proj_kargs(p);
+    std::memcpy(dst, &k, sizeof k);
+}
+
+int proj_angle_chunks(int nangles) { return (nangles + PROJ_ANGLES - 1) / PROJ_ANGLES; }
+
+static const void* proj_queue_kernel(int em_mode)
+{
+    if (em_mode == 2) return reinterpret_cast<const void*>(&k_project_lds_queue<2>);
+    if (em_mode == 1) return reinterpret_cast<const void*>(&k_project_lds_queue<1>);
+    return reinterpret_cast<const void*>(&k_project_lds_queue<0>);
+}
+
+int proj_queue_grid(int em_mode)
+{
+    int dev = 0, cus = 0, per = 0;
+    RAFT_HIP(hipGetDevice(&dev));
+    RAFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RAFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, proj_queue_kernel(em_mode), 64 * PROJ_ANGLES, 0));
+    return std::max(1, cus * std::max(1, per));
+}
+
+void launch_project_queue(const void* d_descs, const ProjQueue& q, int em_mode, int grid, hipStream_t st)
+{
+    if (q.nseg <= 0 || q.first[q.nseg] <= 0) return;
+    if (q.nseg > QUEUE_MAXSEG) fail(RAFT_EINVAL, "internal: too many queue segments");
+    ProjQueue qq = q;
+    const ProjK* descs = static_cast<const ProjK*>(d_descs);
+    void* args[] = {&descs, &qq};
+    RAFT_HIP(hipLaunchKernel(proj_queue_kernel(em_mode), dim3(std::max(1, std::min(grid, q.first[q.nseg]))),
+                             dim3(64 * PROJ_ANGLES), args, 0, st));
+}
+
+void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
+{
+    if (p.S <= 0 || p.g.nrays <= 0 || p.g.nangles <= 0) return;
+    ProjK k = proj_kargs(p);
     const int G = (p.S + 3) / 4;
     if (G > 65535) fail(RAFT_EUNSUPPORTED, "projector grid too large");
     const int nab = (p.g.nangles + PROJ_ANGLES - 1) / PROJ_ANGLES;

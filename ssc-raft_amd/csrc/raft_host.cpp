@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -188,6 +189,34 @@ RayGeom ray_geom_em(int nrays, int nangles, int n)
 }
 
 // ---------------------------------------------------------------------------
+// autotuning keys and candidates
+// ---------------------------------------------------------------------------
+static std::string bp_tune_key(int n, int A, int W, int store, int mode, int groups)
+{
+    char k[160];
+    snprintf(k, sizeof k, "bp/n=%d/A=%d/W=%d/store=%d/mode=%d/groups=%d", n, A, W, store, mode, groups);
+    return k;
+}
+static const std::vector<int> kBpChunks = {BP_CHUNK, 16, 48};
+
+static std::string proj_tune_key(const RayGeom& g, int em, int groups)
+{
+    char k[200];
+    snprintf(k, sizeof k, "proj/Nx=%d/Ny=%d/nrays=%d/A=%d/em=%d/groups=%d/ry=%.6f", g.Nx, g.Ny, g.nrays, g.nangles, em,
+             groups, g.ry);
+    return k;
+}
+// ray-chunk counts of the staged projector: the plan's, then powers of two up to the ray blocks
+static std::vector<int> proj_rc_cands(const RayGeom& g, int planned)
+{
+    std::vector<int> c = {planned};
+    const int nrb = (g.nrays + 31) / 32;
+    for (int r = 1; r <= nrb && r <= 32; r *= 2)
+        if (r != planned) c.push_back(r);
+    return c;
+}
+
+// ---------------------------------------------------------------------------
 // FBP parameters (parameters.cu:131-219, Filter::apply filters.cu:395-438)
 // ---------------------------------------------------------------------------
 struct FbpCfg {
@@ -256,12 +285,44 @@ struct FilterTables {
     FftPlan plan;
 };
 
+// page-locked host buffer (hipHostMalloc), grow-only
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void* ensure(size_t n)
+    {
+        if (bytes >= n) return p;
+        release();
+        void* q = nullptr;
+        const hipError_t e = hipHostMalloc(&q, n, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(RAFT_EHIP, "hipHostMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e));
+        }
+        p = q;
+        bytes = n;
+        return p;
+    }
+};
+
 struct Device {
-    static constexpr int kSlots = 8;
+    static constexpr int kSlots = 12;
     std::mutex mu;
     std::map<std::string, std::unique_ptr<FilterTables>> filters;
     DevBuf ws[kSlots];
     DevBuf tab; // per-call tables (see TableSet)
+    PinnedBuf pin[4];                     // host-pointer entry points: staging of two chunks in and out
+    std::vector<std::vector<char>> keep;  // host bytes of table uploads in flight (TableSet::keep_alive)
     // the slot is a template argument so that every index is checked at compile time
     template <int SLOT> void* workspace(size_t bytes)
     {
@@ -310,6 +371,10 @@ class TableSet {
         if (!host_.empty())
             RAFT_HIP(hipMemcpyAsync(base_, host_.data(), host_.size(), hipMemcpyHostToDevice, st));
     }
+    // for calls that return before their stream is synchronised: the host
+    // bytes of the upload stay alive in the device state until the caller
+    // synchronises and clears it
+    void keep_alive() { D_.keep.push_back(std::move(host_)); }
     char* base() const
     {
         if (!base_) fail(RAFT_EINVAL, "internal: table set used before upload");
@@ -363,16 +428,17 @@ template <typename T> static void upload(DevBuf& b, const std::vector<T>& v, hip
     RAFT_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
 }
 
-// filter spectrum, twiddles and (for sizes the LDS FFT does not take) the
-// real convolution kernel; cached per (n, filter, reg, paganin, offset, pixel)
-static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
+// host copies of the filter tables: the spectrum H (filter, offset phase and
+// 1/(npad*pixel)), the FFT twiddles and, for sizes the LDS FFT does not take,
+// the real convolution kernel g
+struct FilterHost {
+    std::vector<float2> H, tw;
+    std::vector<double> g;
+    FftPlan plan;
+};
+
+static void filter_host_tables(const FbpCfg& c, FilterHost& F)
 {
-    char key[256];
-    snprintf(key, sizeof key, "%d/%d/%.17g/%.17g/%.17g/%.17g", c.npad, c.filter, c.reg, c.paganin, c.offset,
-             c.pixel);
-    auto it = D.filters.find(key);
-    if (it != D.filters.end()) return *it->second;
-    std::unique_ptr<FilterTables> F(new FilterTables());
     const int n = c.npad, M = c.M;
     std::vector<double> Hr(n), Hi(n);
     const double scale = 1.0 / ((double)n * c.pixel);
@@ -388,25 +454,24 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
         Hi[k] = -Hi[n - k];
     }
     if (n % 2 == 0) Hi[n / 2] = 0.0;
-    std::vector<float2> H(n);
-    for (int k = 0; k < n; ++k) H[k] = make_float2((float)Hr[k], (float)Hi[k]);
+    F.H.resize(n);
+    for (int k = 0; k < n; ++k) F.H[k] = make_float2((float)Hr[k], (float)Hi[k]);
     // two-level twiddle tables: W^m = hi[m >> 7] * lo[m & 127], W = exp(-2 pi i / n)
     const int nhi = n / 128 + 1;
-    std::vector<float2> tw(128 + nhi);
+    F.tw.resize(128 + nhi);
     for (int k = 0; k < 128; ++k) {
         const double ang = -2.0 * M_PI * (double)k / (double)n;
-        tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        F.tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     for (int k = 0; k < nhi; ++k) {
         const double ang = -2.0 * M_PI * (double)(((long long)k * 128) % n) / (double)n;
-        tw[128 + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        F.tw[128 + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
-    F->plan = fft_plan(n);
-    upload(F->H, H, st);
-    upload(F->tw, tw, st);
-    if (!F->plan.ok && c.filter != 0) {
+    F.plan = fft_plan(n);
+    F.g.clear();
+    if (!F.plan.ok && c.filter != 0) {
         // g[d] = sum_k H_k exp(2 pi i k d / n): real because H is Hermitian
-        std::vector<double> g(n);
+        F.g.resize(n);
         for (int d = 0; d < n; ++d) {
             double acc = 0.0;
             for (int k = 0; k < n; ++k) {
@@ -414,10 +479,26 @@ static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
                 const double ang = 2.0 * M_PI * (double)kd / (double)n;
                 acc += Hr[k] * std::cos(ang) - Hi[k] * std::sin(ang);
             }
-            g[d] = acc;
+            F.g[d] = acc;
         }
-        upload(F->g, g, st);
     }
+}
+
+// the filter tables on the device, cached per (n, filter, reg, paganin, offset, pixel)
+static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
+{
+    char key[256];
+    snprintf(key, sizeof key, "%d/%d/%.17g/%.17g/%.17g/%.17g", c.npad, c.filter, c.reg, c.paganin, c.offset,
+             c.pixel);
+    auto it = D.filters.find(key);
+    if (it != D.filters.end()) return *it->second;
+    std::unique_ptr<FilterTables> F(new FilterTables());
+    FilterHost h;
+    filter_host_tables(c, h);
+    F->plan = h.plan;
+    upload(F->H, h.H, st);
+    upload(F->tw, h.tw, st);
+    if (!h.g.empty()) upload(F->g, h.g, st);
     RAFT_HIP(hipStreamSynchronize(st));
     auto& ref = *F;
     D.filters[key] = std::move(F);
@@ -449,18 +530,12 @@ static void fbp_geometry(const FbpCfg& c, const float* angles, BpGeom& g, std::v
     bp_window(g);
 }
 
-static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
-                       const int* parami, int store, hipStream_t st)
+// FBP work of one call on stream st; the caller holds D.mu with the device
+// bound.  sync: wait for the stream before returning (the per-call tables in
+// the arena are then free for the next call).
+static void fbp_run(Device& D, float* d_obj, const float* d_tomo, const float* angles, const FbpCfg& c, int store,
+                    hipStream_t st, bool sync)
 {
-    const FbpCfg c = decode_fbp(paramf, parami);
-    if (!angles) fail(RAFT_EINVAL, "null angles");
-    if (store != RAFT_STORE_F32 && store != RAFT_STORE_F16) fail(RAFT_EINVAL, "store must be F32 or F16");
-    if (c.S == 0) return;
-    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
-    bind_device(dev);
-    Device& D = device_state(dev);
-    std::lock_guard<std::mutex> lk(D.mu);
-
     const int A = c.A, N = c.N;
     BpGeom g;
     std::vector<float> wa;
@@ -470,6 +545,7 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
     TableSet tb(D);
     const TabRef dcA = tb.add(g.cA), dcB = tb.add(g.cB), dwa = tb.add(wa);
     tb.upload(st);
+    if (!sync) tb.keep_alive();
 
     const int SB = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
     const size_t per_group = (size_t)A * g.width * 16; // SB values per bin, 16 bytes
@@ -503,13 +579,36 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
         ba.cB = dcB.as<double>();
         ba.xc = g.xc; ba.c0 = g.c0;
         ba.n = N; ba.A = A; ba.S = nb; ba.tlo = g.tlo; ba.W = g.width; ba.mode = 0;
+        ba.q_bytes = (size_t)((B + SB - 1) / SB) * per_group;
+        // angle chunk: tuned on this block's rows (the timed launches write
+        // the same output as the final one)
+        ba.chunk = tune_pick(bp_tune_key(N, A, g.width, store, 0, (nb + SB - 1) / SB), kBpChunks, st,
+                             [&](int c) {
+                                 BpArgs t = ba;
+                                 t.chunk = c;
+                                 launch_backproject(t, store, st);
+                             });
         {
             ProfScope ps(KF_BP, st);
             launch_backproject(ba, store, st);
         }
     }
     // the small tables above are freed when this call returns
-    RAFT_HIP(hipStreamSynchronize(st));
+    if (sync) RAFT_HIP(hipStreamSynchronize(st));
+}
+
+static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                       const int* parami, int store, hipStream_t st)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    if (store != RAFT_STORE_F32 && store != RAFT_STORE_F16) fail(RAFT_EINVAL, "store must be F32 or F16");
+    if (c.S == 0) return;
+    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+    fbp_run(D, d_obj, d_tomo, angles, c, store, st, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,6 +766,12 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
         p.y = nullptr; p.wa = nullptr; p.kl_part = nullptr; p.ex_part = nullptr;
         p.cosv = dc.as<double>(); p.sinv = ds.as<double>();
         p.g = g; p.S = nb; p.tlo = 0; p.W = nrays; p.lds_rc = prc;
+        if (prc > 0)
+            p.lds_rc = tune_pick(proj_tune_key(g, 0, (nb + 3) / 4), proj_rc_cands(g, prc), st, [&](int c) {
+                ProjArgs t = p;
+                t.lds_rc = c;
+                launch_project(t, 0, st);
+            });
         ProfScope ps(KF_PROJ, st);
         launch_project(p, 0, st);
     }
@@ -679,11 +784,22 @@ static void radon_device(int dev, float* d_proj, const float* d_obj, const float
 // kl_only: KL(y || A x) of the given images (one projection, x unchanged), kl[S]
 // kl_iters >= 0: the KL of the first kl_iters iterations only, kl[S][kl_iters]
 // (the others run without the FP64 terms); < 0: of every iteration, kl[S][K]
+// sched (host, [S], non-increasing): slice s runs sched[s] iterations (the
+// iteration count of parami is ignored; no KL); iteration t updates only the
+// slices with sched[s] > t, and the launches cover only their slice groups
 static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
-                       double* kl, long long* kl_ex, hipStream_t st, bool kl_only = false, int kl_iters = -1)
+                       double* kl, long long* kl_ex, hipStream_t st, bool kl_only = false, int kl_iters = -1,
+                       const int* sched = nullptr)
 {
     if (!parami) fail(RAFT_EINVAL, "null parameter array");
-    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3], K = kl_only ? 1 : parami[8];
+    if (sched && parami[2] > 0) {
+        for (int s = 0; s < parami[2]; ++s)
+            if (sched[s] < 0 || (s > 0 && sched[s] > sched[s - 1]))
+                fail(RAFT_EINVAL, "the per-slice iteration counts must be >= 0 and non-increasing");
+        if (kl || kl_ex || kl_only) fail(RAFT_EINVAL, "a scheduled eEM run records no KL");
+    }
+    const int R = parami[0], A = parami[1], S = parami[2], n = parami[3];
+    const int K = kl_only ? 1 : (sched ? (S > 0 ? sched[0] : 0) : parami[8]);
     if (kl_only && !kl && !kl_ex) fail(RAFT_EINVAL, "no KL output");
     int blocksize = parami[9];
     if (R < 2) fail(RAFT_EINVAL, "nrays must be >= 2");
@@ -744,10 +860,17 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     float* xT = static_cast<float*>(D.workspace<2>((size_t)Gb * img * 16));
     float* ones = static_cast<float*>(D.workspace<3>(img * 4 + (size_t)A * g.width * 16));
     float* onerows = ones + img;
-    const int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
-    const int kslots = proj_kl_slots(A, prc);
-    double* part = static_cast<double*>(D.workspace<4>((size_t)B * kslots * 16));
-    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * kslots);
+    // staged-projector ray chunks: the plan's, or the autotuner's choice among
+    // powers of two (made on the first block, below); the KL partial slots
+    // are sized for the largest candidate
+    int prc = proj_plan(rg, cv.data(), sv.data(), B, true);
+    const std::vector<int> rc_cands = prc > 0 ? proj_rc_cands(rg, prc) : std::vector<int>{prc};
+    int kslots = proj_kl_slots(A, prc);
+    size_t kslots_cap = (size_t)kslots;
+    for (int c : rc_cands) kslots_cap = std::max(kslots_cap, (size_t)proj_kl_slots(A, c));
+    double* part = static_cast<double*>(D.workspace<4>((size_t)B * kslots_cap * 16));
+    unsigned long long* expart = reinterpret_cast<unsigned long long*>(part + (size_t)B * kslots_cap);
+    int bp_chunk = BP_CHUNK;
     const bool want_kl = (kl != nullptr) || (kl_ex != nullptr);
     // iterations whose KL is recorded (the first Kkl), the row stride of kl
     const int Kkl = want_kl ? (kl_iters < 0 ? K : std::min(kl_iters, K)) : 0;
@@ -766,6 +889,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
         b1.cA = dcA.as<double>(); b1.cB = dcB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
         b1.n = n; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 0;
+        b1.q_bytes = (size_t)A * g.width * 16;
         launch_backproject(b1, RAFT_STORE_F32, st);
     }
     // ratio rows: margins stay zero, the projector writes [0, R) (none for the KL alone)
@@ -790,12 +914,44 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         BpArgs b;
         b.q = rows; b.out = xil; b.x = xil; b.norm = ones; b.xt = xT;
         b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
-        b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
+        b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1; b.chunk = bp_chunk;
+        b.q_bytes = (size_t)Gb * A * g.width * 16;
         {
             ProfScope ps(KF_EMBP, s);
             launch_backproject(b, RAFT_STORE_F32, s);
         }
         if (graph) launch_counter(counter, 1, s);
+    };
+    // tile variants, timed on the first block's data: the projector writes
+    // only ratio rows (rewritten by the first iteration), the update reads
+    // those rows (nb slices, all their groups) and writes a scratch image
+    auto tune = [&](int nb, const float* y) {
+        if (prc > 0) {
+            ProjArgs tp;
+            tp.img = xil; tp.imgT = xT; tp.out = kl_only ? nullptr : rows; tp.y = y; tp.wa = dwa.as<float>();
+            tp.kl_part = nullptr; tp.ex_part = nullptr;
+            tp.cosv = dc.as<double>(); tp.sinv = ds.as<double>();
+            tp.g = rg; tp.S = nb; tp.tlo = g.tlo; tp.W = g.width;
+            prc = tune_pick(proj_tune_key(rg, 1, (nb + 3) / 4), rc_cands, st, [&](int c) {
+                tp.lds_rc = c;
+                launch_project(tp, 1, st);
+            });
+            kslots = proj_kl_slots(A, prc);
+        }
+        if (kl_only) return;
+        const std::string key = bp_tune_key(n, A, g.width, RAFT_STORE_F32, 1, (nb + 3) / 4);
+        float* tmp = nullptr;
+        if (tune_enabled() && tune_cached(key, -1) < 0)
+            tmp = static_cast<float*>(D.workspace<7>((size_t)((nb + 3) / 4) * img * 16));
+        BpArgs tb;
+        tb.q = rows; tb.out = tmp; tb.x = xil; tb.norm = ones; tb.xt = nullptr;
+        tb.cA = dcA.as<double>(); tb.cB = dcB.as<double>(); tb.xc = g.xc; tb.c0 = g.c0;
+        tb.n = n; tb.A = A; tb.S = nb; tb.tlo = g.tlo; tb.W = g.width; tb.mode = 1;
+        tb.q_bytes = (size_t)Gb * A * g.width * 16;
+        bp_chunk = tune_pick(key, kBpChunks, st, [&](int c) {
+            tb.chunk = c;
+            launch_backproject(tb, RAFT_STORE_F32, st);
+        });
     };
     // Iterations are replayed from a captured hipGraph unless per-kernel timing is on
     // (RAFT_EM_GRAPH=0 disables it).  The graph holds kernels only: the counter it
@@ -806,7 +962,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // are at least two of them
     const bool kl_rest = Kkl > 0 && Kkl == K;
     const int it0 = kl_rest ? 0 : Kkl;
-    const bool use_graph = !kl_only && K - it0 >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
+    const bool use_graph = !kl_only && !sched && K - it0 >= 2 && !prof_enabled() && !(genv && genv[0] == '0');
 
     for (int s0 = 0; s0 < S; s0 += B) {
         const int nb = std::min(B, S - s0);
@@ -816,6 +972,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         unsigned long long* exo = Kkl > 0 ? dex + (size_t)s0 * Kkl : nullptr;
         launch_to_interleaved(x, xil, nb, n, n, false, st);
         launch_to_interleaved(x, xT, nb, n, n, true, st);
+        if (s0 == 0) tune(nb, y);
         for (int it = 0; it < it0; ++it) iteration(it, nb, y, klo, exo, false, st, true);
         if (use_graph) {
             launch_counter(counter, 0, st);
@@ -842,6 +999,14 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
             if (graph) (void)hipGraphDestroy(graph);
             (void)hipStreamDestroy(cs);
             RAFT_HIP(e);
+        } else if (sched) {
+            // slices still running at iteration it: a prefix of the block
+            int active = nb;
+            for (int it = 0; it < K; ++it) {
+                while (active > 0 && sched[s0 + active - 1] <= it) --active;
+                if (active == 0) break;
+                iteration(it, active, y, klo, exo, false, st, false);
+            }
         } else {
             for (int it = it0; it < K; ++it) iteration(it, nb, y, klo, exo, false, st, kl_rest);
         }
@@ -937,6 +1102,7 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
         b0.q = rows; b0.out = tmp; b0.x = nullptr; b0.norm = nullptr; b0.xt = nullptr;
         b0.cA = dcA.as<double>(); b0.cB = dcB.as<double>(); b0.xc = g.xc; b0.c0 = g.c0;
         b0.n = n; b0.A = A; b0.S = nb; b0.tlo = g.tlo; b0.W = g.width; b0.mode = 0;
+        b0.q_bytes = (size_t)Gb * A * g.width * 16;
         launch_backproject(b0, RAFT_STORE_F32, st);
         launch_to_interleaved(tmp, normil, nb, n, n, false, st);
         // the rows' margins (bins outside [0, R)) are zero from here on: the
@@ -957,6 +1123,7 @@ static void tem_device(int dev, float* d_obj, const float* d_count, const float*
             b.q = rows; b.out = xil; b.x = xil; b.norm = normil; b.norm_il = 1; b.xt = xT;
             b.cA = dcA.as<double>(); b.cB = dcB.as<double>(); b.xc = g.xc; b.c0 = g.c0;
             b.n = n; b.A = A; b.S = nb; b.tlo = g.tlo; b.W = g.width; b.mode = 1;
+            b.q_bytes = (size_t)Gb * A * g.width * 16;
             {
                 ProfScope ps(KF_EMBP, st);
                 launch_backproject(b, RAFT_STORE_F32, st);
@@ -1095,8 +1262,40 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
     }
     RAFT_HIP(hipStreamSynchronize(st));
 
-    auto enqueue = [&](BatchBucket* b) {
-        hipStream_t s = b->s;
+    // Work of one bucket in phases.  With the work queue (default) the
+    // backprojections and projections of all buckets run as single persistent
+    // launches over every bucket (one launch per stage and iteration); the
+    // per-bucket phases (copies, filter, eEM set-up) run as graph branches.
+    // RAFT_BATCH_QUEUE=0 runs every stage per bucket, all inside its branch.
+    const char* qenv = std::getenv("RAFT_BATCH_QUEUE");
+    const bool use_queue = !(qenv && qenv[0] == '0');
+    auto fbp_bp_args = [&](BatchBucket* b) {
+        BpArgs ba;
+        ba.q = b->frows.p; ba.out = b->fout.as<float>(); ba.x = nullptr; ba.norm = nullptr; ba.xt = nullptr;
+        ba.cA = b->fcA.as<double>(); ba.cB = b->fcB.as<double>(); ba.xc = b->gf.xc; ba.c0 = b->gf.c0;
+        ba.n = b->N; ba.A = b->A; ba.S = (int)b->scans.size(); ba.tlo = b->gf.tlo; ba.W = b->gf.width; ba.mode = 0;
+        ba.q_bytes = b->frows.bytes;
+        return ba;
+    };
+    auto em_proj_args = [&](BatchBucket* b) {
+        ProjArgs p;
+        p.img = b->xil.as<float>(); p.imgT = b->xT.as<float>(); p.out = b->erows.as<float>();
+        p.y = b->y.as<float>(); p.wa = b->ewa.as<float>(); p.kl_part = nullptr; p.ex_part = nullptr;
+        p.cosv = b->dc.as<double>(); p.sinv = b->ds.as<double>();
+        p.g = b->rg; p.S = (int)b->scans.size(); p.tlo = b->ge.tlo; p.W = b->ge.width; p.lds_rc = b->prc;
+        return p;
+    };
+    auto em_bp_args = [&](BatchBucket* b) {
+        BpArgs bu;
+        bu.q = b->erows.as<float>(); bu.out = b->xil.as<float>(); bu.x = b->xil.as<float>();
+        bu.norm = b->ones.as<float>(); bu.xt = b->xT.as<float>();
+        bu.cA = b->ecA.as<double>(); bu.cB = b->ecB.as<double>(); bu.xc = b->ge.xc; bu.c0 = b->ge.c0;
+        bu.n = b->N; bu.A = b->A; bu.S = (int)b->scans.size(); bu.tlo = b->ge.tlo; bu.W = b->ge.width; bu.mode = 1;
+        bu.q_bytes = b->erows.bytes;
+        return bu;
+    };
+    // phase 1 (branch): gathers, FBP filter, eEM set-up
+    auto enqueue_setup = [&](BatchBucket* b, hipStream_t s) {
         const int N = b->N, A = b->A, S = (int)b->scans.size();
         const size_t img = (size_t)N * N, sino = (size_t)A * N;
         if (do_fbp) {
@@ -1113,14 +1312,6 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             fa.N = N; fa.A = A; fa.S = S; fa.npad = b->cf.npad; fa.L = b->cf.L; fa.tlo = b->gf.tlo; fa.W = b->gf.width;
             fa.none = (b->cf.filter == 0);
             launch_fbp_filter(fa, b->F->plan, store, s);
-            BpArgs ba;
-            ba.q = b->frows.p; ba.out = b->fout.as<float>(); ba.x = nullptr; ba.norm = nullptr; ba.xt = nullptr;
-            ba.cA = b->fcA.as<double>(); ba.cB = b->fcB.as<double>(); ba.xc = b->gf.xc; ba.c0 = b->gf.c0;
-            ba.n = N; ba.A = A; ba.S = S; ba.tlo = b->gf.tlo; ba.W = b->gf.width; ba.mode = 0;
-            launch_backproject(ba, store, s);
-            for (int k = 0; k < S; ++k)
-                RAFT_HIP(hipMemcpyAsync(d_fbp[b->scans[k]], b->fout.as<float>() + k * img, img * 4,
-                                        hipMemcpyDeviceToDevice, s));
         }
         if (do_em) {
             const BpGeom& g = b->ge;
@@ -1137,29 +1328,153 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             b1.q = onerows; b1.out = ones; b1.x = nullptr; b1.norm = nullptr; b1.xt = nullptr;
             b1.cA = b->ecA.as<double>(); b1.cB = b->ecB.as<double>(); b1.xc = g.xc; b1.c0 = g.c0;
             b1.n = N; b1.A = A; b1.S = 1; b1.tlo = g.tlo; b1.W = g.width; b1.mode = 0;
+            b1.q_bytes = (size_t)A * g.width * 16;
             launch_backproject(b1, RAFT_STORE_F32, s);
             const int G = (S + 3) / 4;
             launch_fill(b->erows.as<float>(), (long long)G * A * g.width * 4, 0.0f, s);
             launch_to_interleaved(b->x.as<float>(), b->xil.as<float>(), S, N, N, false, s);
             launch_to_interleaved(b->x.as<float>(), b->xT.as<float>(), S, N, N, true, s);
+        }
+    };
+    auto enqueue_fbp_scatter = [&](BatchBucket* b, hipStream_t s) {
+        const size_t img = (size_t)b->N * b->N;
+        for (size_t k = 0; k < b->scans.size(); ++k)
+            RAFT_HIP(hipMemcpyAsync(d_fbp[b->scans[k]], b->fout.as<float>() + k * img, img * 4,
+                                    hipMemcpyDeviceToDevice, s));
+    };
+    auto enqueue_em_finish = [&](BatchBucket* b, hipStream_t s) {
+        const int N = b->N, S = (int)b->scans.size();
+        const size_t img = (size_t)N * N;
+        launch_from_interleaved(b->xil.as<float>(), b->x.as<float>(), S, N, N, s);
+        for (int k = 0; k < S; ++k)
+            RAFT_HIP(hipMemcpyAsync(d_em[b->scans[k]], b->x.as<float>() + k * img, img * 4,
+                                    hipMemcpyDeviceToDevice, s));
+    };
+    // the whole bucket in its branch (RAFT_BATCH_QUEUE=0)
+    auto enqueue = [&](BatchBucket* b) {
+        hipStream_t s = b->s;
+        enqueue_setup(b, s);
+        if (do_fbp) {
+            launch_backproject(fbp_bp_args(b), store, s);
+            enqueue_fbp_scatter(b, s);
+        }
+        if (do_em) {
             for (int it = 0; it < K; ++it) {
-                ProjArgs p;
-                p.img = b->xil.as<float>(); p.imgT = b->xT.as<float>(); p.out = b->erows.as<float>();
-                p.y = b->y.as<float>(); p.wa = b->ewa.as<float>(); p.kl_part = nullptr; p.ex_part = nullptr;
-                p.cosv = b->dc.as<double>(); p.sinv = b->ds.as<double>();
-                p.g = b->rg; p.S = S; p.tlo = g.tlo; p.W = g.width; p.lds_rc = b->prc;
-                launch_project(p, 1, s);
-                BpArgs bu;
-                bu.q = b->erows.as<float>(); bu.out = b->xil.as<float>(); bu.x = b->xil.as<float>(); bu.norm = ones;
-                bu.xt = b->xT.as<float>();
-                bu.cA = b->ecA.as<double>(); bu.cB = b->ecB.as<double>(); bu.xc = g.xc; bu.c0 = g.c0;
-                bu.n = N; bu.A = A; bu.S = S; bu.tlo = g.tlo; bu.W = g.width; bu.mode = 1;
-                launch_backproject(bu, RAFT_STORE_F32, s);
+                launch_project(em_proj_args(b), 1, s);
+                launch_backproject(em_bp_args(b), RAFT_STORE_F32, s);
             }
-            launch_from_interleaved(b->xil.as<float>(), b->x.as<float>(), S, N, N, s);
-            for (int k = 0; k < S; ++k)
-                RAFT_HIP(hipMemcpyAsync(d_em[b->scans[k]], b->x.as<float>() + k * img, img * 4,
-                                        hipMemcpyDeviceToDevice, s));
+            enqueue_em_finish(b, s);
+        }
+    };
+
+    // queue descriptors: segments in decreasing cost per item (the longest
+    // items are taken first), at most QUEUE_MAXSEG per launch
+    struct QLaunch {
+        std::vector<BatchBucket*> seg;
+        size_t desc_off = 0;
+        int counter = 0; // index of its counter pair
+    };
+    std::vector<QLaunch> q_fbp, q_proj, q_embp;
+    std::vector<BatchBucket*> em_direct; // buckets whose projection takes the direct form
+    std::vector<char> qdesc;
+    int ncounters = 0;
+    auto plan_queue = [&](std::vector<BatchBucket*> v, auto cost, std::vector<QLaunch>& out, size_t dsize,
+                          auto fill) {
+        std::stable_sort(v.begin(), v.end(), [&](BatchBucket* x, BatchBucket* y) { return cost(x) > cost(y); });
+        for (size_t i = 0; i < v.size(); i += QUEUE_MAXSEG) {
+            QLaunch L;
+            L.seg.assign(v.begin() + i, v.begin() + std::min(v.size(), i + QUEUE_MAXSEG));
+            L.desc_off = (qdesc.size() + 255) & ~(size_t)255;
+            qdesc.resize(L.desc_off + dsize * L.seg.size());
+            for (size_t k = 0; k < L.seg.size(); ++k) fill(L.seg[k], qdesc.data() + L.desc_off + k * dsize);
+            L.counter = ncounters++;
+            out.push_back(std::move(L));
+        }
+    };
+    DevBuf qbuf, qctr;
+    if (use_queue) {
+        std::vector<BatchBucket*> all(buckets.begin(), buckets.end()), staged;
+        for (BatchBucket* b : buckets) (b->prc > 0 ? staged : em_direct).push_back(b);
+        if (do_fbp)
+            plan_queue(all, [](BatchBucket* b) { return (double)b->A; }, q_fbp, bp_desc_size(),
+                       [&](BatchBucket* b, void* d) { bp_desc_fill(fbp_bp_args(b), store, d); });
+        if (do_em) {
+            plan_queue(staged,
+                       [](BatchBucket* b) {
+                           const int nrb = (b->N + 31) / 32;
+                           return (double)b->N * (double)((nrb + b->prc - 1) / b->prc);
+                       },
+                       q_proj, proj_desc_size(), [&](BatchBucket* b, void* d) { proj_desc_fill(em_proj_args(b), d); });
+            plan_queue(all, [](BatchBucket* b) { return (double)b->A; }, q_embp, bp_desc_size(),
+                       [&](BatchBucket* b, void* d) { bp_desc_fill(em_bp_args(b), RAFT_STORE_F32, d); });
+        }
+        if (!qdesc.empty()) {
+            qbuf.alloc(qdesc.size());
+            RAFT_HIP(hipMemcpyAsync(qbuf.p, qdesc.data(), qdesc.size(), hipMemcpyHostToDevice, st));
+        }
+        if (ncounters) {
+            qctr.alloc((size_t)ncounters * 2 * sizeof(int));
+            RAFT_HIP(hipMemsetAsync(qctr.p, 0, (size_t)ncounters * 2 * sizeof(int), st));
+        }
+        RAFT_HIP(hipStreamSynchronize(st));
+    }
+    auto bp_queue_of = [&](const QLaunch& L, int per_group_slices) {
+        BpQueue q;
+        q.counter = qctr.as<int>() + 2 * L.counter;
+        q.nseg = (int)L.seg.size();
+        q.first[0] = 0;
+        for (int k = 0; k < q.nseg; ++k) {
+            BatchBucket* b = L.seg[k];
+            const int G = ((int)b->scans.size() + per_group_slices - 1) / per_group_slices;
+            q.tiles[k] = bp_tiles(b->N);
+            q.first[k + 1] = q.first[k] + q.tiles[k] * G;
+        }
+        return q;
+    };
+    auto proj_queue_of = [&](const QLaunch& L) {
+        ProjQueue q;
+        q.counter = qctr.as<int>() + 2 * L.counter;
+        q.nseg = (int)L.seg.size();
+        q.first[0] = 0;
+        for (int k = 0; k < q.nseg; ++k) {
+            BatchBucket* b = L.seg[k];
+            const int G = ((int)b->scans.size() + 3) / 4;
+            q.nab[k] = proj_angle_chunks(b->A);
+            q.first[k + 1] = q.first[k] + q.nab[k] * b->prc * G;
+        }
+        return q;
+    };
+    const int sb_fbp = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
+    const int grid_fbp = use_queue && do_fbp ? bp_queue_grid(store, 0, 32) : 0;
+    const int grid_embp = use_queue && do_em ? bp_queue_grid(RAFT_STORE_F32, 1, 32) : 0;
+    const int grid_proj = use_queue && do_em ? proj_queue_grid(1) : 0;
+    // the stages after the per-bucket set-up, on the capture stream
+    auto enqueue_queued = [&](hipStream_t s, hipEvent_t evf) {
+        if (do_fbp) {
+            for (const QLaunch& L : q_fbp)
+                launch_backproject_queue(qbuf.as<char>() + L.desc_off, bp_queue_of(L, sb_fbp), store, 0, 32, grid_fbp,
+                                         s);
+            for (BatchBucket* b : buckets) enqueue_fbp_scatter(b, s);
+        }
+        if (do_em) {
+            for (int it = 0; it < K; ++it) {
+                // direct-form buckets (sparse angle sets) on their branches, beside the queue
+                if (!em_direct.empty()) {
+                    RAFT_HIP(hipEventRecord(evf, s));
+                    for (BatchBucket* b : em_direct) {
+                        RAFT_HIP(hipStreamWaitEvent(b->s, evf, 0));
+                        launch_project(em_proj_args(b), 1, b->s);
+                        RAFT_HIP(hipEventRecord(b->done, b->s));
+                    }
+                }
+                for (const QLaunch& L : q_proj)
+                    launch_project_queue(qbuf.as<char>() + L.desc_off, proj_queue_of(L), 1, grid_proj, s);
+                for (BatchBucket* b : em_direct) RAFT_HIP(hipStreamWaitEvent(s, b->done, 0));
+                for (const QLaunch& L : q_embp)
+                    launch_backproject_queue(qbuf.as<char>() + L.desc_off, bp_queue_of(L, 4), RAFT_STORE_F32, 1, 32,
+                                             grid_embp, s);
+            }
+            for (BatchBucket* b : buckets) enqueue_em_finish(b, s);
         }
     };
 
@@ -1194,10 +1509,12 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
             RAFT_HIP(hipEventRecord(fork, cs));
             for (BatchBucket* b : buckets) {
                 RAFT_HIP(hipStreamWaitEvent(b->s, fork, 0));
-                enqueue(b);
+                if (use_queue) enqueue_setup(b, b->s);
+                else enqueue(b);
                 RAFT_HIP(hipEventRecord(b->done, b->s));
                 RAFT_HIP(hipStreamWaitEvent(cs, b->done, 0));
             }
+            if (use_queue) enqueue_queued(cs, fork);
         } catch (...) {
             (void)hipStreamEndCapture(cs, &graph);
             throw;
@@ -1272,33 +1589,154 @@ struct Stream {
     ~Stream() { if (s) (void)hipStreamDestroy(s); }
 };
 
+// Host memcpy split over a small pool of threads (pageable caller arrays to
+// and from page-locked staging: one thread moves ~10 GB/s, the pool several
+// times that).  copy() blocks until every part is done.
+class CopyPool {
+  public:
+    explicit CopyPool(int nthreads) : n_(std::max(1, nthreads))
+    {
+        for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { worker(i); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        go_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void copy(void* dst, const void* src, size_t bytes)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            bytes_ = bytes;
+            left_ = n_ - 1;
+            ++gen_;
+        }
+        go_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return left_ == 0; });
+    }
+
+  private:
+    void part(int i)
+    {
+        // 4 KB-aligned split points
+        const size_t lo = ((bytes_ * (size_t)i / n_) >> 12) << 12;
+        const size_t hi = (i + 1 == n_) ? bytes_ : ((bytes_ * (size_t)(i + 1) / n_) >> 12) << 12;
+        if (hi > lo) std::memcpy(dst_ + lo, src_ + lo, hi - lo);
+    }
+    void worker(int i)
+    {
+        long seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu_);
+            go_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            lk.unlock();
+            part(i);
+            lk.lock();
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable go_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    long gen_ = 0;
+    int left_ = 0;
+    bool stop_ = false;
+};
+
+static int copy_threads()
+{
+    const char* e = std::getenv("RAFT_COPY_THREADS");
+    if (e && std::atoi(e) > 0) return std::min(64, std::atoi(e));
+    return (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
 static void fbp_host(int* gpus, int ngpus, float* obj, float* tomo, float* angles, float* paramf, int* parami)
 {
     const FbpCfg c = decode_fbp(paramf, parami);
     if (!obj || !tomo || !angles) fail(RAFT_EINVAL, "null data pointer");
     check_gpus(gpus, ngpus);
     run_slabs(gpus, ngpus, c.S, [&](int dev, int first, int cnt) {
+        if (cnt <= 0) return;
         bind_device(dev);
-        Stream st;
+        // Pipeline over chunks of whole slice groups, two in flight: the copy
+        // pool moves chunk k from the caller's array into page-locked staging
+        // while the GPU filters and backprojects chunk k-1 and the DMA engines
+        // move chunks in and out on their own streams.
         const size_t tin = (size_t)c.A * c.N, oout = (size_t)c.N * c.N;
-        // device copies of the slab in blocks that fit next to the workspace
-        int B = c.blocksize > 0 ? std::min(c.blocksize, cnt) : cnt;
-        const size_t need = (tin + oout) * 4 + (size_t)c.A * (size_t)(1.5 * c.N + 64) * 4;
-        const size_t budget = block_budget();
-        while (B > 1 && (size_t)B * need > budget) B = (B + 1) / 2;
-        DevBuf dt((size_t)B * tin * 4), dob((size_t)B * oout * 4);
-        std::vector<int> pi(parami, parami + 10);
-        for (int s0 = 0; s0 < cnt; s0 += B) {
-            const int nb = std::min(B, cnt - s0);
-            RAFT_HIP(hipMemcpyAsync(dt.p, tomo + (size_t)(first + s0) * tin, (size_t)nb * tin * 4,
-                                    hipMemcpyHostToDevice, st.s));
-            pi[2] = nb;
-            pi[9] = 0;
-            fbp_device(dev, dob.as<float>(), dt.as<float>(), angles, paramf, pi.data(), RAFT_STORE_F32, st.s);
-            RAFT_HIP(hipMemcpyAsync(obj + (size_t)(first + s0) * oout, dob.p, (size_t)nb * oout * 4,
-                                    hipMemcpyDeviceToHost, st.s));
+        const size_t target = (size_t)64 << 20; // ~64 MB of sinogram per chunk
+        int C = (int)std::max<size_t>(1, target / (tin * 4));
+        if (c.blocksize > 0) C = std::min(C, c.blocksize);
+        C = std::max(BP_SLICES_F32, (C / BP_SLICES_F32) * BP_SLICES_F32);
+        C = std::min(C, cnt);
+        const int nch = (cnt + C - 1) / C;
+        Device& D = device_state(dev);
+        std::lock_guard<std::mutex> lk(D.mu);
+        void* pin_in[2] = {D.pin[0].ensure((size_t)C * tin * 4), D.pin[1].ensure((size_t)C * tin * 4)};
+        void* pin_out[2] = {D.pin[2].ensure((size_t)C * oout * 4), D.pin[3].ensure((size_t)C * oout * 4)};
+        float* d_in[2] = {static_cast<float*>(D.workspace<8>((size_t)C * tin * 4)),
+                          static_cast<float*>(D.workspace<9>((size_t)C * tin * 4))};
+        float* d_out[2] = {static_cast<float*>(D.workspace<10>((size_t)C * oout * 4)),
+                           static_cast<float*>(D.workspace<11>((size_t)C * oout * 4))};
+        Stream s_in, s_comp, s_out;
+        std::vector<hipEvent_t> ev;
+        auto mk = [&]() {
+            hipEvent_t e = nullptr;
+            RAFT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev.push_back(e);
+            return e;
+        };
+        struct Cleanup {
+            Device& D;
+            std::vector<hipEvent_t>& ev;
+            hipStream_t s[3];
+            ~Cleanup()
+            {
+                for (hipStream_t x : s) (void)hipStreamSynchronize(x);
+                for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+                D.keep.clear();
+            }
+        } cleanup{D, ev, {s_in.s, s_comp.s, s_out.s}};
+        hipEvent_t ev_h2d[2] = {mk(), mk()}, ev_comp[2] = {mk(), mk()}, ev_d2h[2] = {mk(), mk()};
+        CopyPool pool(copy_threads());
+        auto rows_of = [&](int k) { return std::min(C, cnt - k * C); };
+        auto host_out = [&](int k) {
+            const int b = k & 1;
+            RAFT_HIP(hipEventSynchronize(ev_d2h[b]));
+            pool.copy(obj + (size_t)(first + k * C) * oout, pin_out[b], (size_t)rows_of(k) * oout * 4);
+        };
+        for (int k = 0; k < nch; ++k) {
+            const int b = k & 1, nb = rows_of(k);
+            if (k >= 2) RAFT_HIP(hipEventSynchronize(ev_h2d[b])); // pin_in[b] read by the copy of chunk k-2
+            pool.copy(pin_in[b], tomo + (size_t)(first + k * C) * tin, (size_t)nb * tin * 4);
+            if (k >= 2) RAFT_HIP(hipStreamWaitEvent(s_in.s, ev_comp[b], 0)); // d_in[b] read by chunk k-2's FBP
+            RAFT_HIP(hipMemcpyAsync(d_in[b], pin_in[b], (size_t)nb * tin * 4, hipMemcpyHostToDevice, s_in.s));
+            RAFT_HIP(hipEventRecord(ev_h2d[b], s_in.s));
+            RAFT_HIP(hipStreamWaitEvent(s_comp.s, ev_h2d[b], 0));
+            if (k >= 2) RAFT_HIP(hipStreamWaitEvent(s_comp.s, ev_d2h[b], 0)); // d_out[b] read by chunk k-2's copy
+            FbpCfg ck = c;
+            ck.S = nb;
+            fbp_run(D, d_out[b], d_in[b], angles, ck, RAFT_STORE_F32, s_comp.s, false);
+            RAFT_HIP(hipEventRecord(ev_comp[b], s_comp.s));
+            RAFT_HIP(hipStreamWaitEvent(s_out.s, ev_comp[b], 0));
+            RAFT_HIP(hipMemcpyAsync(pin_out[b], d_out[b], (size_t)nb * oout * 4, hipMemcpyDeviceToHost, s_out.s));
+            RAFT_HIP(hipEventRecord(ev_d2h[b], s_out.s));
+            if (k >= 1) host_out(k - 1); // pin_out of chunk k-1 -> the caller's array
         }
-        RAFT_HIP(hipStreamSynchronize(st.s));
+        host_out(nch - 1);
     });
 }
 
@@ -1640,6 +2078,16 @@ int raft_eem_device_kl(int device, float* d_obj, const float* d_tomo, const floa
         if (kl_iterations < 0) fail(RAFT_EINVAL, "kl_iterations must be >= 0");
         eem_device(device, d_obj, d_tomo, angles, parami, kl_iterations > 0 ? kl : nullptr,
                    kl_iterations > 0 ? kl_excluded : nullptr, static_cast<hipStream_t>(stream), false, kl_iterations);
+    });
+}
+
+int raft_eem_device_sched(int device, float* d_obj, const float* d_tomo, const float* angles, const int* parami,
+                          const int* iterations, void* stream)
+{
+    return guarded([&] {
+        if (!iterations) fail(RAFT_EINVAL, "null iteration counts");
+        eem_device(device, d_obj, d_tomo, angles, parami, nullptr, nullptr, static_cast<hipStream_t>(stream), false,
+                   -1, iterations);
     });
 }
 

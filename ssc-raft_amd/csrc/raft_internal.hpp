@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -69,7 +70,8 @@ struct BpGeom {
 // Tile of the backprojection kernel: 64 x 32 pixels, 256 threads, 4 x 2 pixels each.
 constexpr int BP_TILE_X = 64;
 constexpr int BP_TILE_Y = 32;
-constexpr int BP_CHUNK = 32;   // angles staged per LDS round
+constexpr int BP_CHUNK = 32;   // angles staged per LDS round (default; 16 and 48 are autotuner variants)
+constexpr int BP_GROUP = 16;   // angles per partial sum of the FP32 two-level summation
 constexpr int BP_SLOT = 80;    // LDS entries per staged angle window (>= |(63, 31)| + 5)
 // staged rows hold the bins of BP_SLICES_F32 (FP32) or BP_SLICES_F16 (FP16)
 // slices side by side: [group][angle][bin][slice]
@@ -153,8 +155,35 @@ struct BpArgs {
     int n, A, S, tlo, W;
     int mode;           // 0 = store the sums (FBP, backones), 1 = EM update x * bp / norm
     int norm_il = 0;    // mode 1: norm is per slice, interleaved [G][n][n][4] (tEM)
+    int chunk = 0;      // angles per LDS round (16, 32 or 48; 0 = 32)
+    size_t q_bytes = 0; // bytes of the buffer behind q when known: launches that would read past it are refused
 };
 void launch_backproject(const BpArgs& a, int store, hipStream_t st);
+bool bp_chunk_supported(int chunk);
+// tiles per slice group of an n x n image
+int bp_tiles(int n);
+
+// ---------------------------------------------------------------------------
+// Persistent work-queue launches: one resident grid takes items from a
+// device counter over several volumes ("segments", e.g. the scans of a batch
+// grouped by geometry).  Segment b owns items [first[b], first[b+1]); the
+// kernels decode an item into (volume b, its work unit).  counter points at
+// two ints that are zero between launches (the last workgroup clears them).
+// ---------------------------------------------------------------------------
+constexpr int QUEUE_MAXSEG = 32;
+struct BpQueue {
+    int* counter;
+    int nseg;
+    int first[QUEUE_MAXSEG + 1];
+    int tiles[QUEUE_MAXSEG]; // tiles per slice group of segment b
+};
+// device descriptor of one volume for launch_backproject_queue (bp_desc_size() bytes)
+size_t bp_desc_size();
+void bp_desc_fill(const BpArgs& b, int store, void* dst);
+// resident workgroups of the queue kernel (CUs x blocks per CU)
+int bp_queue_grid(int store, int mode, int chunk);
+void launch_backproject_queue(const void* d_descs, const BpQueue& q, int store, int mode, int chunk, int grid,
+                              hipStream_t st);
 
 struct ProjArgs {
     const float* img;   // interleaved [G][Ny][Nx][4]
@@ -178,6 +207,19 @@ void launch_project(const ProjArgs& a, int em_mode, hipStream_t st);
 // chunks, or 0 when a chunk of 8 angles cannot be staged (sparse angles) or
 // RAFT_PROJ_LDS=0.  have_T: the transposed image will be passed as imgT.
 int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool have_T);
+// persistent work-queue form of the staged projector (see BpQueue); segment b
+// has nab[b] angle chunks x RC ray chunks x slice groups items
+struct ProjQueue {
+    int* counter;
+    int nseg;
+    int first[QUEUE_MAXSEG + 1];
+    int nab[QUEUE_MAXSEG];
+};
+size_t proj_desc_size();
+void proj_desc_fill(const ProjArgs& p, void* dst);
+int proj_angle_chunks(int nangles);
+int proj_queue_grid(int em_mode);
+void launch_project_queue(const void* d_descs, const ProjQueue& q, int em_mode, int grid, hipStream_t st);
 // the staged form's tile constants: rays per patch, samples per step, angles
 // per workgroup, staged 16-byte entries per workgroup
 void proj_tile_consts(int* c4);
@@ -204,6 +246,14 @@ void launch_background(float* data, const float* flat, const float* dark, int R,
 
 // p[0 .. n) = v (a kernel, so it may be captured into a graph)
 void launch_fill(float* p, long long n, float v, hipStream_t st);
+
+// autotuner (autotune.cpp): the cached choice for key, else the fastest of
+// cands timed with launch(c) on stream st (or cands[0] when tuning is off)
+bool tune_enabled();
+int tune_pick(const std::string& key, const std::vector<int>& cands, hipStream_t st,
+              const std::function<void(int)>& launch);
+// the cached choice for key, or dflt
+int tune_cached(const std::string& key, int dflt);
 
 // profiling hooks (events around kernel families)
 enum KernelFamily { KF_FILTER = 0, KF_BP = 1, KF_PROJ = 2, KF_EMBP = 3, KF_COUNT = 4 };

@@ -6,7 +6,8 @@ import numpy
 from .rafttypes import CNICE, libraft, raft_check, ctypes
 
 __all__ = ["fbp_device", "radon_device", "eem_device", "eem_kl_device", "batch_device", "lowpass_device",
-           "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info", "proj_plan_info",
+           "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info", "proj_plan_info", "autotune_choices",
+           "autotune_clear", "eem_device_sched",
            "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
 
 KF_FILTER, KF_BP, KF_PROJ, KF_EMBP = 0, 1, 2, 3
@@ -260,3 +261,44 @@ def proj_plan_info(angles, Nx, Ny=None, nrays=None, nslices=4, ax=1.0, ay=1.0, e
     if err:
         raft_check("raft_proj_plan_info")
     return rc.value, tuple(int(c) for c in consts)
+
+
+def autotune_choices():
+    """The autotuner's choices held by the library: {key: {"choice": c, "ms":
+    [[candidate, ms], ...]}} (raft_autotune_dump)."""
+    import json
+
+    n = libraft.raft_autotune_dump(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    libraft.raft_autotune_dump(buf, n)
+    return json.loads(buf.value.decode())
+
+
+def autotune_clear():
+    """Forget the choices held in memory (raft_autotune_clear)."""
+    libraft.raft_autotune_clear()
+
+
+def eem_device_sched(tomo, angles, iterations, obj=None, stream=None):
+    """eEM with an iteration count per slice (raft_eem_device_sched):
+    iterations [S] must be non-increasing; slice s stops after iterations[s].
+    obj [S, n, n] is the in/out initial guess (ones if None).  Returns obj."""
+    import torch
+
+    _check_tensor(tomo, "tomo")
+    S, A, R = tomo.shape
+    if obj is None:
+        obj = torch.ones((S, R, R), dtype=torch.float32, device=tomo.device)
+    _check_tensor(obj, "obj")
+    it = CNICE(numpy.asarray(iterations), numpy.int32)
+    if it.shape != (S,):
+        raise ValueError(f"iterations must hold one count per slice ({S}), got shape {it.shape}")
+    angles = CNICE(numpy.asarray(angles))
+    parami = CNICE(numpy.array([R, A, S, obj.shape[-1], 0, 0, 0, 1, int(it.max()) if S else 0, 0]), numpy.int32)
+    rc = libraft.raft_eem_device_sched(tomo.device.index or 0, ctypes.c_void_p(obj.data_ptr()),
+                                       ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
+                                       parami.ctypes.data_as(ctypes.c_void_p), it.ctypes.data_as(ctypes.c_void_p),
+                                       _stream(tomo, stream))
+    if rc:
+        raft_check("raft_eem_device_sched")
+    return obj

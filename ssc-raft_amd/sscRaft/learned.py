@@ -25,7 +25,7 @@ import time
 import numpy as np
 
 from . import phantom as ph
-from .device import eem_device, eem_kl_device, fbp_device, radon_device
+from .device import eem_device, eem_device_sched, eem_kl_device, fbp_device, radon_device
 from .geometries.parallel.fbp_methods import fbp_params
 from .nn import UNet, _stream, nn_call
 
@@ -284,6 +284,13 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
            "accepted": int(accept.sum()),
            "kl_start_net_median": float(np.median(kl0[0])), "kl_start_fbp_median": float(np.median(kl0[1])),
            "kl_target_median": float(np.median(target))}
+    q = [0.1, 0.25, 0.5, 0.75, 0.9, 0.99, 1.0]
+    res["iterations_quantiles"] = {"q": q, "plain": np.quantile(k_plain, q).tolist(),
+                                   "fbp_start": np.quantile(k_fbp[k_fbp >= 0], q).tolist() if (k_fbp >= 0).any()
+                                   else None,
+                                   "learned": np.quantile(k_net[reached], q).tolist() if reached.any() else None}
+    res["iterations_sum"] = {"plain": int(k_plain.sum()), "fbp_start": int(k_fbp[k_fbp >= 0].sum()),
+                             "learned": int(k_net[reached].sum())}
     miss = np.nonzero(~reached)[0]
     res["not_reached"] = [{"slice": int(s), "target": float(target[s]), "best_kl": float(kl_net[s].min()),
                            "best_iter": int(kl_net[s].argmin()), "kl_start": float(kl_net[s, 0]),
@@ -310,8 +317,71 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
         res["fbp_start_seconds"] = t_fbp
         res["fbp_start_speedup"] = t_plain / t_fbp
     res["warm_start_seconds"] = t_ws
+    res["scheduled"] = evaluate_scheduled(net, p_hat, y, ang, x0, xf, k_plain, k_net, k_fbp, cfg, timed)
     log(res)
     return res
+
+
+def sched_eem(y, ang, x_start, counts):
+    """eEM where slice s runs counts[s] iterations (raft_eem_device_sched):
+    the slices are gathered in decreasing count order, run, and scattered
+    back.  Returns the images [S, N, N]."""
+    import torch
+
+    order = np.argsort(-np.asarray(counts), kind="stable")
+    idx = torch.from_numpy(order).to(y.device)
+    ys = y.index_select(0, idx).contiguous()
+    xs = x_start.index_select(0, idx).contiguous()
+    eem_device_sched(ys, ang, np.asarray(counts)[order].astype(np.int32), obj=xs)
+    x = torch.empty_like(xs)
+    x.index_copy_(0, idx, xs)
+    return x
+
+
+def evaluate_scheduled(net, p_hat, y, ang, x0, xf, k_plain, k_net, k_fbp, cfg, timed):
+    """Time-to-KL when every slice stops as soon as it reaches its target
+    (raft_eem_device_sched): each start runs its own per-slice counts, found
+    by the untimed runs of evaluate(), so the batch costs the sum of its
+    slices' iterations instead of its slowest slice's count times the batch.
+      plain   eEM from ones, k_plain[s] iterations;
+      fbp     FBP start, k_fbp[s] iterations;
+      learned warm_start (FBP, network, both starts' KL, the acceptance rule),
+              then k_net[s] iterations; a slice whose accepted start never
+              reaches its target within the explored iterations runs from its
+              FBP start for k_fbp[s] iterations instead.
+    Slices that no start brings to the target are counted and reported."""
+    import torch
+
+    dev = y.device
+    k_plain = np.asarray(k_plain, np.int64)
+    k_net = np.asarray(k_net, np.int64)
+    k_fbp = np.asarray(k_fbp, np.int64)
+    fallback = (k_net < 0) & (k_fbp >= 0)
+    k_learn = np.where(k_net >= 0, k_net, np.where(fallback, k_fbp, 0))
+    fb_idx = torch.from_numpy(np.nonzero(fallback)[0]).to(dev)
+    r = {"slices": int(len(k_plain)), "learned_reached": int(((k_net >= 0) | fallback).sum()),
+         "learned_from_fbp_start": int(fallback.sum()), "fbp_start_reached": int((k_fbp >= 0).sum()),
+         "iterations_sum": {"plain": int(k_plain.sum()), "fbp_start": int(np.maximum(k_fbp, 0).sum()),
+                            "learned": int(k_learn.sum())}}
+    ones = torch.ones_like(x0)
+    sched_eem(y, ang, ones, k_plain)  # warm-up of the scheduled path
+    _, t_plain = timed(lambda: sched_eem(y, ang, torch.ones_like(x0), k_plain))
+
+    def fbp_run():
+        x = torch.clamp(fbp_start(p_hat, ang, cfg), min=cfg.floor)
+        return sched_eem(y, ang, x, np.maximum(k_fbp, 0))
+
+    def learned_run():
+        xs, _, xfb, _ = warm_start(net, p_hat, y, ang, cfg)
+        if fb_idx.numel():
+            xs.index_copy_(0, fb_idx, xfb.index_select(0, fb_idx))
+        return sched_eem(y, ang, xs, k_learn)
+
+    _, t_fbp = timed(fbp_run)
+    _, t_learn = timed(learned_run)
+    r.update(plain_seconds=t_plain, fbp_start_seconds=t_fbp, learned_seconds=t_learn,
+             speedup=t_plain / t_learn, fbp_start_speedup=t_plain / t_fbp)
+    return r
 
 
 def default_weights_path():

@@ -83,6 +83,12 @@ libraft.raft_fbp_plan_info.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
 libraft.raft_fbp_plan_info.restype = _i
 libraft.raft_proj_plan_info.argtypes = [_vp, _i, _i, _i, _i, _i, _f, _f, _i, _vp, _vp]
 libraft.raft_proj_plan_info.restype = _i
+libraft.raft_autotune_dump.argtypes = [ctypes.c_char_p, _i]
+libraft.raft_autotune_dump.restype = _i
+libraft.raft_autotune_clear.argtypes = []
+libraft.raft_autotune_clear.restype = None
+libraft.raft_eem_device_sched.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp]
+libraft.raft_eem_device_sched.restype = _i
 libraft.raft_set_profiling.argtypes = [_i]
 libraft.raft_set_profiling.restype = None
 libraft.raft_last_kernel_ms.argtypes = [_i]

@@ -28,6 +28,7 @@ def declared_functions():
     names = set()
     for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
         text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        text = re.sub(r"^\s*#.*$", "", text, flags=re.M)  # preprocessor lines (macros are not exports)
         names |= set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", text))
     return sorted(names)
 
@@ -405,3 +406,21 @@ def test_projector_plan_keeps_every_sample_in_the_staged_box(em, nx, ny, nrays, 
             pitch = w + ((1 - w) & 3)
             assert np.all(rows * pitch <= CAP), (a0, ib, int((rows * pitch).max()), CAP)
     assert W >= 1
+
+
+def test_autotune_cache_file_round_trip(tmp_path, monkeypatch):
+    """RAFT_AUTOTUNE_CACHE: choices written by an earlier process are read back
+    (no GPU needed: reading and dumping the cache is host work)."""
+    from sscRaft import device as dv
+
+    f = tmp_path / "tune.json"
+    f.write_text('{\n "bp/n=64/A=48/W=99/store=0/mode=0/groups=1": {"choice": 16, "ms": [[32, 0.5], [16, 0.4]]},\n'
+                 ' "proj/Nx=64/Ny=64/nrays=64/A=64/em=1/groups=1/ry=1.000000": {"choice": 4, "ms": []}\n}\n')
+    monkeypatch.setenv("RAFT_AUTOTUNE_CACHE", str(f))
+    dv.autotune_clear()
+    c = dv.autotune_choices()
+    assert c["bp/n=64/A=48/W=99/store=0/mode=0/groups=1"]["choice"] == 16
+    assert c["proj/Nx=64/Ny=64/nrays=64/A=64/em=1/groups=1/ry=1.000000"]["choice"] == 4
+    monkeypatch.delenv("RAFT_AUTOTUNE_CACHE")
+    dv.autotune_clear()
+    assert dv.autotune_choices() == {}

@@ -585,3 +585,59 @@ def test_zero_slices_are_a_no_op(gpu):
     proj = dv.radon_device(torch.zeros((0, n, n), device="cuda"), ang, out=torch.zeros((0, a, n), device="cuda"))
     assert proj.shape == (0, a, n)
     torch.cuda.synchronize()
+
+
+def test_batch_work_queue_matches_per_bucket_branches(gpu, monkeypatch):
+    """The persistent work-queue launches (one per stage and iteration over all
+    buckets, default) and the per-bucket graph branches (RAFT_BATCH_QUEUE=0)
+    run the same per-tile arithmetic: FBP (FP32 and FP16 rows) and eEM outputs
+    are bit-identical.  Sizes 40..96 (several buckets, ragged slice groups)."""
+    import torch
+    from sscRaft import device as dv
+
+    sizes = [64, 40, 96, 64, 40, 72, 64, 96, 40]
+    K = 4
+    data = []
+    for k, n in enumerate(sizes):
+        img = ph.phantoms(n, 1, config=6, first=40 + k)
+        ang = ph.uniform_angles(n)
+        p = ph.poisson_log(oracle.radon(img, ang), 1e4, seed=40 + k)[0]
+        data.append((torch.from_numpy(p).cuda(), torch.from_numpy(np.maximum(p, 0)).cuda(), ang,
+                     [0, 0, 1, 0, 2.0 / n, 2.0 / n, 0]))
+    for store in ("f32", "f16"):
+        outs = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("RAFT_BATCH_QUEUE", mode)
+            fouts = [torch.zeros((n, n), device="cuda") for n in sizes]
+            ems = [torch.ones((n, n), device="cuda") for n in sizes]
+            dv.batch_device([d[0] for d in data], [d[2] for d in data], [d[3] for d in data], pad=2, filter_type=7,
+                            store=store, fbp_out=fouts, ys=[d[1] for d in data], em=ems, iterations=K)
+            outs[mode] = ([f.cpu().numpy() for f in fouts], [e.cpu().numpy() for e in ems])
+        for a, b in zip(outs["1"][0] + outs["1"][1], outs["0"][0] + outs["0"][1]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_eem_schedule_matches_per_slice_runs(gpu):
+    """raft_eem_device_sched: slice s after iterations[s] iterations equals a
+    run of that slice alone for that many iterations, bit for bit, including
+    slices that stop inside a slice group and slices that never start; the
+    staged and direct projector forms both."""
+    import torch
+    from sscRaft import device as dv
+
+    counts = [7, 7, 5, 4, 4, 2, 1, 0, 0]
+    for n, a in [(64, 64), (96, 24)]:  # staged form, direct form (sparse angles)
+        img = ph.phantoms(n, len(counts), config=8, first=n)
+        ang = ph.uniform_angles(a)
+        y = torch.from_numpy(np.maximum(ph.poisson_log(oracle.radon(img, ang), 1e4, seed=n), 0)
+                             .astype(np.float32)).cuda()
+        x0 = torch.from_numpy(np.random.default_rng(n).uniform(0.5, 1.5, (len(counts), n, n)).astype(np.float32)).cuda()
+        xs = dv.eem_device_sched(y, ang, counts, obj=x0.clone())
+        for s, k in enumerate(counts):
+            if k == 0:
+                assert torch.equal(xs[s], x0[s])
+                continue
+            x1, _, _ = dv.eem_device(y[s:s + 1].contiguous(), ang, k, obj=x0[s:s + 1].clone(), with_kl=False)
+            assert torch.equal(xs[s], x1[0]), (n, a, s, k)
+    with pytest.raises(ValueError, match="non-increasing"):
+        dv.eem_device_sched(y[:3].contiguous(), ang, [1, 2, 1])
