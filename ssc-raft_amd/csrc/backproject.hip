@@ -16,7 +16,12 @@
 // 16-byte loads) into LDS, then every thread sweeps the chunk:
 //     u = fma(X_p, cA, fma(Y_q, cB, c0));  T = (int)u;  acc[0..SB) += win[T]
 // in IEEE double for the bin coordinate (bit-identical to the CPU oracle's
-// expression) and FP32 for the values.  The staged rows already carry
+// expression) and FP32 for the values.  The FP32-row kernels also come in a
+// "pair" form (PAIR = 2): one workgroup takes two 4-slice groups, stages both
+// groups' windows side by side in LDS and reads both entries of a bin (two
+// 16-byte LDS reads, the second at an immediate offset), so every FP64 bin
+// index and LDS address serves 8 slices.  Each slice's sum is formed in the
+// same order in both forms, so they agree bit for bit.  The staged rows already carry
 // dtheta_a * angular scale and zeros outside the valid bin range, so the
 // inner loop has no range test and no multiply.  Per-chunk partial sums are
 // added to the running total (two-level summation over angles).
@@ -89,19 +94,27 @@ template <> struct Acc<__half> {
 };
 
 // LDS of one workgroup: the staged windows of CHUNK angles and their coefficients
-template <int CHUNK> struct BpSmem {
-    uint4 win[CHUNK * BP_SLOT];
+template <int PAIR> constexpr int bp_slot() { return PAIR == 2 ? BP_SLOT_PAIR : BP_SLOT; }
+template <int CHUNK, int PAIR> struct BpSmem {
+    uint4 win[PAIR * CHUNK * bp_slot<PAIR>()]; // PAIR halves of CHUNK * slot entries
     double cA[CHUNK], cB[CHUNK];
     int tl[CHUNK], w[CHUNK], boff[CHUNK];
 };
 
-// One 64 x 32 pixel tile of slice group g.  Starts with a barrier (the LDS
-// may still be read by the previous tile of a persistent workgroup) and
-// touches LDS only through sm.
-template <typename QT, int MODE, int CHUNK>
-__device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSmem<CHUNK>& sm)
+// One 64 x 32 pixel tile of slice groups PAIR g .. PAIR g + PAIR - 1 (a
+// missing second group of a pair reads zeros and stores nothing).  Starts
+// with a barrier (the LDS may still be read by the previous tile of a
+// persistent workgroup) and touches LDS only through sm.
+template <typename QT, int MODE, int CHUNK, int PAIR>
+__device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSmem<CHUNK, PAIR>& sm)
 {
-    constexpr int SB = Acc<QT>::SB;
+    static_assert(PAIR == 1 || (PAIR == 2 && Acc<QT>::SB == 4), "pairs are formed of FP32-row groups");
+    constexpr int SB1 = Acc<QT>::SB;   // slices per 16-byte entry
+    constexpr int SB = SB1 * PAIR;     // slices per pixel of this workgroup
+    constexpr int SLOT = bp_slot<PAIR>();  // LDS entries per staged angle window
+    constexpr int HALF = CHUNK * SLOT * (int)sizeof(uint4); // byte offset of the second half
+    const int g = gp * PAIR;           // first slice group
+    const bool second = PAIR == 2 && (g + 1) * SB1 < a.S; // the pair's second group exists
     constexpr int TX = BP_TILE_X, TY = BP_TILE_Y;
     uint4* win = sm.win;
     double* s_cA = sm.cA;
@@ -154,17 +167,22 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSme
             s_cB[tid] = cB;
             s_tl[tid] = tl;
             s_w[tid] = th - tl + 1;
-            s_boff[tid] = (tid * BP_SLOT - tl) * (int)sizeof(uint4); // wraps for large tl: used modulo 2^32
+            s_boff[tid] = (tid * SLOT - tl) * (int)sizeof(uint4); // wraps for large tl: used modulo 2^32
 
         }
         __syncthreads();
         // ---- stage the windows: 16-byte coalesced loads ----
-        for (int e = tid; e < nA * BP_SLOT; e += 256) {
-            const int al = e / BP_SLOT;
-            const int w = e - al * BP_SLOT;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (w < s_w[al]) v = qg[(size_t)(a0 + al) * a.W + (s_tl[al] - a.tlo) + w];
+        for (int e = tid; e < nA * SLOT; e += 256) {
+            const int al = e / SLOT;
+            const int w = e - al * SLOT;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u), v2 = make_uint4(0u, 0u, 0u, 0u);
+            const size_t src = (size_t)(a0 + al) * a.W + (s_tl[al] - a.tlo) + w;
+            if (w < s_w[al]) {
+                v = qg[src];
+                if (second) v2 = qg[src + (size_t)a.A * a.W];
+            }
             win[e] = v;
+            if (PAIR == 2) win[CHUNK * SLOT + e] = v2;
         }
         __syncthreads();
         // ---- sweep ----
@@ -173,7 +191,7 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSme
         // depend on CHUNK (a multiple of BP_GROUP), so every CHUNK variant
         // gives bit-identical sums.  FP16 rows sum into the totals directly
         // (their error is the FP16 storage; 64 fewer registers, occupancy 3).
-        constexpr bool TWO_LEVEL = SB == 4;
+        constexpr bool TWO_LEVEL = SB1 == 4;
         static_assert(CHUNK % BP_GROUP == 0, "the angle chunk must hold whole summation groups");
         const char* wbase = reinterpret_cast<const char*>(win);
 #pragma unroll 1
@@ -199,16 +217,22 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSme
 #pragma unroll
                     for (int p = 0; p < 4; ++p) T[q][p] = (int)fma(X[p], cA, rt);
                 }
-                uint4 v[2][4];
+                uint4 v[2][4][PAIR];
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        const char* e = wbase + (((unsigned)T[q][p] << 4) + boff);
+#pragma unroll
+                        for (int h = 0; h < PAIR; ++h) v[q][p][h] = *reinterpret_cast<const uint4*>(e + h * HALF);
+                    }
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
 #pragma unroll
                     for (int p = 0; p < 4; ++p)
-                        v[q][p] = *reinterpret_cast<const uint4*>(wbase + (((unsigned)T[q][p] << 4) + boff));
 #pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) Acc<QT>::add(TWO_LEVEL ? part[q][p] : tot[q][p], v[q][p]);
+                        for (int h = 0; h < PAIR; ++h)
+                            Acc<QT>::add((TWO_LEVEL ? part[q][p] : tot[q][p]) + h * SB1, v[q][p][h]);
             }
             if constexpr (TWO_LEVEL) {
 #pragma unroll
@@ -232,29 +256,35 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSme
             const int i = i0 + tx + 16 * p;
             if (i >= a.n) continue;
             if (MODE == 1) {
-                // eEM update on the interleaved image (SB == 4 here)
-                const size_t pix = ((size_t)g * nn + (size_t)j * a.n + i) * 4;
-                float4 nv;
-                if (a.norm_il) {
-                    nv = *reinterpret_cast<const float4*>(a.norm + pix);
-                } else {
-                    const float n1 = a.norm[(size_t)j * a.n + i];
-                    nv = make_float4(n1, n1, n1, n1);
+                // eEM update on the interleaved image (4 slices per group here)
+#pragma unroll
+                for (int h = 0; h < PAIR; ++h) {
+                    const int gh = g + h;
+                    if (h > 0 && !second) break;
+                    const size_t pix = ((size_t)gh * nn + (size_t)j * a.n + i) * 4;
+                    float4 nv;
+                    if (a.norm_il) {
+                        nv = *reinterpret_cast<const float4*>(a.norm + pix);
+                    } else {
+                        const float n1 = a.norm[(size_t)j * a.n + i];
+                        nv = make_float4(n1, n1, n1, n1);
+                    }
+                    // slices at or past S (the padding of the last group, or slices
+                    // whose iterations are done in a scheduled run) keep their values
+                    float4 xv = *reinterpret_cast<const float4*>(a.x + pix);
+                    const int s0 = gh * 4;
+                    const float* t4 = tot[q][p] + 4 * h;
+                    xv.x = s0 + 0 < a.S ? xv.x * t4[0] / nv.x : xv.x;
+                    xv.y = s0 + 1 < a.S ? xv.y * t4[1] / nv.y : xv.y;
+                    xv.z = s0 + 2 < a.S ? xv.z * t4[2] / nv.z : xv.z;
+                    xv.w = s0 + 3 < a.S ? xv.w * t4[3] / nv.w : xv.w;
+                    *reinterpret_cast<float4*>(a.out + pix) = xv;
+                    if (a.xt) *reinterpret_cast<float4*>(a.xt + ((size_t)gh * nn + (size_t)i * a.n + j) * 4) = xv;
                 }
-                // slices at or past S (the padding of the last group, or slices
-                // whose iterations are done in a scheduled run) keep their values
-                float4 xv = *reinterpret_cast<const float4*>(a.x + pix);
-                const int s0 = g * 4;
-                xv.x = s0 + 0 < a.S ? xv.x * tot[q][p][0] / nv.x : xv.x;
-                xv.y = s0 + 1 < a.S ? xv.y * tot[q][p][1] / nv.y : xv.y;
-                xv.z = s0 + 2 < a.S ? xv.z * tot[q][p][2] / nv.z : xv.z;
-                xv.w = s0 + 3 < a.S ? xv.w * tot[q][p][3] / nv.w : xv.w;
-                *reinterpret_cast<float4*>(a.out + pix) = xv;
-                if (a.xt) *reinterpret_cast<float4*>(a.xt + ((size_t)g * nn + (size_t)i * a.n + j) * 4) = xv;
             } else {
 #pragma unroll
                 for (int s = 0; s < SB; ++s) {
-                    const int sl = g * SB + s;
+                    const int sl = g * SB1 + s;
                     if (sl < a.S) a.out[(size_t)sl * nn + (size_t)j * a.n + i] = tot[q][p][s];
                 }
             }
@@ -262,23 +292,24 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int g, BpSme
     }
 }
 
-template <typename QT, int MODE, int CHUNK>
-__global__ __launch_bounds__(256) void k_backproject(BpKArgs a)
+// the pair form is held to 256 VGPRs: two waves per SIMD, two workgroups per CU
+template <typename QT, int MODE, int CHUNK, int PAIR>
+__global__ __launch_bounds__(256, PAIR) void k_backproject(BpKArgs a)
 {
-    __shared__ BpSmem<CHUNK> sm;
-    bp_tile<QT, MODE, CHUNK>(a, blockIdx.x, blockIdx.y, sm);
+    __shared__ BpSmem<CHUNK, PAIR> sm;
+    bp_tile<QT, MODE, CHUNK, PAIR>(a, blockIdx.x, blockIdx.y, sm);
 }
 
 // Persistent work-queue form: a resident grid takes items from a device
 // counter; item i of launch segment b (items [first[b], first[b + 1])) is
-// tile (i - first[b]) % tiles of slice group (i - first[b]) / tiles of the
-// volume described by descs[b].  The host orders the segments by cost per
+// tile (i - first[b]) % tiles of slice group (or pair of groups)
+// (i - first[b]) / tiles of the volume described by descs[b].  The host orders the segments by cost per
 // item (angles), largest first.  The last workgroup to leave clears the
 // counters for the next launch.
-template <typename QT, int MODE, int CHUNK>
-__global__ __launch_bounds__(256) void k_backproject_queue(const BpKArgs* __restrict__ descs, BpQueue qu)
+template <typename QT, int MODE, int CHUNK, int PAIR>
+__global__ __launch_bounds__(256, PAIR) void k_backproject_queue(const BpKArgs* __restrict__ descs, BpQueue qu)
 {
-    __shared__ BpSmem<CHUNK> sm;
+    __shared__ BpSmem<CHUNK, PAIR> sm;
     __shared__ int s_item;
     for (;;) {
         __syncthreads(); // every thread has read the previous item
@@ -292,7 +323,7 @@ __global__ __launch_bounds__(256) void k_backproject_queue(const BpKArgs* __rest
         b = __builtin_amdgcn_readfirstlane(b);
         const int local = item - qu.first[b];
         const int tiles = qu.tiles[b];
-        bp_tile<QT, MODE, CHUNK>(descs[b], local % tiles, local / tiles, sm);
+        bp_tile<QT, MODE, CHUNK, PAIR>(descs[b], local % tiles, local / tiles, sm);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -429,40 +460,62 @@ void bp_desc_fill(const BpArgs& b, int store, void* dst)
     std::memcpy(dst, &k, sizeof k);
 }
 
-// kernel variant: (FP32 rows, store) x (FBP sums, eEM update) x angles per LDS round
-template <int CHUNK> static const void* bp_kernel(int store, int mode, bool queue)
+// kernel variant: (FP32 rows, store) x (FBP sums, eEM update) x angles per LDS
+// round x (one group or a pair of FP32-row groups per workgroup)
+template <int CHUNK, int PAIR> static const void* bp_kernel(int store, int mode, bool queue)
 {
     if (mode == 1)
-        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK>)
-                     : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK>);
-    if (store == RAFT_STORE_F16)
-        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK>)
-                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK>);
-    return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK>)
-                 : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK>);
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK, PAIR>)
+                     : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK, PAIR>);
+    if (store == RAFT_STORE_F16) {
+        if (PAIR != 1) fail(RAFT_EINVAL, "internal: the pair form takes FP32 rows");
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK, 1>)
+                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK, 1>);
+    }
+    return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK, PAIR>)
+                 : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK, PAIR>);
 }
 
-static const void* bp_kernel_any(int store, int mode, int chunk, bool queue)
+static const void* bp_kernel_any(int store, int mode, int variant, bool queue)
 {
     if (mode == 1 && store != RAFT_STORE_F32) fail(RAFT_EINVAL, "eEM update runs on FP32 rows");
-    switch (chunk) {
-    case 16: return bp_kernel<16>(store, mode, queue);
-    case 48: return bp_kernel<48>(store, mode, queue);
-    case 0:
-    case 32: return bp_kernel<32>(store, mode, queue);
-    default: fail(RAFT_EINVAL, "internal: unsupported backprojection angle chunk " + std::to_string(chunk));
+    if (!bp_variant_supported(variant, mode == 1 ? RAFT_STORE_F32 : store))
+        fail(RAFT_EINVAL, "internal: unsupported backprojection variant " + std::to_string(variant));
+    switch (variant == 0 ? BP_CHUNK : variant) {
+    case 16: return bp_kernel<16, 1>(store, mode, queue);
+    case 32: return bp_kernel<32, 1>(store, mode, queue);
+    case 48: return bp_kernel<48, 1>(store, mode, queue);
+    case BP_PAIR | 16: return bp_kernel<16, 2>(store, mode, queue);
+    default: return bp_kernel<32, 2>(store, mode, queue); // BP_PAIR | 32
     }
 }
 
-bool bp_chunk_supported(int chunk) { return chunk == 16 || chunk == 32 || chunk == 48; }
+bool bp_variant_supported(int variant, int store)
+{
+    switch (variant) {
+    case 0:
+    case 16:
+    case 32:
+    case 48: return true;
+    case BP_PAIR | 16:
+    case BP_PAIR | 32: return store == RAFT_STORE_F32;
+    default: return false;
+    }
+}
+
+int bp_slices_per_item(int store, int mode, int variant)
+{
+    const int sb = (store == RAFT_STORE_F16 && mode == 0) ? BP_SLICES_F16 : BP_SLICES_F32;
+    return (variant & BP_PAIR) ? 2 * sb : sb;
+}
 
 void launch_backproject(const BpArgs& b, int store, hipStream_t st)
 {
     if (b.S <= 0 || b.n <= 0) return;
     bp_check_rows(b, store);
     BpKArgs k = bp_kargs(b);
-    const int sb = (store == RAFT_STORE_F16) ? 8 : 4;
-    const int groups = (b.S + sb - 1) / sb;
+    const int per = bp_slices_per_item(store, b.mode, b.chunk);
+    const int groups = (b.S + per - 1) / per;
     if (groups > 65535) fail(RAFT_EUNSUPPORTED, "too many slice groups for one backprojection launch");
     const void* fn = bp_kernel_any(store, b.mode, b.chunk, false);
     void* args[] = {&k};

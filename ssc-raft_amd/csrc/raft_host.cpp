@@ -197,7 +197,17 @@ static std::string bp_tune_key(int n, int A, int W, int store, int mode, int gro
     snprintf(k, sizeof k, "bp/n=%d/A=%d/W=%d/store=%d/mode=%d/groups=%d", n, A, W, store, mode, groups);
     return k;
 }
-static const std::vector<int> kBpChunks = {BP_CHUNK, 16, 48};
+// backprojector variants the tuner times: angle chunks, and for FP32 rows the
+// pair form (two slice groups per workgroup) where its narrower window holds
+// the tile's bins (rr = bin steps per pixel step); every variant gives the
+// same sums
+static const std::vector<int>& bp_cands(int store, double rr = 1.0)
+{
+    static const std::vector<int> f32 = {BP_CHUNK, 16, 48, BP_PAIR | 32, BP_PAIR | 16};
+    static const std::vector<int> single = {BP_CHUNK, 16, 48};
+    const bool pair_fits = rr * std::hypot((double)(BP_TILE_X - 1), (double)(BP_TILE_Y - 1)) + 5.0 <= BP_SLOT_PAIR;
+    return (store == RAFT_STORE_F16 || !pair_fits) ? single : f32;
+}
 
 static std::string proj_tune_key(const RayGeom& g, int em, int groups)
 {
@@ -582,7 +592,7 @@ static void fbp_run(Device& D, float* d_obj, const float* d_tomo, const float* a
         ba.q_bytes = (size_t)((B + SB - 1) / SB) * per_group;
         // angle chunk: tuned on this block's rows (the timed launches write
         // the same output as the final one)
-        ba.chunk = tune_pick(bp_tune_key(N, A, g.width, store, 0, (nb + SB - 1) / SB), kBpChunks, st,
+        ba.chunk = tune_pick(bp_tune_key(N, A, g.width, store, 0, (nb + SB - 1) / SB), bp_cands(store), st,
                              [&](int c) {
                                  BpArgs t = ba;
                                  t.chunk = c;
@@ -948,7 +958,7 @@ static void eem_device(int dev, float* d_obj, const float* d_tomo, const float* 
         tb.cA = dcA.as<double>(); tb.cB = dcB.as<double>(); tb.xc = g.xc; tb.c0 = g.c0;
         tb.n = n; tb.A = A; tb.S = nb; tb.tlo = g.tlo; tb.W = g.width; tb.mode = 1;
         tb.q_bytes = (size_t)Gb * A * g.width * 16;
-        bp_chunk = tune_pick(key, kBpChunks, st, [&](int c) {
+        bp_chunk = tune_pick(key, bp_cands(RAFT_STORE_F32, rr), st, [&](int c) {
             tb.chunk = c;
             launch_backproject(tb, RAFT_STORE_F32, st);
         });

@@ -71,8 +71,15 @@ struct BpGeom {
 constexpr int BP_TILE_X = 64;
 constexpr int BP_TILE_Y = 32;
 constexpr int BP_CHUNK = 32;   // angles staged per LDS round (default; 16 and 48 are autotuner variants)
+// variant flag: FP32 rows, two 4-slice groups per workgroup (8 slices per bin
+// lookup); BP_PAIR | 16 and BP_PAIR | 32 are the autotuner's pair variants
+constexpr int BP_PAIR = 1024;
 constexpr int BP_GROUP = 16;   // angles per partial sum of the FP32 two-level summation
 constexpr int BP_SLOT = 80;    // LDS entries per staged angle window (>= |(63, 31)| + 5)
+// the pair form's window (two workgroups per CU fit 160 KB of LDS at 32
+// angles per round): takes geometries whose window rr |(63, 31)| + 5 <= 76,
+// rr = bin steps per pixel step (1 for FBP; (nrays - 1)/(n - 1) for eEM)
+constexpr int BP_SLOT_PAIR = 76;
 // staged rows hold the bins of BP_SLICES_F32 (FP32) or BP_SLICES_F16 (FP16)
 // slices side by side: [group][angle][bin][slice]
 constexpr int BP_SLICES_F32 = 4;
@@ -155,11 +162,14 @@ struct BpArgs {
     int n, A, S, tlo, W;
     int mode;           // 0 = store the sums (FBP, backones), 1 = EM update x * bp / norm
     int norm_il = 0;    // mode 1: norm is per slice, interleaved [G][n][n][4] (tEM)
-    int chunk = 0;      // angles per LDS round (16, 32 or 48; 0 = 32)
+    int chunk = 0;      // variant: angles per LDS round (16, 32 or 48; 0 = 32), | BP_PAIR for the pair form
     size_t q_bytes = 0; // bytes of the buffer behind q when known: launches that would read past it are refused
 };
 void launch_backproject(const BpArgs& a, int store, hipStream_t st);
-bool bp_chunk_supported(int chunk);
+// whether variant (see BpArgs::chunk) exists for rows of this store
+bool bp_variant_supported(int variant, int store);
+// slices one workgroup (one queue item) of this variant covers
+int bp_slices_per_item(int store, int mode, int variant);
 // tiles per slice group of an n x n image
 int bp_tiles(int n);
 

@@ -1,8 +1,11 @@
 """The tile autotuner on the GPU: the first call of a geometry times every
-candidate (backprojector angle chunks 32/16/48, staged-projector ray-chunk
-counts) and keeps the fastest; every candidate computes the same values, so
-a choice changes speed only -- checked here by forcing each candidate through
-the cache file and comparing outputs bit for bit."""
+candidate (backprojector angle chunks 32/16/48 and, for FP32 rows, the pair
+form that takes two slice groups per workgroup at 32 or 16 angles per round;
+staged-projector ray-chunk counts) and keeps the fastest; every candidate
+computes the same values, so a choice changes speed only -- checked here by
+forcing each candidate through the cache file and comparing outputs bit for
+bit.  The slice counts leave the last pair of slice groups without its second
+group (FBP) or with a partial second group (eEM)."""
 import json
 
 import numpy as np
@@ -12,6 +15,9 @@ import oracle
 from sscRaft import phantom as ph
 
 pytestmark = pytest.mark.gpu
+
+PAIR = 1024  # BP_PAIR: the pair-form flag of a backprojector variant
+BP_VARIANTS = [16, 32, 48, PAIR | 16, PAIR | 32]
 
 
 def _force(tmp_path, monkeypatch, entries):
@@ -27,7 +33,7 @@ def test_tuner_times_candidates_and_every_chunk_agrees(gpu, tmp_path, monkeypatc
     import torch
     from sscRaft import device as dv
 
-    n, a, s = 200, 180, 6
+    n, a, s = 200, 180, 10
     y, ang = ph.poisson_log(oracle.radon(ph.phantoms(n, s, config=7), ph.uniform_angles(a)), 1e4, seed=5), \
         ph.uniform_angles(a)
     paramf, parami = oracle.fbp_params(n, a, s, pad=2, filter_type=7, pixel=2.0 / n)
@@ -39,10 +45,10 @@ def test_tuner_times_candidates_and_every_chunk_agrees(gpu, tmp_path, monkeypatc
     keys = [k for k in choices if k.startswith(f"bp/n={n}/A={a}/")]
     assert len(keys) == 1, choices
     rec = choices[keys[0]]
-    assert sorted(c for c, _ in rec["ms"]) == [16, 32, 48] and rec["choice"] in (16, 32, 48)
+    assert sorted(c for c, _ in rec["ms"]) == BP_VARIANTS and rec["choice"] in BP_VARIANTS
     assert json.loads((tmp_path / "fresh.json").read_text())[keys[0]]["choice"] == rec["choice"]
     print(f"tuned {keys[0]}: {rec}")
-    for chunk in (16, 32, 48):
+    for chunk in BP_VARIANTS:
         _force(tmp_path, monkeypatch, {keys[0]: chunk})
         out = dv.fbp_device(t, ang, paramf, parami).cpu().numpy()
         np.testing.assert_array_equal(out, ref)
@@ -72,7 +78,8 @@ def test_eem_and_radon_variants_agree(gpu, tmp_path, monkeypatch):
     assert len(bp_key) == 1 and len(pr_key) == 1 and len(rd_key) == 1, choices
     rcs = [c for c, _ in choices[pr_key[0]]["ms"]]
     assert len(rcs) >= 2
-    for chunk, rc in [(16, rcs[0]), (48, rcs[-1]), (32, rcs[1])]:
+    assert sorted(c for c, _ in choices[bp_key[0]]["ms"]) == BP_VARIANTS
+    for chunk, rc in [(16, rcs[0]), (48, rcs[-1]), (32, rcs[1]), (PAIR | 32, rcs[0]), (PAIR | 16, rcs[1])]:
         _force(tmp_path, monkeypatch, {bp_key[0]: chunk, pr_key[0]: rc, rd_key[0]: rc})
         x1, kl1, _ = dv.eem_device(y, ang, K)
         assert torch.equal(x1, x0)
