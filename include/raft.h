@@ -93,6 +93,22 @@ void raft_clear_error(void);
 int raft_fbp_device(int device, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
                     const int* parami, int store, void* stream);
 
+/* raft_fbp_device that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, same kernels and the same output bit for bit; the filter and
+ * angle tables and the filtered rows live in one fixed device array of
+ * RAFT_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends, and the slices run in blocks of as many slice groups (4 slices
+ * for F32 rows, 8 for F16) as fit there.  A geometry whose tables plus one
+ * slice group of rows (nangles x staged width x 16 bytes) exceed the array is
+ * refused with RAFT_EUNSUPPORTED (2048 angles x 2048 rays at pad 2 needs
+ * 90.75 MiB per group).  Synchronous: it waits for its stream before returning.
+ * Calls on one device share its array and are serialised by the library. */
+#define RAFT_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int raft_fbp_device_noalloc(int device, float* d_obj, const float* d_tomo, const float* angles,
+                            const float* paramf, const int* parami, int store, void* stream);
+
 /* Radon on one device: d_obj [nslices][Ny][Nx] -> d_proj [nslices][nangles][nrays]. */
 int raft_radon_device(int device, float* d_proj, const float* d_obj, const float* angles, int Nx, int Ny,
                       int nrays, int nangles, int nslices, float ax, float ay, void* stream);

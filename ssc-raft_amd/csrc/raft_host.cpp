@@ -494,12 +494,19 @@ static void filter_host_tables(const FbpCfg& c, FilterHost& F)
     }
 }
 
-// the filter tables on the device, cached per (n, filter, reg, paganin, offset, pixel)
-static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
+// the key of a filter table set: (n, filter, reg, paganin, offset, pixel)
+static std::string filter_key(const FbpCfg& c)
 {
     char key[256];
     snprintf(key, sizeof key, "%d/%d/%.17g/%.17g/%.17g/%.17g", c.npad, c.filter, c.reg, c.paganin, c.offset,
              c.pixel);
+    return key;
+}
+
+// the filter tables on the device, cached per filter_key
+static FilterTables& filter_tables(Device& D, const FbpCfg& c, hipStream_t st)
+{
+    const std::string key = filter_key(c);
     auto it = D.filters.find(key);
     if (it != D.filters.end()) return *it->second;
     std::unique_ptr<FilterTables> F(new FilterTables());
@@ -619,6 +626,109 @@ static void fbp_device(int dev, float* d_obj, const float* d_tomo, const float* 
     Device& D = device_state(dev);
     std::lock_guard<std::mutex> lk(D.mu);
     fbp_run(D, d_obj, d_tomo, angles, c, store, st, true);
+}
+
+// FBP with every intermediate in the library's fixed device scratch array
+// (noalloc.hip) instead of the workspace: the filter tables, the angle
+// cosines, sines and weights, then as many slice groups of filtered rows as
+// fit.  The volume runs in blocks of that many groups through the same
+// kernels with the same arguments as fbp_run, so the output is fbp_run's bit
+// for bit (slice groups are independent, and every angle chunk of the
+// backprojector gives the same sums).  Nothing here allocates device memory:
+// no workspace, no table arena, no cached device filter tables, no tuning
+// runs (the backprojector takes the tuner's cached chunk or its default).
+// Nothing in the scratch must start at zero: the filter writes every entry of
+// the rows the backprojector reads, and the tables are copied in whole.
+static std::mutex g_host_filters_mu;
+static std::map<std::string, std::unique_ptr<FilterHost>> g_host_filters; // host tables per filter_key
+
+static void fbp_device_noalloc(int dev, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                               const int* parami, int store, hipStream_t st)
+{
+    const FbpCfg c = decode_fbp(paramf, parami);
+    if (!angles) fail(RAFT_EINVAL, "null angles");
+    if (store != RAFT_STORE_F32 && store != RAFT_STORE_F16) fail(RAFT_EINVAL, "store must be F32 or F16");
+    if (c.S == 0) return;
+    if (!d_obj || !d_tomo) fail(RAFT_EINVAL, "null data pointer");
+    const int A = c.A, N = c.N;
+    BpGeom g;
+    std::vector<float> wa;
+    fbp_geometry(c, angles, g, wa);
+    const FilterHost* F = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_host_filters_mu);
+        auto& slot = g_host_filters[filter_key(c)];
+        if (!slot) {
+            std::unique_ptr<FilterHost> h(new FilterHost());
+            filter_host_tables(c, *h);
+            slot = std::move(h);
+        }
+        F = slot.get();
+    }
+
+    // scratch layout: tables (256-byte aligned pieces), then the rows
+    std::vector<char> host;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        if (bytes) std::memcpy(host.data() + off, src, bytes);
+        return off;
+    };
+    const size_t oH = put(F->H.data(), F->H.size() * sizeof(float2));
+    const size_t oTw = put(F->tw.data(), F->tw.size() * sizeof(float2));
+    const size_t oG = put(F->g.data(), F->g.size() * sizeof(double));
+    const size_t oA = put(g.cA.data(), (size_t)A * sizeof(double));
+    const size_t oB = put(g.cB.data(), (size_t)A * sizeof(double));
+    const size_t oW = put(wa.data(), (size_t)A * sizeof(float));
+    const size_t oRows = (host.size() + 255) & ~(size_t)255;
+
+    const int SB = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
+    const size_t per_group = (size_t)A * g.width * 16;
+    const size_t cap = (size_t)RAFT_NOALLOC_SCRATCH_BYTES;
+    if (oRows > cap || cap - oRows < per_group)
+        fail(RAFT_EUNSUPPORTED, "raft_fbp_device_noalloc: tables (" + std::to_string(oRows) +
+                                    " bytes) and one slice group of filtered rows (" + std::to_string(per_group) +
+                                    " bytes) exceed the fixed scratch of " + std::to_string(cap) + " bytes");
+    const int groups_fit = (int)std::min<size_t>((cap - oRows) / per_group, 1 << 20);
+    int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
+    B = std::min(c.S, ((B + SB - 1) / SB) * SB);
+    B = std::min(B, groups_fit * SB);
+
+    bind_device(dev);
+    Device& D = device_state(dev);
+    std::lock_guard<std::mutex> lk(D.mu);
+    char* base = static_cast<char*>(noalloc_scratch(dev));
+    RAFT_HIP(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    void* q = base + oRows;
+    const size_t q_bytes = (size_t)((B + SB - 1) / SB) * per_group;
+    const int chunk = tune_cached(bp_tune_key(N, A, g.width, store, 0, (B + SB - 1) / SB), BP_CHUNK);
+
+    for (int s0 = 0; s0 < c.S; s0 += B) {
+        const int nb = std::min(B, c.S - s0);
+        FilterArgs fa;
+        fa.tomo = d_tomo + (size_t)s0 * A * N;
+        fa.q = q;
+        fa.H = reinterpret_cast<const float2*>(base + oH);
+        fa.tw = reinterpret_cast<const float2*>(base + oTw);
+        fa.g = F->g.empty() ? nullptr : reinterpret_cast<const double*>(base + oG);
+        fa.wa = reinterpret_cast<const float*>(base + oW);
+        fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = g.tlo; fa.W = g.width;
+        fa.none = (c.filter == 0);
+        launch_fbp_filter(fa, F->plan, store, st);
+        BpArgs ba;
+        ba.q = q;
+        ba.out = d_obj + (size_t)s0 * N * N;
+        ba.x = nullptr; ba.norm = nullptr; ba.xt = nullptr;
+        ba.cA = reinterpret_cast<const double*>(base + oA);
+        ba.cB = reinterpret_cast<const double*>(base + oB);
+        ba.xc = g.xc; ba.c0 = g.c0;
+        ba.n = N; ba.A = A; ba.S = nb; ba.tlo = g.tlo; ba.W = g.width; ba.mode = 0;
+        ba.q_bytes = q_bytes;
+        ba.chunk = bp_variant_supported(chunk, store) ? chunk : BP_CHUNK;
+        launch_backproject(ba, store, st);
+    }
+    // the host copy of the tables stays alive until the stream is done
+    RAFT_HIP(hipStreamSynchronize(st));
 }
 
 // ---------------------------------------------------------------------------
@@ -2050,6 +2160,14 @@ int raft_fbp_device(int device, float* d_obj, const float* d_tomo, const float* 
 {
     return guarded([&] {
         fbp_device(device, d_obj, d_tomo, angles, paramf, parami, store, static_cast<hipStream_t>(stream));
+    });
+}
+
+int raft_fbp_device_noalloc(int device, float* d_obj, const float* d_tomo, const float* angles, const float* paramf,
+                            const int* parami, int store, void* stream)
+{
+    return guarded([&] {
+        fbp_device_noalloc(device, d_obj, d_tomo, angles, paramf, parami, store, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -257,6 +257,10 @@ void launch_background(float* data, const float* flat, const float* dark, int R,
 // p[0 .. n) = v (a kernel, so it may be captured into a graph)
 void launch_fill(float* p, long long n, float v, hipStream_t st);
 
+// device address of this device's copy of the fixed scratch array of
+// RAFT_NOALLOC_SCRATCH_BYTES (noalloc.hip); dev must be the current device
+void* noalloc_scratch(int dev);
+
 // autotuner (autotune.cpp): the cached choice for key, else the fastest of
 // cands timed with launch(c) on stream st (or cands[0] when tuning is off)
 bool tune_enabled();

@@ -28,24 +28,32 @@ def _check_tensor(t, name):
         raise ValueError(f"{name} must be a contiguous float32 CUDA tensor")
 
 
-def fbp_device(tomo, angles, paramf, parami, out=None, store="f32", stream=None):
-    """FBP of tomo [S, A, N] (CUDA float32) -> out [S, N, N]."""
+def fbp_device(tomo, angles, paramf, parami, out=None, store="f32", stream=None, noalloc=False):
+    """FBP of tomo [S, A, N] (CUDA float32) -> out [S, N, N].
+    noalloc: run raft_fbp_device_noalloc (no device allocation by the
+    library: its intermediates live in a fixed per-device scratch array;
+    out must then be given, since allocating it here would defeat the point)."""
     import torch
 
     _check_tensor(tomo, "tomo")
     N, A, S = int(parami[0]), int(parami[1]), int(parami[2])
     if out is None:
+        if noalloc:
+            raise ValueError("noalloc=True needs a preallocated out tensor")
         out = torch.empty((S, N, N), dtype=torch.float32, device=tomo.device)
     _check_tensor(out, "out")
     angles = CNICE(numpy.asarray(angles))
     paramf = CNICE(numpy.asarray(paramf), numpy.float32)
     parami = CNICE(numpy.asarray(parami), numpy.int32)
-    rc = libraft.raft_fbp_device(tomo.device.index or 0, ctypes.c_void_p(out.data_ptr()),
+    if out.numel() < S * N * N or tomo.numel() < S * A * N or angles.shape[0] < A:
+        raise ValueError("tomo, out or angles smaller than parami says")
+    fn = libraft.raft_fbp_device_noalloc if noalloc else libraft.raft_fbp_device
+    rc = fn(tomo.device.index or 0, ctypes.c_void_p(out.data_ptr()),
                                  ctypes.c_void_p(tomo.data_ptr()), angles.ctypes.data_as(ctypes.c_void_p),
                                  paramf.ctypes.data_as(ctypes.c_void_p), parami.ctypes.data_as(ctypes.c_void_p),
                                  1 if store == "f16" else 0, _stream(tomo, stream))
     if rc:
-        raft_check("raft_fbp_device")
+        raft_check("raft_fbp_device_noalloc" if noalloc else "raft_fbp_device")
     return out
 
 
