@@ -555,19 +555,106 @@ __global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int 
     }
 }
 
-// out[sl][x][y] = in[sl][y][x] for s x s complex planes (32 x 32 LDS tiles)
+// Fused form of k_bst_p2c and the inverse FFT of the grid rows (k_ifft_rows):
+// one workgroup per destination row tyd of the shifted frequency grid and
+// group of up to BST_SPB slices.  The FP64 polar coordinates of the row's
+// cells do not depend on the slice and are computed once; for each slice the
+// interpolated cells go straight into LDS, the row is inverse-transformed
+// there (conjugate, forward FFT, conjugate), and only the N columns
+// [L, L + N) the crop keeps are written, as rows of out [S][s][N].  The grid
+// is never written to HBM.  The expressions are those of the two unfused
+// kernels, so the values are the same bit for bit.
+constexpr int BST_SPB = 8;
+struct BstRowsK {
+    const float2* polar; // [S][A2][n]
+    float2* out;         // [S][s][N]
+    int n, A2, s, S, N, L;
+};
+
+template <int THREADS, int E>
+__global__ __launch_bounds__(THREADS) void k_bst_rows(BstRowsK a, const float2* twg, FftK p)
+{
+    extern __shared__ float2 z[];
+    const int n = a.n, s = a.s;
+    float2* tws = z + lds_elems(n);
+    const int ntw = 128 + n / 128 + 1;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = twg[k];
+    const Tw tw{tws, tws + 128};
+    const int tyd = blockIdx.x;
+    const int sl0 = blockIdx.y * BST_SPB, sl1 = min(a.S, sl0 + BST_SPB);
+    const int ty = (tyd + s - s / 2) % s; // (ty + s/2) % s == tyd
+    // the cells txd = threadIdx.x + j THREADS of this row (THREADS E >= s)
+    int o0[E], o1[E]; // polar offsets of (iarc, irho) and (uarc, irho); -1: zero cell (or past s)
+    float pf[E], tf[E], sg[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int txd = threadIdx.x + j * THREADS;
+        o0[j] = -1;
+        o1[j] = 0;
+        pf[j] = tf[j] = 0.0f;
+        sg[j] = 1.0f;
+        if (txd < s) {
+            const int tx = (txd + s - s / 2) % s;
+            const int posx = tx - s / 2, posy = ty - s / 2;
+            const double rho = (double)n * hypot((double)posx, (double)posy) / (double)s;
+            const double angle = (double)a.A2 * (0.5 * atan2((double)posy, (double)posx) / M_PI + 0.5);
+            const long long irho = (long long)rho;
+            int iarc = (int)angle;
+            sg[j] = ((tx & 1) == (ty & 1)) ? 1.0f : -1.0f;
+            if (irho < n / 2 - 1) {
+                pf[j] = (float)(rho - (double)irho);
+                tf[j] = (float)((double)iarc - angle);
+                iarc = iarc % a.A2;
+                const int uarc = (iarc + 1) % a.A2;
+                o0[j] = iarc * n + (int)irho;
+                o1[j] = uarc * n + (int)irho;
+            }
+        }
+    }
+    for (int sl = sl0; sl < sl1; ++sl) {
+        const float2* P = a.polar + (size_t)sl * a.A2 * n;
+        __syncthreads(); // the previous slice's row is read (and the twiddles are in)
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int txd = threadIdx.x + j * THREADS;
+            if (txd < s) {
+                float2 val = make_float2(0.0f, 0.0f);
+                if (o0[j] >= 0) {
+                    const float pfrac = pf[j], tfrac = tf[j];
+                    const float2 a0 = P[o0[j]], a1 = P[o0[j] + 1];
+                    const float2 b0 = P[o1[j]], b1 = P[o1[j] + 1];
+                    const float2 i0 = make_float2(a0.x * (1.0f - pfrac) + a1.x * pfrac,
+                                                  a0.y * (1.0f - pfrac) + a1.y * pfrac);
+                    const float2 i1 = make_float2(b0.x * (1.0f - pfrac) + b1.x * pfrac,
+                                                  b0.y * (1.0f - pfrac) + b1.y * pfrac);
+                    val = make_float2(i0.x * tfrac + i1.x * (1.0f - tfrac), i0.y * tfrac + i1.y * (1.0f - tfrac));
+                }
+                const float2 v = make_float2(val.x * sg[j], val.y * sg[j]);
+                z[pix(txd)] = make_float2(v.x, -v.y);
+            }
+        }
+        __syncthreads();
+        fft_run<THREADS, E>(z, tw, p);
+        for (int c = threadIdx.x; c < a.N; c += THREADS) {
+            const float2 v = z[pix(a.L + c)];
+            a.out[((size_t)sl * s + tyd) * a.N + c] = make_float2(v.x, -v.y);
+        }
+    }
+}
+
 // out[sl][c][y] = in[sl][y][c0 + c] for the nc columns [c0, c0 + nc) of an
-// s x s slice: the columns the crop keeps, as rows for the second 1-D pass
-__global__ __launch_bounds__(256) void k_transpose_c(const float2* in, float2* out, int s, int c0, int nc)
+// s-row slice of row pitch ip: the columns the crop keeps, as rows for the
+// second 1-D pass (32 x 32 LDS tiles)
+__global__ __launch_bounds__(256) void k_transpose_c(const float2* in, float2* out, int s, int c0, int nc, int ip)
 {
     __shared__ float2 t[32][33];
     const int sl = blockIdx.z;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    const float2* src = in + (size_t)sl * s * s;
+    const float2* src = in + (size_t)sl * s * ip;
     float2* dst = out + (size_t)sl * nc * s;
     for (int k = threadIdx.x; k < 32 * 32; k += 256) {
         const int dy = k / 32, dx = k % 32;
-        if (y0 + dy < s && x0 + dx < nc) t[dy][dx] = src[(size_t)(y0 + dy) * s + c0 + x0 + dx];
+        if (y0 + dy < s && x0 + dx < nc) t[dy][dx] = src[(size_t)(y0 + dy) * ip + c0 + x0 + dx];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < 32 * 32; k += 256) {
@@ -596,6 +683,15 @@ void launch_bst_polar_t(const BstPolarK& k, const FftK& p, size_t lds, hipStream
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL((k_bst_polar<THREADS, E>), dim3((unsigned)((long long)k.S * 2 * k.A)), dim3(THREADS), lds, st,
                        k, p);
+}
+
+template <int THREADS, int E>
+void launch_bst_rows_t(const BstRowsK& k, const float2* tw, const FftK& p, size_t lds, hipStream_t st)
+{
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_rows<THREADS, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_bst_rows<THREADS, E>), dim3((unsigned)k.s, (unsigned)((k.S + BST_SPB - 1) / BST_SPB)),
+                       dim3(THREADS), lds, st, k, tw, p);
 }
 
 template <int THREADS, int E>
@@ -729,13 +825,37 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
         }
     }
     RAFT_HIP(hipGetLastError());
-    const long long cells = (long long)a.S * s * s;
-    hipLaunchKernelGGL(k_bst_p2c, dim3((unsigned)std::min<long long>((cells + 255) / 256, 65536)), dim3(256), 0, st,
-                       a.polar, a.cart, n, 2 * a.A, s, a.S);
-    RAFT_HIP(hipGetLastError());
-    // 2-D inverse FFT: all rows, then only the N columns [L, L + N) the crop
-    // keeps (transposed to rows); the other columns are never needed
-    for (int pass = 0; pass < 2; ++pass) {
+    if (a.fused) {
+        // polar -> Cartesian rows, inverse-transformed in LDS, N kept columns
+        // per row -> cart [S][s][N]; then transposed to cartT [S][N][s]
+        const BstRowsK rk{a.polar, a.cart, n, 2 * a.A, s, a.S, a.N, a.L};
+        if (n > 8192) launch_bst_rows_t<1024, 16>(rk, a.tw, p, lds, st);
+        else {
+            switch (plan.threads) {
+            case 64: launch_bst_rows_t<64, 8>(rk, a.tw, p, lds, st); break;
+            case 128: launch_bst_rows_t<128, 8>(rk, a.tw, p, lds, st); break;
+            case 192: launch_bst_rows_t<192, 8>(rk, a.tw, p, lds, st); break;
+            case 256: launch_bst_rows_t<256, 8>(rk, a.tw, p, lds, st); break;
+            case 384: launch_bst_rows_t<384, 8>(rk, a.tw, p, lds, st); break;
+            case 512: launch_bst_rows_t<512, 8>(rk, a.tw, p, lds, st); break;
+            case 768: launch_bst_rows_t<768, 8>(rk, a.tw, p, lds, st); break;
+            default: launch_bst_rows_t<1024, 8>(rk, a.tw, p, lds, st); break;
+            }
+        }
+        RAFT_HIP(hipGetLastError());
+        dim3 g((a.N + 31) / 32, (s + 31) / 32, a.S);
+        hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s, 0, a.N, a.N);
+        RAFT_HIP(hipGetLastError());
+    } else {
+        const long long cells = (long long)a.S * s * s;
+        hipLaunchKernelGGL(k_bst_p2c, dim3((unsigned)std::min<long long>((cells + 255) / 256, 65536)), dim3(256), 0,
+                           st, a.polar, a.cart, n, 2 * a.A, s, a.S);
+        RAFT_HIP(hipGetLastError());
+    }
+    // 2-D inverse FFT: all rows (unfused form), then only the N columns
+    // [L, L + N) the crop keeps (transposed to rows); the other columns are
+    // never needed
+    for (int pass = a.fused ? 1 : 0; pass < 2; ++pass) {
         float2* d = pass == 0 ? a.cart : a.cartT;
         const long long rows = (long long)a.S * (pass == 0 ? s : a.N);
         if (n > 8192) launch_ifft_t<1024, 16>(d, a.tw, p, rows, lds, st);
@@ -754,7 +874,7 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
         RAFT_HIP(hipGetLastError());
         if (pass == 0) {
             dim3 g((a.N + 31) / 32, (s + 31) / 32, a.S);
-            hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s, a.L, a.N);
+            hipLaunchKernelGGL(k_transpose_c, g, dim3(256), 0, st, a.cart, a.cartT, s, a.L, a.N, s);
             RAFT_HIP(hipGetLastError());
         }
     }

@@ -19,10 +19,16 @@
 // samples read staged zeros, the inner loop has no range test.  The gathers
 // of a direct form are limited by the L1 tag rate (a wave's 64 gathers touch
 // ~38 distinct 64-byte lines); staged, a step costs ~17 lines per staged row
-// and the gathers become LDS reads.  Rows are staged at an odd pitch
-// (= 1 mod 4 entries of 16 bytes; fewer bank conflicts measured than with
-// even pitches), and near-vertical chunks stage from the transposed image,
-// so that neighbouring rays always step mostly along a staged row.
+// and the gathers become LDS reads.  Rows are staged at a pitch that is a
+// multiple of 16 entries of 16 bytes whenever the box still fits the tile
+// (else at the narrowest pitch of 1 mod 4, which the host admission assumes):
+// the 16 lanes a 16-byte LDS read serves together hold 16 neighbouring rays
+// at one sample, whose entries then fall on distinct bank groups unless two
+// share a column in different rows (a pitch of 1 mod 4 also makes lanes ~11
+// apart collide at slopes near 1: about twice the conflict cycles in a bank
+// simulation of this pattern).  Near-vertical chunks stage from the
+// transposed image, so that neighbouring rays always step mostly along a
+// staged row.
 // The host admits a launch to this form only when every chunk's union box
 // fits PL_CAP entries at every patch position (proj_plan); sparse angle sets
 // use the direct form.
@@ -368,7 +374,8 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
     int4* s_wbox = reinterpret_cast<int4*>(tile);
     static_assert(PROJ_ANGLES * PL_BATCH * sizeof(int4) <= sizeof(float4) * PL_CAP, "box scratch exceeds the tile");
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // uniform: scalar row loops
     const int a0 = ab * PROJ_ANGLES;
     const int ang = a0 + wave;
     const bool have = ang < a.nangles;
@@ -377,7 +384,12 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
     const int amid = min(a0 + PROJ_ANGLES / 2, a.nangles - 1);
     const bool steep = fabs(a.sinv[amid]) > fabs(a.cosv[amid]) && a.imgT != nullptr;
     const int nRow = steep ? a.Nx : a.Ny, nCol = steep ? a.Ny : a.Nx;
-    const float4* __restrict__ src = (steep ? a.imgT : a.img) + (size_t)g * ((size_t)a.Nx * a.Ny);
+    const size_t plane = (size_t)a.Nx * a.Ny;
+    const float4* __restrict__ src = (steep ? a.imgT : a.img) + (size_t)g * plane;
+    // the plane through a buffer resource (the host admits planes below 2^31
+    // bytes): byte offsets at or past its end read as zero
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(src), 0, (int)(plane * sizeof(float4)), 0x00020000);
 
     const double cs = have ? a.cosv[ang] : 1.0, sn = have ? a.sinv[ang] : 0.0;
     const double bxk = -sn * a.ry, byi = sn * a.rx;
@@ -461,32 +473,45 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
             }
             for (int st = 0; st < nst; ++st) {
                 const int ks = kb + st * PL_K;
-                const int4 u = s_ubox[st];
+                // the union box is the same for every lane: scalar registers
+                const int ux = __builtin_amdgcn_readfirstlane(s_ubox[st].x);
+                const int uy = __builtin_amdgcn_readfirstlane(s_ubox[st].y);
+                const int uz = __builtin_amdgcn_readfirstlane(s_ubox[st].z);
+                const int uw = __builtin_amdgcn_readfirstlane(s_ubox[st].w);
+                const int4 u = make_int4(ux, uy, uz, uw);
                 int nr = u.y - u.x;
                 const int w = u.w - u.z;
                 if (nr <= 0) continue; // no wave has samples here (uniform)
-                const int pitch = w + ((1 - w) & 3);
+                // a multiple of 16 entries when that fits the tile, else 1 mod 4
+                const int pitch16 = (w + 15) & ~15;
+                const int pitch = nr * pitch16 <= PL_CAP ? pitch16 : w + ((1 - w) & 3);
                 // the launch was admitted only if every union fits; the clamp
                 // keeps every LDS access in bounds regardless
                 nr = min(nr, PL_CAP / pitch);
                 // stage rows [u.x, u.x + nr) x cols [u.z, u.w): one wave per row,
-                // eight rows' loads in flight before their LDS writes
+                // eight rows' loads in flight before their LDS writes.  Row
+                // R = u.x + rr of lane column C is the buffer offset
+                // base + rr * row16 (mod 2^32): rows outside the image fall
+                // past the buffer's end (negative ones wrap to >= 2^31), and a
+                // lane whose column lies outside starts at 2^31, so both read
+                // zeros with no test in the loop
                 {
                     const int C = u.z + lane;
                     const bool cin = lane < w && (unsigned)C < (unsigned)nCol;
+                    const unsigned row16 = (unsigned)nCol * 16u;
+                    const unsigned base = cin ? (unsigned)(u.x * nCol + C) * 16u : 0x80000000u;
+                    const bool wr = lane < w;
                     for (int rr0 = wave; rr0 < nr; rr0 += 8 * PROJ_ANGLES) {
                         float4 v[8];
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int rr = rr0 + q * PROJ_ANGLES;
-                            const int R = u.x + rr;
-                            v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if (cin && rr < nr && (unsigned)R < (unsigned)nRow) v[q] = src[(size_t)R * nCol + C];
+                            v[q] = rr < nr ? gather(rs, base + (unsigned)rr * row16) : make_float4(0.f, 0.f, 0.f, 0.f);
                         }
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int rr = rr0 + q * PROJ_ANGLES;
-                            if (lane < w && rr < nr) tile[rr * pitch + lane] = v[q];
+                            if (wr && rr < nr) tile[rr * pitch + lane] = v[q];
                         }
                     }
                     // columns past the first 64 (boxes wider than a wave)
@@ -755,7 +780,7 @@ int proj_plan(const RayGeom& g, const double* c, const double* s, int S, bool ha
         const double rows = std::floor(wr + 1e-9) + 5.0, cols = std::floor(wc + 1e-9) + 5.0;
         if (!(rows < 4096.0 && cols < 4096.0)) return 0;
         const int ic = (int)cols;
-        const long long pitch = ic + ((1 - ic) & 3);
+        const long long pitch = ic + ((1 - ic) & 3); // the kernel's narrowest row pitch
         if ((long long)rows * pitch > PL_CAP) return 0;
     }
     // ray chunks: enough workgroups to fill the device several times

@@ -765,16 +765,22 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
     TableSet tb(D);
     const TabRef dH = tb.add(H), dwa = tb.add(ones);
     tb.upload(st);
+    // the fused form (default) never holds the whole n x n grid: the kept N
+    // columns of every row, then their transpose; RAFT_BST_FUSED=0 runs the
+    // gather and the row transforms as separate passes over the grid
+    const char* fenv = std::getenv("RAFT_BST_FUSED");
+    const bool fused = !(fenv && fenv[0] == '0');
     const size_t s2 = (size_t)n * n;
-    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * n * 8 + 2 * s2 * 8;
+    const size_t cart_elems = fused ? (size_t)n * N : s2;
+    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * n * 8 + (cart_elems + (size_t)n * N) * 8;
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
     B = ((B + 3) / 4) * 4;
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
     float* q = static_cast<float*>(D.workspace<0>((size_t)B * A * n * 4));
     float2* polar = static_cast<float2*>(D.workspace<1>((size_t)B * 2 * A * n * 8));
-    float2* cart = static_cast<float2*>(D.workspace<2>((size_t)B * s2 * 8));
-    float2* cartT = static_cast<float2*>(D.workspace<3>((size_t)B * s2 * 8));
+    float2* cart = static_cast<float2*>(D.workspace<2>((size_t)B * cart_elems * 8));
+    float2* cartT = static_cast<float2*>(D.workspace<3>((size_t)B * n * N * 8));
     for (int s0 = 0; s0 < c.S; s0 += B) {
         const int nb = std::min(B, c.S - s0);
         FilterArgs fa;
@@ -793,6 +799,7 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
         ba.tw = F.tw.as<float2>();
         ba.N = N; ba.A = A; ba.S = nb; ba.L = c.L;
         ba.scale = (float)((double)n * c.pixel * 4.0);
+        ba.fused = fused ? 1 : 0;
         launch_bst(ba, F.plan, st);
     }
     RAFT_HIP(hipStreamSynchronize(st));

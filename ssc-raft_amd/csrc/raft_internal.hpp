@@ -136,8 +136,10 @@ struct FilterArgs {
 void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
 
 // BST after the row filter: q = filtered padded rows (interleaved [G][A][n][4],
-// window [0, n)), polar [S][2A][n], cart / cartT [S][n][n] complex work planes,
-// obj [S][N][N]; tw the twiddle tables of n
+// window [0, n)), polar [S][2A][n], obj [S][N][N]; tw the twiddle tables of n.
+// fused: cart [S][n][N] (the kept columns of the inverse-transformed rows) and
+// cartT [S][N][n]; unfused: cart [S][n][n] (the whole frequency grid) and cartT
+// [S][N][n]
 struct BstArgs {
     const float* q;
     float2* polar;
@@ -147,6 +149,7 @@ struct BstArgs {
     const float2* tw;
     int N, A, S, L;
     float scale;        // 4 n pixel
+    int fused = 1;      // gather + row inverse FFT in one kernel (k_bst_rows)
 };
 void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st);
 

@@ -344,7 +344,8 @@ def test_projector_plan_keeps_every_sample_in_the_staged_box(em, nx, ny, nrays, 
     [0, nk)): the patch box (FP64 extremes over the patch corners, margins -1
     and +3) contains each of the 32 x 32 samples, and the union of all 8 boxes
     (a superset of the kernel's union, which skips waves with no samples in
-    the step) fits PL_CAP entries at the kernel's odd pitch."""
+    the step) fits PL_CAP entries at the kernel's narrowest pitch (1 mod 4
+    entries; the kernel widens it to a multiple of 16 only where that fits)."""
     from sscRaft import device as dv
 
     rng = np.random.default_rng(seed)
@@ -403,7 +404,7 @@ def test_projector_plan_keeps_every_sample_in_the_staged_box(em, nx, ny, nrays, 
             assert np.all(inside | ~valid), (a0, ib)
             rows = hi_r.max(axis=0) - lo_r.min(axis=0)
             w = (hi_c.max(axis=0) - lo_c.min(axis=0)).astype(np.int64)
-            pitch = w + ((1 - w) & 3)
+            pitch = w + ((1 - w) & 3)  # the narrowest pitch the kernel stages at
             assert np.all(rows * pitch <= CAP), (a0, ib, int((rows * pitch).max()), CAP)
     assert W >= 1
 

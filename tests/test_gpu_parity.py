@@ -549,6 +549,24 @@ def test_bst(gpu, n, a, pad, filt, offset):
     assert e <= 2e-5
 
 
+@pytest.mark.parametrize("n,a,pad,slices", [(64, 96, 2, 2), (100, 90, 1, 11), (48, 40, 0, 9)])
+def test_bst_fused_rows_match_separate_passes(gpu, monkeypatch, n, a, pad, slices):
+    """The fused polar-to-Cartesian gather + row inverse FFT (one kernel per
+    grid row, RAFT_BST_FUSED default) gives the separate passes' result bit
+    for bit; the slice counts leave the last group of slices of a workgroup
+    partial."""
+    import sscRaft
+
+    y, ang = noisy_sino(n, a, slices, seed=3 * n + a)
+    dic = fbp_dic(ang, pad=pad, filt="ramp", pixel=2.0 / n)
+    dic["method"] = "BST"
+    fused = sscRaft.fbp(y, dic=dict(dic))
+    monkeypatch.setenv("RAFT_BST_FUSED", "0")
+    separate = sscRaft.fbp(y, dic=dict(dic))
+    assert np.isfinite(fused).all()
+    np.testing.assert_array_equal(fused, separate)
+
+
 def test_blocked_calls_match_one_block(gpu):
     """Volumes split into blocks of slices (parami[9] = blocksize, the
     reference's memory-driven blocking) give what one block gives: FBP and
