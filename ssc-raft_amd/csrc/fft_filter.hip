@@ -468,7 +468,9 @@ struct BstPolarK {
 
 // row r of slice sl: r < A -> q[sl][r][n/2 - 1 - idx], r >= A -> q[sl][r - A][n/2 + idx],
 // idx < n/2, weight 4 (2 at idx 0), zeros for idx >= n/2; forward FFT; times
-// sigma_k = 2 / (n max(min(k, n - k), 1/2))
+// sigma_k = 2 / (n max(min(k, n - k), 1/2)).  Both rows of an angle are real,
+// so one complex FFT carries the two: z = left + i right, and with
+// W_k = conj(Z_{(n - k) mod n}), LEFT_k = (Z_k + W_k) / 2, RIGHT_k = (Z_k - W_k) / 2i.
 template <int THREADS, int E>
 __global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
 {
@@ -477,25 +479,29 @@ __global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
     const int ntw = 128 + a.n / 128 + 1;
     for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
     const Tw tw{tws, tws + 128};
-    const int sl = blockIdx.x / (2 * a.A), r = blockIdx.x % (2 * a.A);
-    const int ang = r % a.A, right = r / a.A;
+    const int sl = blockIdx.x / a.A, ang = blockIdx.x % a.A;
     const int center = a.n / 2;
     const float* row = a.q + ((size_t)(sl / 4) * a.A + ang) * a.n * 4 + (sl % 4);
     for (int idx = threadIdx.x; idx < a.n; idx += THREADS) {
-        float v = 0.0f;
+        float vl = 0.0f, vr = 0.0f;
         if (idx < a.n / 2) {
-            const int m = right ? center + idx : center - 1 - idx;
-            v = row[(size_t)m * 4] * (idx == 0 ? 2.0f : 4.0f);
+            const float w = idx == 0 ? 2.0f : 4.0f;
+            vl = row[(size_t)(center - 1 - idx) * 4] * w;
+            vr = row[(size_t)(center + idx) * 4] * w;
         }
-        z[pix(idx)] = make_float2(v, 0.0f);
+        z[pix(idx)] = make_float2(vl, vr);
     }
     __syncthreads();
     fft_run<THREADS, E>(z, tw, p);
-    float2* out = a.polar + ((size_t)sl * 2 * a.A + r) * a.n;
+    float2* outl = a.polar + ((size_t)sl * 2 * a.A + ang) * a.n;
+    float2* outr = outl + (size_t)a.A * a.n;
     for (int k = threadIdx.x; k < a.n; k += THREADS) {
         const float sig = 2.0f / ((float)a.n * fmaxf(fminf((float)k, (float)(a.n - k)), 0.5f));
-        const float2 v = z[pix(k)];
-        out[k] = make_float2(v.x * sig, v.y * sig);
+        const float2 zk = z[pix(k)];
+        const float2 zm = z[pix(k == 0 ? 0 : a.n - k)];
+        const float hs = 0.5f * sig;
+        outl[k] = make_float2((zk.x + zm.x) * hs, (zk.y - zm.y) * hs);
+        outr[k] = make_float2((zk.y + zm.y) * hs, (zm.x - zk.x) * hs);
     }
 }
 
@@ -681,8 +687,8 @@ void launch_bst_polar_t(const BstPolarK& k, const FftK& p, size_t lds, hipStream
 {
     RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_polar<THREADS, E>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_bst_polar<THREADS, E>), dim3((unsigned)((long long)k.S * 2 * k.A)), dim3(THREADS), lds, st,
-                       k, p);
+    hipLaunchKernelGGL((k_bst_polar<THREADS, E>), dim3((unsigned)((long long)k.S * k.A)), dim3(THREADS), lds, st, k,
+                       p);
 }
 
 template <int THREADS, int E>

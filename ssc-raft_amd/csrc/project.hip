@@ -91,6 +91,7 @@ struct ProjK {
     int tlo, W;
     int RC;          // ray chunks per angle chunk (staged form; 1 for the direct form)
     int kl_stride;   // KL partials per slice = nangles * RC
+    int xcd_order;   // staged form: give each XCD a contiguous range of the grid (see k_project_lds)
 };
 
 __device__ __forceinline__ bool in_image(double kr, double bxk, double uxi, double cs, double uyi, int Nx, int Ny)
@@ -574,11 +575,27 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
     if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc);
 }
 
+// The hardware deals workgroups to the 8 XCDs round-robin by linear block
+// id, so in grid order XCD x would run every 8th angle chunk.  With
+// xcd_order the linear id l is renumbered (l mod 8) * (total / 8) + l / 8, so
+// that XCD x runs the contiguous range [x total / 8, (x + 1) total / 8) of
+// (angle chunk, ray chunk, slice group) items: consecutive angle chunks of the
+// same rays, whose staged boxes overlap, share that XCD's L2.  Only the order
+// changes; every item computes the same values.
 template <int EM>
 __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
 {
     __shared__ ProjSmem sm;
-    proj_lds_item<EM>(a, blockIdx.x, blockIdx.y, blockIdx.z, sm);
+    unsigned ab = blockIdx.x, rc = blockIdx.y, g = blockIdx.z;
+    const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+    if (a.xcd_order && (total & 7u) == 0) {
+        const unsigned l = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+        const unsigned m = (l & 7u) * (total >> 3) + (l >> 3);
+        ab = m % nx;
+        rc = (m / nx) % ny;
+        g = m / (nx * ny);
+    }
+    proj_lds_item<EM>(a, (int)ab, (int)rc, (int)g, sm);
 }
 
 // Persistent work-queue form of the staged projector over several volumes:
@@ -663,6 +680,10 @@ static ProjK proj_kargs(const ProjArgs& p)
     k.Nx = p.g.Nx; k.Ny = p.g.Ny; k.nrays = p.g.nrays; k.nangles = p.g.nangles; k.nk = p.g.nk; k.S = p.S;
     k.ci = p.g.ci; k.ck = p.g.ck; k.ucx = p.g.ucx; k.ucy = p.g.ucy; k.ry = p.g.ry; k.rx = p.g.rx;
     k.scale = p.g.scale;
+    {
+        const char* e = std::getenv("RAFT_PROJ_XCD_ORDER");
+        k.xcd_order = (e && e[0] == '0') ? 0 : 1;
+    }
     k.tlo = p.tlo; k.W = p.W;
     k.RC = p.lds_rc > 0 ? p.lds_rc : 1;
     k.kl_stride = p.g.nangles * k.RC;
