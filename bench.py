@@ -11,8 +11,11 @@ Also reported on the same JSON line:
                 2 FLOP per gathered sample over its average launch time (HIP
                 events on the launch stream) against the 157.3 TFLOP/s FP32
                 vector peak; beside it the effective gather bandwidth, the
-                measured HBM traffic and its fraction of 8 TB/s, and the
-                counter VALU issue rate (profiles/);
+                measured HBM traffic and its fraction of 8 TB/s, the
+                counter VALU issue rate (profiles/), and `lds`: the gathers'
+                LDS read bytes (4 per sample) over the launch time against
+                the device's conflict-free LDS read rate measured at start-up
+                by raft_lds_read_probe (the binding limit of the FP32 rows);
   cpu_baseline  the FP32 restatement in oracle/ timed on this host's cores on
                 a bounded sample (whole slices of the same workload);
   fp16_rows     the same FBP with FP16-stored filtered rows;
@@ -90,9 +93,10 @@ def read_traffic(name):
 
 
 VALU_PEAK = 157.3e12  # FLOP/s, MI355X FP32 vector
+LDS_RATE = None  # measured LDS read rate of the device (raft_lds_read_probe), set in main()
 
 
-def valu_roofline(kernel, samples, launch_s, bytes_gathered, traffic=None, valu_issue=None):
+def valu_roofline(kernel, samples, launch_s, bytes_gathered, traffic=None, valu_issue=None, lds_bytes_per_sample=4.0):
     """Roofline block of a gather-accumulate kernel (backprojection, projector).
 
     These kernels do O(N^3) gathers on O(N^2) data served from LDS, so the
@@ -107,6 +111,15 @@ def valu_roofline(kernel, samples, launch_s, bytes_gathered, traffic=None, valu_
              traffic=traffic, kernel=kernel, launch_ms=launch_s * 1e3, samples_per_launch=samples,
              samples_per_s=samples / launch_s, effective_gather_GBps=bytes_gathered / launch_s / 1e9,
              hbm_frac=(traffic / launch_s / HBM_PEAK) if traffic else None, valu_issue_per_clk_per_cu=valu_issue)
+    if LDS_RATE:
+        # the gathers' LDS reads (16-byte entries of 4 FP32 slices: 4 bytes per
+        # sample) against the device's measured conflict-free LDS read rate
+        lds = samples * lds_bytes_per_sample
+        r["lds"] = dict(read_bytes_per_launch=lds, achieved_TBps=lds / launch_s / 1e12,
+                        peak_TBps=LDS_RATE["bytes_per_s"] / 1e12, frac=lds / launch_s / LDS_RATE["bytes_per_s"],
+                        peak_note="raft_lds_read_probe: conflict-free 16-byte LDS reads on every CU, "
+                                  f"{LDS_RATE['bytes_per_clk_per_cu']:.1f} B/clk/CU at the "
+                                  f"{LDS_RATE['clock_khz'] / 1e6:.2f} GHz clock attribute")
     return r
 
 
@@ -630,6 +643,10 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    global LDS_RATE
+    LDS_RATE = dv.lds_read_probe(0)
+    log(f"LDS read probe: {LDS_RATE['bytes_per_s'] / 1e12:.1f} TB/s "
+        f"({LDS_RATE['bytes_per_clk_per_cu']:.1f} B/clk/CU)")
     if args.only == "batch":
         print(json.dumps({"metric": METRIC, "batch": batch_bench(args, torch, dv, ph)}), flush=True)
         return

@@ -253,6 +253,14 @@ void raft_autotune_clear(void);
 void raft_set_profiling(int on);
 double raft_last_kernel_ms(int which);
 
+/* LDS read rate of the device: conflict-free 16-byte LDS reads (the access of
+ * the backprojector's and staged projector's gathers) by every CU, timed with
+ * HIP events.  *bytes_per_s = LDS bytes delivered per second (whole device),
+ * *cus = compute units, *clock_khz = the device's peak clock attribute.  For
+ * the roofline of the gather kernels, whose bound is LDS read bandwidth.  No
+ * counterpart in the reference.  Synchronous. */
+int raft_lds_read_probe(int device, double* bytes_per_s, int* cus, int* clock_khz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -426,7 +426,59 @@ __global__ __launch_bounds__(256) void k_layout(const float* in, float* out, int
     }
 }
 
+// LDS read-rate probe: every lane reads ITERS conflict-free 16-byte entries
+// (a wave reads 64 consecutive entries), two integer adds per read keep the
+// loads live; every index is masked to the 4096-entry array
+constexpr int LDSP_ITERS = 4096;
+constexpr int LDSP_TILE = 4096;
+__global__ __launch_bounds__(256) void k_lds_probe(unsigned* out)
+{
+    __shared__ uint4 t[LDSP_TILE];
+    for (int i = threadIdx.x; i < LDSP_TILE; i += 256) t[i] = make_uint4(i, i + 1, i + 2, i + 3);
+    __syncthreads();
+    unsigned acc = 0, acc2 = 0;
+    unsigned idx = threadIdx.x;
+#pragma unroll 8
+    for (int it = 0; it < LDSP_ITERS; ++it) {
+        idx = (idx + 64) & (LDSP_TILE - 1);
+        const uint4 v = t[idx];
+        acc = acc + v.x + v.y;
+        acc2 = acc2 + v.z + v.w;
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc ^ acc2;
+}
+
 } // namespace
+
+double lds_read_probe(int* cus_out, int* clock_khz)
+{
+    int dev = 0, cus = 0, clk = 0;
+    RAFT_HIP(hipGetDevice(&dev));
+    RAFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RAFT_HIP(hipDeviceGetAttribute(&clk, hipDeviceAttributeClockRate, dev));
+    const int blocks = cus * 8;
+    DevBuf out((size_t)blocks * 256 * sizeof(unsigned));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    RAFT_HIP(hipEventCreate(&e0));
+    RAFT_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    for (int rep = 0; rep < 3; ++rep) {
+        RAFT_HIP(hipEventRecord(e0, nullptr));
+        hipLaunchKernelGGL(k_lds_probe, dim3(blocks), dim3(256), 0, nullptr, out.as<unsigned>());
+        RAFT_HIP(hipGetLastError());
+        RAFT_HIP(hipEventRecord(e1, nullptr));
+        RAFT_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        RAFT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0 && (best == 0.0f || ms < best)) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (cus_out) *cus_out = cus;
+    if (clock_khz) *clock_khz = clk;
+    const double bytes = (double)blocks * 256 * LDSP_ITERS * 16;
+    return bytes / ((double)best * 1e-3);
+}
 
 // the staged rows a launch over b.S slices reads must lie in the buffer behind q
 static void bp_check_rows(const BpArgs& b, int store)

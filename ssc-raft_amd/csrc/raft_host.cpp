@@ -1573,7 +1573,10 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
     };
     const int sb_fbp = (store == RAFT_STORE_F16) ? BP_SLICES_F16 : BP_SLICES_F32;
     const int grid_fbp = use_queue && do_fbp ? bp_queue_grid(store, 0, 32) : 0;
-    const int grid_embp = use_queue && do_em ? bp_queue_grid(RAFT_STORE_F32, 1, 32) : 0;
+    // the eEM update takes the pair form (two slice groups per item: one FP64
+    // bin index per 8 slices; a bucket's scans fill many groups)
+    const int embp_variant = BP_PAIR | 32;
+    const int grid_embp = use_queue && do_em ? bp_queue_grid(RAFT_STORE_F32, 1, embp_variant) : 0;
     const int grid_proj = use_queue && do_em ? proj_queue_grid(1) : 0;
     // the stages after the per-bucket set-up, on the capture stream
     auto enqueue_queued = [&](hipStream_t s, hipEvent_t evf) {
@@ -1598,8 +1601,9 @@ static void batch_device(int dev, int nscans, const int* nrays, const int* nangl
                     launch_project_queue(qbuf.as<char>() + L.desc_off, proj_queue_of(L), 1, grid_proj, s);
                 for (BatchBucket* b : em_direct) RAFT_HIP(hipStreamWaitEvent(s, b->done, 0));
                 for (const QLaunch& L : q_embp)
-                    launch_backproject_queue(qbuf.as<char>() + L.desc_off, bp_queue_of(L, 4), RAFT_STORE_F32, 1, 32,
-                                             grid_embp, s);
+                    launch_backproject_queue(qbuf.as<char>() + L.desc_off,
+                                             bp_queue_of(L, bp_slices_per_item(RAFT_STORE_F32, 1, embp_variant)),
+                                             RAFT_STORE_F32, 1, embp_variant, grid_embp, s);
             }
             for (BatchBucket* b : buckets) enqueue_em_finish(b, s);
         }
@@ -2268,6 +2272,15 @@ int raft_proj_plan_info(const float* angles, int nangles, int Nx, int Ny, int nr
                              : ray_geom_radon(Nx, Ny, nrays, nangles, (double)ax, (double)ay);
         if (ray_chunks) *ray_chunks = proj_plan(g, cv.data(), sv.data(), nslices, true);
         if (tile_consts) proj_tile_consts(tile_consts);
+    });
+}
+
+int raft_lds_read_probe(int device, double* bytes_per_s, int* cus, int* clock_khz)
+{
+    return guarded([&] {
+        bind_device(device);
+        const double r = lds_read_probe(cus, clock_khz);
+        if (bytes_per_s) *bytes_per_s = r;
     });
 }
 

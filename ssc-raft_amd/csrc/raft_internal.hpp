@@ -173,6 +173,9 @@ void launch_backproject(const BpArgs& a, int store, hipStream_t st);
 bool bp_variant_supported(int variant, int store);
 // slices one workgroup (one queue item) of this variant covers
 int bp_slices_per_item(int store, int mode, int variant);
+// bytes per second of conflict-free 16-byte LDS reads by every CU of the
+// current device (raft_lds_read_probe)
+double lds_read_probe(int* cus, int* clock_khz);
 // tiles per slice group of an n x n image
 int bp_tiles(int n);
 

@@ -5,7 +5,7 @@ import numpy
 
 from .rafttypes import CNICE, libraft, raft_check, ctypes
 
-__all__ = ["fbp_device", "radon_device", "eem_device", "eem_kl_device", "batch_device", "lowpass_device",
+__all__ = ["fbp_device", "lds_read_probe", "radon_device", "eem_device", "eem_kl_device", "batch_device", "lowpass_device",
            "bst_device", "tem_device", "background_device", "set_profiling", "last_kernel_ms", "fbp_plan_info", "proj_plan_info", "autotune_choices",
            "autotune_clear", "eem_device_sched",
            "KF_FILTER", "KF_BP", "KF_PROJ", "KF_EMBP"]
@@ -55,6 +55,19 @@ def fbp_device(tomo, angles, paramf, parami, out=None, store="f32", stream=None,
     if rc:
         raft_check("raft_fbp_device_noalloc" if noalloc else "raft_fbp_device")
     return out
+
+
+def lds_read_probe(device=0):
+    """LDS read rate of the device (raft_lds_read_probe): conflict-free
+    16-byte reads by every CU.  Returns dict(bytes_per_s, cus, clock_khz,
+    bytes_per_clk_per_cu)."""
+    r = ctypes.c_double()
+    cus = ctypes.c_int()
+    clk = ctypes.c_int()
+    if libraft.raft_lds_read_probe(int(device), ctypes.byref(r), ctypes.byref(cus), ctypes.byref(clk)):
+        raft_check("raft_lds_read_probe")
+    per = r.value / max(cus.value, 1) / (max(clk.value, 1) * 1e3)
+    return dict(bytes_per_s=r.value, cus=cus.value, clock_khz=clk.value, bytes_per_clk_per_cu=per)
 
 
 def radon_device(img, angles, nrays=None, ax=1.0, ay=1.0, out=None, stream=None):

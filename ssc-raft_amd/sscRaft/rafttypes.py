@@ -55,6 +55,8 @@ libraft.raft_fbp_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _i, _vp]
 libraft.raft_fbp_device.restype = _i
 libraft.raft_fbp_device_noalloc.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _i, _vp]
 libraft.raft_fbp_device_noalloc.restype = _i
+libraft.raft_lds_read_probe.argtypes = [_i, _vp, _vp, _vp]
+libraft.raft_lds_read_probe.restype = _i
 libraft.raft_radon_device.argtypes = [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _f, _f, _vp]
 libraft.raft_radon_device.restype = _i
 libraft.raft_eem_device.argtypes = [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]

@@ -567,6 +567,43 @@ def test_bst_fused_rows_match_separate_passes(gpu, monkeypatch, n, a, pad, slice
     np.testing.assert_array_equal(fused, separate)
 
 
+def test_slab_split_over_repeated_device_ids(gpu):
+    """The host-pointer entry points give each entry of gpus[] one contiguous
+    slab of slices, each run by its own host thread bound to that device
+    (fbp.cu:267-300, em_rt.cu:221-259, radon_rt.cu:259-294).  With the one
+    device listed several times, the threads share it.  eEM (+ KL) and Radon
+    equal the one-slab results bit for bit (slices are independent).  The
+    FBP filter transforms the rows of two slices in one complex FFT, so FBP
+    is bit-identical when the slabs keep those pairs (even slab sizes) and
+    within FP32 rounding (rel-L2 1e-6) when a slab boundary splits one.  A
+    device number past the device count fails the whole call before any
+    thread starts."""
+    import sscRaft
+
+    n, a, S = 48, 40, 7  # three slabs: 3, 3 and 1 slices
+    y, ang = noisy_sino(n, a, S + 1, seed=31)
+    d = dict(fbp_dic(ang, pad=2, pixel=2.0 / n), gpu=[0])
+    one = sscRaft.fbp(y, dic=dict(d))
+    two = sscRaft.fbp(y, dic=dict(d, gpu=[0, 0]))  # slabs of 4 and 4
+    np.testing.assert_array_equal(two, one)
+    y = y[:S]
+    one = one[:S]
+    three = sscRaft.fbp(y, dic=dict(d, gpu=[0, 0, 0]))
+    assert rel(three, one) < 1e-6
+    yem = np.maximum(y, 0).astype(np.float32)
+    x1, kl1, ex1 = sscRaft.eEMRT_GPU_KL(yem, ang, 4, [0], 0)
+    x3, kl3, ex3 = sscRaft.eEMRT_GPU_KL(yem, ang, 4, [0, 0, 0], 0)
+    np.testing.assert_array_equal(x3, x1)
+    np.testing.assert_array_equal(kl3, kl1)
+    np.testing.assert_array_equal(ex3, ex1)
+    img = ph.phantoms(n, S, config=9, first=40)
+    p1 = sscRaft.radon_RT(img, ang, [0], pixel=2.0 / n)
+    p3 = sscRaft.radon_RT(img, ang, [0, 0, 0], pixel=2.0 / n)
+    np.testing.assert_array_equal(p3, p1)
+    with pytest.raises(RuntimeError, match="invalid device"):
+        sscRaft.fbp(y, dic=dict(fbp_dic(ang, pad=2, pixel=2.0 / n), gpu=[0, 4096]))
+
+
 def test_blocked_calls_match_one_block(gpu):
     """Volumes split into blocks of slices (parami[9] = blocksize, the
     reference's memory-driven blocking) give what one block gives: FBP and
