@@ -29,6 +29,13 @@
 // simulation of this pattern).  Near-vertical chunks stage from the
 // transposed image, so that neighbouring rays always step mostly along a
 // staged row.
+// Steps whose union box is empty (no wave has samples there) are dropped from
+// a batch's step list.  In the prefetch form (the default), the first PL_PF
+// rows per wave of the next step's box are loaded into registers before the
+// current step's gathers and written to LDS after them, so the staging
+// latency overlaps the gathers; rows past 8 PL_PF and columns past 64 are
+// staged after the barrier, as in the plain form.  Both forms stage the same
+// values, so they agree bit for bit.
 // The host admits a launch to this form only when every chunk's union box
 // fits PL_CAP entries at every patch position (proj_plan); sparse angle sets
 // use the direct form.
@@ -74,6 +81,9 @@ constexpr int PL_RAYS = 32;   // rays per patch (staged form)
 constexpr int PL_K = 32;      // samples per ray per step (staged form)
 constexpr int PL_BATCH = 64;  // steps whose boxes are computed together
 constexpr int PL_CAP = 4480;  // staged 16-byte entries per workgroup (70 KB: two workgroups per CU)
+// rows per wave staged through registers ahead of the gathers (prefetch form):
+// 56 rows cover ~96% of the union boxes at 2048 angles of 2048^2
+constexpr int PL_PF = 7;
 
 struct ProjK {
     const float4* img;
@@ -143,17 +153,19 @@ __device__ __forceinline__ float ratio(float yv, float v)
 // Output of one ray for the 4 slices of group g: Radon line sums, eEM ratio
 // row (+ KL terms), or tEM flat * exp(-v) row.  `part` of the `nsplit` lanes
 // that hold the ray's sum (nsplit in {1, 2, 4}): part 0 writes the row; the
-// FP64 KL terms of slices s with s % nsplit == part are computed by that lane,
-// so each (ray, slice) term is added exactly once.
+// FP64 KL terms of slices s = part + j * nsplit are computed by that lane and
+// summed in its slot j, so each (ray, slice) term is added exactly once.
 template <int EM>
 __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, float4 acc, double* klsum,
-                                        unsigned long long* exc, int part = 0, int nsplit = 1)
+                                        unsigned* exc, int part = 0, int nsplit = 1)
 {
     const float v[4] = {acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale};
     if (EM == 1 && a.kl_part) {
-        // lane `part` takes slices part, part + nsplit, ...: the slice is picked
-        // by selects so that all lanes run one FP64 block per round
-        for (int j = 0; j < 4 / nsplit; ++j) {
+        // the slice is picked by selects so that all lanes run one FP64 block
+        // per round; slots are indexed by constants (registers, not scratch)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= 4 / nsplit) break;
             const int s = part + j * nsplit;
             const int sl = g * 4 + s;
             if (sl < a.S) {
@@ -161,11 +173,8 @@ __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, f
                 const double vd = (double)vs, yd = (double)a.y[((size_t)sl * a.nangles + ang) * a.nrays + i];
                 const bool ok = vd > 0.0 && yd >= 0.0;
                 const double t = ok ? vd - yd + (yd > 0.0 ? yd * log(yd / vd) : 0.0) : 0.0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    klsum[q] += q == s ? t : 0.0;
-                    exc[q] += (q == s && !ok) ? 1ull : 0ull;
-                }
+                klsum[j] += t;
+                exc[j] += ok ? 0u : 1u;
             }
         }
     }
@@ -202,14 +211,20 @@ __device__ __forceinline__ void ray_out(const ProjK& a, int g, int ang, int i, f
     }
 }
 
-// deterministic KL partial of one wave (one angle): wave reduction in fixed order
+// deterministic KL partial of one wave (one angle): wave reduction in fixed
+// order.  Slice s sits in slot s / nsplit of the lanes whose part is s % nsplit
+// (zero elsewhere)
 __device__ __forceinline__ void kl_flush(const ProjK& a, int g, int slot, int lane, const double* klsum,
-                                         const unsigned long long* exc)
+                                         const unsigned* exc, int part, int nsplit)
 {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        double d = klsum[s];
-        unsigned long long e = exc[s];
+        const bool mine = part < nsplit && s % nsplit == part;
+        const int j = s / nsplit;
+        const double kj = j == 0 ? klsum[0] : (j == 1 ? klsum[1] : (j == 2 ? klsum[2] : klsum[3]));
+        const unsigned ej = j == 0 ? exc[0] : (j == 1 ? exc[1] : (j == 2 ? exc[2] : exc[3]));
+        double d = mine ? kj : 0.0;
+        unsigned long long e = mine ? ej : 0u;
         for (int off = 32; off > 0; off >>= 1) {
             d += __shfl_xor(d, off);
             e += __shfl_xor(e, off);
@@ -255,7 +270,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
     const bool sweep = (bxk != 0.0 && fabs(bxk) < 1.0e-9) || (cs != 0.0 && fabs(cs) < 1.0e-9);
 
     double klsum[4] = {0.0, 0.0, 0.0, 0.0};
-    unsigned long long exc[4] = {0, 0, 0, 0};
+    unsigned exc[4] = {0, 0, 0, 0};
 
     for (int rb = 0; rb < nblocks; ++rb) {
         const int i = (rb << lgR) + r;
@@ -322,7 +337,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
         if (have && kp < nsplit && i < a.nrays) ray_out<EM>(a, g, ang, i, acc, klsum, exc, kp, nsplit);
         __syncthreads(); // keep the waves (angles) of the workgroup on the same ray block
     }
-    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang, lane, klsum, exc);
+    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang, lane, klsum, exc, kp, min(Kp, 4));
 }
 
 // ---- staged form ----
@@ -354,10 +369,124 @@ __device__ __forceinline__ unsigned tile_off(int R, int C, int pitch16, int moff
     return e;
 }
 
+// Wait for every vector-memory load of this wave (s_waitcnt vmcnt(0); expcnt
+// and lgkmcnt left free: 0xF70 in the gfx9 encoding).  The staging writes sit
+// under lane masks, so on some paths the compiler sees a load whose wait was
+// skipped and then waits conservatively (vmcnt(0)) at the next reuse of a
+// register -- which would fall inside the gathers.  One explicit wait where
+// the loads are complete anyway keeps every later count exact.
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// Union box of one step in staged (row, col) coordinates: rows
+// [ux, ux + nr) x cols [uz, uz + w), staged at row pitch `pitch` entries --
+// a multiple of 16 when that fits the tile, else the narrowest pitch of
+// 1 mod 4.  Every field is the same for every lane (scalar registers).
+struct StepBox {
+    int ux, uz, nr, w, pitch;
+};
+
+__device__ __forceinline__ StepBox step_box(const int4& ub)
+{
+    StepBox b;
+    b.ux = __builtin_amdgcn_readfirstlane(ub.x);
+    const int uy = __builtin_amdgcn_readfirstlane(ub.y);
+    b.uz = __builtin_amdgcn_readfirstlane(ub.z);
+    const int uw = __builtin_amdgcn_readfirstlane(ub.w);
+    b.nr = uy - b.ux;
+    b.w = uw - b.uz;
+    b.pitch = 1;
+    if (b.nr > 0) {
+        const int pitch16 = (b.w + 15) & ~15;
+        b.pitch = b.nr * pitch16 <= PL_CAP ? pitch16 : b.w + ((1 - b.w) & 3);
+        // the launch was admitted only if every union fits; the clamp keeps
+        // every LDS access in bounds regardless
+        b.nr = min(b.nr, PL_CAP / b.pitch);
+    }
+    return b;
+}
+
+// Buffer offset of staged row 0 for this lane's column.  Row R = ux + rr of
+// lane column C is base + rr * nCol * 16 (mod 2^32): rows outside the image
+// fall past the buffer's end (negative ones wrap to >= 2^31), and a lane
+// whose column lies outside starts at 2^31, so both read zeros with no test
+// in the loops.
+__device__ __forceinline__ unsigned stage_base(const StepBox& b, int nCol, int lane)
+{
+    const int C = b.uz + lane;
+    const bool cin = lane < b.w && (unsigned)C < (unsigned)nCol;
+    return cin ? (unsigned)(b.ux * nCol + C) * 16u : 0x80000000u;
+}
+
+// Rows rr = first, first + 8, ... of the box (one wave per row, eight rows'
+// loads in flight before their LDS writes), then the columns past the first
+// 64 of every row (boxes wider than a wave); all of them complete on return.
+__device__ __forceinline__ void stage_rows(float4* tile, __amdgpu_buffer_rsrc_t rs, const StepBox& b, int nCol,
+                                           int lane, int wave, int first)
+{
+    const unsigned row16 = (unsigned)nCol * 16u;
+    const unsigned base = stage_base(b, nCol, lane);
+    const bool wr = lane < b.w;
+    for (int rr0 = first; rr0 < b.nr; rr0 += 8 * PROJ_ANGLES) {
+        float4 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int rr = rr0 + q * PROJ_ANGLES;
+            v[q] = rr < b.nr ? gather(rs, base + (unsigned)rr * row16) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int rr = rr0 + q * PROJ_ANGLES;
+            if (wr && rr < b.nr) tile[rr * b.pitch + lane] = v[q];
+        }
+    }
+    for (int cc0 = 64; cc0 < b.w; cc0 += 64) {
+        // column uz + cc0 + lane of row ux + rr: inside the image, or OFF_NONE
+        const int C = b.uz + cc0 + lane;
+        const bool cin = cc0 + lane < b.w && (unsigned)C < (unsigned)nCol;
+        for (int rr = wave; rr < b.nr; rr += PROJ_ANGLES) {
+            // rows outside the image fall past the buffer's end, as in stage_base
+            const unsigned off = cin ? (unsigned)((b.ux + rr) * nCol + C) * 16u : OFF_NONE;
+            const float4 v = gather(rs, off);
+            if (cc0 + lane < b.w) tile[rr * b.pitch + cc0 + lane] = v;
+        }
+    }
+    wait_loads();
+}
+
+// Prefetch: rows rr = wave + 8 q, q < PF, of the box into registers ...
+template <int PF>
+__device__ __forceinline__ void stage_issue(__amdgpu_buffer_rsrc_t rs, const StepBox& b, int nCol, int lane, int wave,
+                                            float4 (&pv)[PF])
+{
+    const unsigned row16 = (unsigned)nCol * 16u;
+    const unsigned base = stage_base(b, nCol, lane);
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int rr = wave + q * PROJ_ANGLES;
+        pv[q] = rr < b.nr ? gather(rs, base + (unsigned)rr * row16) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// ... and from the registers into the tile (after the barrier that ends the
+// previous step's gathers)
+template <int PF>
+__device__ __forceinline__ void stage_commit(float4* tile, const StepBox& b, int lane, int wave, const float4 (&pv)[PF])
+{
+    const bool wr = lane < b.w;
+    wait_loads();
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int rr = wave + q * PROJ_ANGLES;
+        if (wr && rr < b.nr) tile[rr * b.pitch + lane] = pv[q];
+    }
+}
+
 // LDS of one staged-form workgroup
 struct ProjSmem {
     float4 tile[PL_CAP];
-    int4 ubox[PL_BATCH];
+    int4 ubox[PL_BATCH]; // union boxes of the batch's steps with samples
+    int kst[PL_BATCH];   // their first sample index
+    int nst;             // their count
     int kr[PROJ_ANGLES][2];
 };
 
@@ -365,11 +494,13 @@ struct ProjSmem {
 // [8 ab, 8 ab + 8)), ray chunk rc (ray blocks rc, rc + RC, ...), slice group
 // g.  Every LDS write is preceded by a barrier, so a persistent workgroup can
 // run items back to back.
-template <int EM>
+template <int EM, int PF>
 __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, int g, ProjSmem& sm)
 {
     float4* tile = sm.tile;
     int4* s_ubox = sm.ubox;
+    int* s_kst = sm.kst;
+    int* s_nst = &sm.nst;
     int(*s_kr)[2] = sm.kr;
     // per-wave boxes of a batch of steps, computed while no step is staged
     int4* s_wbox = reinterpret_cast<int4*>(tile);
@@ -403,7 +534,7 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
     const int nrb = (a.nrays + PL_RAYS - 1) / PL_RAYS;
 
     double klsum[4] = {0.0, 0.0, 0.0, 0.0};
-    unsigned long long exc[4] = {0, 0, 0, 0};
+    unsigned exc[4] = {0, 0, 0, 0};
 
     for (int rb = rc; rb < nrb; rb += a.RC) {
         const int ib = rb * PL_RAYS;
@@ -454,79 +585,63 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
                 // the tile (aliased by s_wbox) is free: the last step ended with a barrier
                 s_wbox[wave * PL_BATCH + lane] = bx;
                 __syncthreads();
-                if (threadIdx.x < nst) {
+                if (threadIdx.x < 64) { // wave 0, one lane per step (nst <= 64)
                     int4 u = make_int4(0x7fffffff, -0x7fffffff, 0x7fffffff, -0x7fffffff);
                     bool any = false;
+                    if (threadIdx.x < nst) {
 #pragma unroll
-                    for (int w = 0; w < PROJ_ANGLES; ++w) {
-                        const int4 b = s_wbox[w * PL_BATCH + threadIdx.x];
-                        if (b.y > b.x) {
-                            u.x = min(u.x, b.x);
-                            u.y = max(u.y, b.y);
-                            u.z = min(u.z, b.z);
-                            u.w = max(u.w, b.w);
-                            any = true;
+                        for (int w = 0; w < PROJ_ANGLES; ++w) {
+                            const int4 b = s_wbox[w * PL_BATCH + threadIdx.x];
+                            if (b.y > b.x) {
+                                u.x = min(u.x, b.x);
+                                u.y = max(u.y, b.y);
+                                u.z = min(u.z, b.z);
+                                u.w = max(u.w, b.w);
+                                any = true;
+                            }
                         }
                     }
-                    s_ubox[threadIdx.x] = any ? u : make_int4(0, 0, 0, 0);
+                    // steps where no wave has samples are dropped; the list keeps
+                    // the others in order, with their first sample index
+                    const unsigned long long m = __ballot(any);
+                    const int idx = __popcll(m & ((1ull << threadIdx.x) - 1ull));
+                    if (any) {
+                        s_ubox[idx] = u;
+                        s_kst[idx] = kb + (int)threadIdx.x * PL_K;
+                    }
+                    if (threadIdx.x == 0) *s_nst = __popcll(m);
                 }
                 __syncthreads();
             }
-            for (int st = 0; st < nst; ++st) {
-                const int ks = kb + st * PL_K;
-                // the union box is the same for every lane: scalar registers
-                const int ux = __builtin_amdgcn_readfirstlane(s_ubox[st].x);
-                const int uy = __builtin_amdgcn_readfirstlane(s_ubox[st].y);
-                const int uz = __builtin_amdgcn_readfirstlane(s_ubox[st].z);
-                const int uw = __builtin_amdgcn_readfirstlane(s_ubox[st].w);
-                const int4 u = make_int4(ux, uy, uz, uw);
-                int nr = u.y - u.x;
-                const int w = u.w - u.z;
-                if (nr <= 0) continue; // no wave has samples here (uniform)
-                // a multiple of 16 entries when that fits the tile, else 1 mod 4
-                const int pitch16 = (w + 15) & ~15;
-                const int pitch = nr * pitch16 <= PL_CAP ? pitch16 : w + ((1 - w) & 3);
-                // the launch was admitted only if every union fits; the clamp
-                // keeps every LDS access in bounds regardless
-                nr = min(nr, PL_CAP / pitch);
-                // stage rows [u.x, u.x + nr) x cols [u.z, u.w): one wave per row,
-                // eight rows' loads in flight before their LDS writes.  Row
-                // R = u.x + rr of lane column C is the buffer offset
-                // base + rr * row16 (mod 2^32): rows outside the image fall
-                // past the buffer's end (negative ones wrap to >= 2^31), and a
-                // lane whose column lies outside starts at 2^31, so both read
-                // zeros with no test in the loop
-                {
-                    const int C = u.z + lane;
-                    const bool cin = lane < w && (unsigned)C < (unsigned)nCol;
-                    const unsigned row16 = (unsigned)nCol * 16u;
-                    const unsigned base = cin ? (unsigned)(u.x * nCol + C) * 16u : 0x80000000u;
-                    const bool wr = lane < w;
-                    for (int rr0 = wave; rr0 < nr; rr0 += 8 * PROJ_ANGLES) {
-                        float4 v[8];
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int rr = rr0 + q * PROJ_ANGLES;
-                            v[q] = rr < nr ? gather(rs, base + (unsigned)rr * row16) : make_float4(0.f, 0.f, 0.f, 0.f);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int rr = rr0 + q * PROJ_ANGLES;
-                            if (wr && rr < nr) tile[rr * pitch + lane] = v[q];
-                        }
-                    }
-                    // columns past the first 64 (boxes wider than a wave)
-                    for (int rr = wave; rr < nr && w > 64; rr += PROJ_ANGLES) {
-                        const int R = u.x + rr;
-                        for (int cc = 64 + lane; cc < w; cc += 64) {
-                            const int C2 = u.z + cc;
-                            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                            if ((unsigned)R < (unsigned)nRow && (unsigned)C2 < (unsigned)nCol) v = src[(size_t)R * nCol + C2];
-                            tile[rr * pitch + cc] = v;
-                        }
+            const int nsv = __builtin_amdgcn_readfirstlane(*s_nst); // steps with samples
+            // Steps are staged in order.  With PF > 0, the first PF rows per wave
+            // of step st + 1 are loaded into registers before step st's gathers and
+            // written to LDS after them, so their latency overlaps the gathers; the
+            // rest of a box (rows past 8 PF, columns past 64) is staged as in the
+            // plain form.  The staged values, and so every sum, are the same.
+            // Every listed step has a box, so every prefetch is consumed by a
+            // commit: no load is left in flight when its registers are reused.
+            float4 pv[PF > 0 ? PF : 1];
+            StepBox bnext{};
+            if constexpr (PF > 0) {
+                if (nsv > 0) {
+                    bnext = step_box(s_ubox[0]);
+                    stage_issue<PF>(rs, bnext, nCol, lane, wave, pv);
+                }
+            }
+            for (int st = 0; st < nsv; ++st) {
+                const int ks = __builtin_amdgcn_readfirstlane(s_kst[st]);
+                const StepBox b = PF > 0 ? bnext : step_box(s_ubox[st]);
+                if constexpr (PF > 0) stage_commit<PF>(tile, b, lane, wave, pv);
+                stage_rows(tile, rs, b, nCol, lane, wave, wave + PF * PROJ_ANGLES);
+                __syncthreads();
+                if constexpr (PF > 0) {
+                    if (st + 1 < nsv) {
+                        bnext = step_box(s_ubox[st + 1]);
+                        stage_issue<PF>(rs, bnext, nCol, lane, wave, pv);
                     }
                 }
-                __syncthreads();
+                const int pitch = b.pitch;
                 if (have && ks < wkh && ks + PL_K > wkl) {
                     // byte offset of sample (R, C) in the tile: R * pitch16 - off16 + C * 16
                     // (24-bit multiply-add, then shift-add).  Every sample lies in its wave's
@@ -536,7 +651,7 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
                     // the launch with.  No clamp is needed; the staging clamp of nr keeps the
                     // tile's writes in bounds
                     const int pitch16 = pitch << 4;
-                    const int moff16 = -((u.x * pitch + u.z) << 4);
+                    const int moff16 = -((b.ux * pitch + b.uz) << 4);
                     const char* tb = reinterpret_cast<const char*>(tile);
                     double kr = (double)(ks + kp) - a.ck;
                     float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f); // this step's sum (two-level summation)
@@ -572,7 +687,7 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
         // both phases hold the ray's sum and share its KL terms
         if (ray_ok && (EM == 1 || kp == 0)) ray_out<EM>(a, g, ang, i, acc, klsum, exc, kp, EM == 1 ? 2 : 1);
     }
-    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc);
+    if (EM == 1 && a.kl_part && have) kl_flush(a, g, ang * a.RC + rc, lane, klsum, exc, kp, 2);
 }
 
 // The hardware deals workgroups to the 8 XCDs round-robin by linear block
@@ -582,7 +697,7 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
 // (angle chunk, ray chunk, slice group) items: consecutive angle chunks of the
 // same rays, whose staged boxes overlap, share that XCD's L2.  Only the order
 // changes; every item computes the same values.
-template <int EM>
+template <int EM, int PF>
 __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
 {
     __shared__ ProjSmem sm;
@@ -595,7 +710,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds(ProjK a)
         rc = (m / nx) % ny;
         g = m / (nx * ny);
     }
-    proj_lds_item<EM>(a, (int)ab, (int)rc, (int)g, sm);
+    proj_lds_item<EM, PF>(a, (int)ab, (int)rc, (int)g, sm);
 }
 
 // Persistent work-queue form of the staged projector over several volumes:
@@ -619,7 +734,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds_queue(const
         b = __builtin_amdgcn_readfirstlane(b);
         const int local = item - qu.first[b];
         const int nab = qu.nab[b], RC = descs[b].RC;
-        proj_lds_item<EM>(descs[b], local % nab, (local / nab) % RC, local / (nab * RC), sm);
+        proj_lds_item<EM, 0>(descs[b], local % nab, (local / nab) % RC, local / (nab * RC), sm);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -738,9 +853,19 @@ void launch_project(const ProjArgs& p, int em_mode, hipStream_t st)
     const dim3 block(64 * PROJ_ANGLES);
     if (p.lds_rc > 0) {
         const dim3 grid(nab, k.RC, G);
-        if (em_mode == 2) hipLaunchKernelGGL((k_project_lds<2>), grid, block, 0, st, k);
-        else if (em_mode) hipLaunchKernelGGL((k_project_lds<1>), grid, block, 0, st, k);
-        else hipLaunchKernelGGL((k_project_lds<0>), grid, block, 0, st, k);
+        // the prefetch form by default; RAFT_PROJ_PREFETCH=0 selects the plain
+        // one (same values bit for bit)
+        const char* pfe = std::getenv("RAFT_PROJ_PREFETCH");
+        const bool pf = !(pfe && pfe[0] == '0');
+        if (pf) {
+            if (em_mode == 2) hipLaunchKernelGGL((k_project_lds<2, PL_PF>), grid, block, 0, st, k);
+            else if (em_mode) hipLaunchKernelGGL((k_project_lds<1, PL_PF>), grid, block, 0, st, k);
+            else hipLaunchKernelGGL((k_project_lds<0, PL_PF>), grid, block, 0, st, k);
+        } else {
+            if (em_mode == 2) hipLaunchKernelGGL((k_project_lds<2, 0>), grid, block, 0, st, k);
+            else if (em_mode) hipLaunchKernelGGL((k_project_lds<1, 0>), grid, block, 0, st, k);
+            else hipLaunchKernelGGL((k_project_lds<0, 0>), grid, block, 0, st, k);
+        }
     } else {
         const dim3 grid(nab, G);
         if (em_mode == 2) hipLaunchKernelGGL((k_project<2>), grid, block, 0, st, k);

@@ -175,6 +175,41 @@ def test_radon_projector_forms(gpu, monkeypatch, form, ny, nx, a):
     assert e <= FP32_TOL
 
 
+@pytest.mark.parametrize("ny,nx,a,s", [(64, 64, 64, 5), (200, 200, 200, 4), (31, 31, 40, 3), (256, 256, 240, 4),
+                                       (1024, 1024, 960, 4)])
+def test_staged_projector_prefetch_is_bit_identical(gpu, monkeypatch, ny, nx, a, s):
+    """The staged projector's prefetch form (the default: the next step's
+    first rows loaded into registers during the current step's gathers)
+    stages the same values as the plain form (RAFT_PROJ_PREFETCH=0), so Radon
+    sums, eEM iterates, KL and excluded counts, and tEM iterates agree bit for
+    bit.
+    Most union boxes here are taller than the rows prefetched per step (56 in
+    the Radon form, 40 in the eEM form), so the plain staging of the rest
+    runs too."""
+    import sscRaft
+
+    rng = np.random.default_rng(ny * a + s)
+    img = rng.random((s, ny, nx)).astype(np.float32)
+    ang = ph.uniform_angles(a)
+    monkeypatch.setenv("RAFT_PROJ_PREFETCH", "0")
+    r0 = sscRaft.radon_RT(img, ang, [0], pixel=2.0 / nx)
+    monkeypatch.delenv("RAFT_PROJ_PREFETCH")
+    r1 = sscRaft.radon_RT(img, ang, [0], pixel=2.0 / nx)
+    np.testing.assert_array_equal(r0, r1)
+    if ny != nx or ny > 256:
+        return
+    y = np.maximum(ph.poisson_log(r0 / ny, 1e4, seed=a), 0)
+    monkeypatch.setenv("RAFT_PROJ_PREFETCH", "0")
+    e0 = sscRaft.eEMRT_GPU_KL(y, ang, 4, [0], 0)
+    t0 = sscRaft.tEMRT_GPU_(y * 0.01, np.ones((s, 1, nx), np.float32), ang, 3, [0], 0)
+    monkeypatch.delenv("RAFT_PROJ_PREFETCH")
+    e1 = sscRaft.eEMRT_GPU_KL(y, ang, 4, [0], 0)
+    t1 = sscRaft.tEMRT_GPU_(y * 0.01, np.ones((s, 1, nx), np.float32), ang, 3, [0], 0)
+    for u, v in zip(e0, e1):
+        np.testing.assert_array_equal(u, v)
+    np.testing.assert_array_equal(t0, t1)
+
+
 @pytest.mark.parametrize("n,a,k", [(64, 64, 5), (40, 48, 3)])
 def test_eem_projector_forms_agree(gpu, monkeypatch, n, a, k):
     """eEM with the staged projector (default for these dense sets) and with
