@@ -378,6 +378,59 @@ __global__ __launch_bounds__(THREADS) void k_filter_fft_entry(FilterK a, FftK p)
     }
 }
 
+// Whole-entry form for FP32 rows with the first pair's filtered window kept
+// in LDS (after the twiddle tables) instead of registers: without the held
+// window and with the register budget of WPE waves per SIMD (80 VGPRs at 768
+// threads), two workgroups fit a CU (twice 77 KB of LDS at n = 6144), so one
+// workgroup's barriers overlap the other's passes.  The arithmetic of each
+// pair is that of k_filter_fft, so the stored entries are the same bit for
+// bit.
+template <int THREADS, int E, int WPE>
+__global__ __launch_bounds__(THREADS, WPE) void k_filter_fft_entry_lds(FilterK a, FftK p)
+{
+    extern __shared__ float2 z[];
+    float2* tws = z + lds_elems(a.npad);
+    const int ntw = 128 + a.npad / 128 + 1;
+    float2* win = tws + ntw; // [W]: pair 0's window
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
+    const Tw tw{tws, tws + 128};
+    const int ang = blockIdx.x % a.A, g = blockIdx.x / a.A;
+    const float wv = a.wa[ang];
+    const float sign1 = a.none ? 1.0f : -1.0f;
+    uint4* dst = reinterpret_cast<uint4*>(static_cast<float*>(a.q)) + ((size_t)g * a.A + ang) * a.W;
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr) {
+        const PairIdx ix{g, ang, g * 4 + 2 * pr};
+        __syncthreads(); // the twiddles are in; the previous pair's window reads are done
+        load_pair<THREADS>(a, ix, z);
+        __syncthreads();
+        if (!a.none) {
+            fft_run<THREADS, E>(z, tw, p);
+            for (int k = threadIdx.x; k < a.npad; k += THREADS) {
+                const float2 v = cmul(z[pix(k)], a.H[k]);
+                z[pix(k)] = make_float2(v.x, -v.y);
+            }
+            __syncthreads();
+            fft_run<THREADS, E>(z, tw, p);
+        }
+        for (int w = threadIdx.x; w < a.W; w += THREADS) {
+            const int T = a.tlo + w;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (T >= 0 && T < a.npad) {
+                const float2 zv = z[pix(T)];
+                v = make_float2(zv.x * wv, sign1 * zv.y * wv);
+            }
+            if (pr == 0) {
+                win[w] = v; // read back by the same thread
+            } else {
+                const float2 h = win[w];
+                dst[w] = make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), __float_as_uint(v.x),
+                                    __float_as_uint(v.y));
+            }
+        }
+    }
+}
+
 // direct circular convolution for sizes the LDS FFT does not cover:
 // q[m] = sum_j g[(m - j) mod n] u[j]
 template <typename OT>
@@ -432,12 +485,31 @@ void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipSt
     hipLaunchKernelGGL((k_filter_fft<OT, THREADS, E>), dim3(blocks), dim3(THREADS), lds, st, k, p);
 }
 
+template <int THREADS, int WPE>
+void launch_fft_lds_entry(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
+{
+    const void* fn = reinterpret_cast<const void*>(&k_filter_fft_entry_lds<THREADS, 8, WPE>);
+    RAFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_filter_fft_entry_lds<THREADS, 8, WPE>), dim3(blocks / 2), dim3(THREADS), lds, st, k, p);
+}
+
 // THREADS = the smallest of 64, 128, 192, 256, 384, 512, 768, 1024 that is
 // >= n/8, with E = 8 elements per thread; n in (8192, 16384] runs 1024
 // threads with E = 16
 template <typename OT>
 void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
 {
+    // FP32 rows, n in (2048, 6144]: the two-workgroup form when two of them
+    // fit a CU's 160 KB of LDS (RAFT_FFT_FORM=regs keeps the window in
+    // registers instead: one workgroup per CU at n = 6144)
+    const char* fe = std::getenv("RAFT_FFT_FORM");
+    const size_t lw = lds + sizeof(float2) * (size_t)k.W;
+    if (sizeof(OT) == 4 && p.n > 2048 && p.n <= 6144 && 2 * lw <= 160 * 1024 && !(fe && fe[0] == 'r') &&
+        !std::getenv("RAFT_FILTER_PAIRS")) {
+        if (p.n <= 3072) launch_fft_lds_entry<384, 3>(k, p, blocks, lw, st);
+        else launch_fft_lds_entry<768, 6>(k, p, blocks, lw, st);
+        return;
+    }
     if (p.n > 8192) {
         launch_fft_t<OT, 1024, 16>(k, p, blocks, lds, st);
         return;

@@ -110,6 +110,25 @@ def test_fbp_2048(gpu):
     assert e <= FP32_TOL
 
 
+@pytest.mark.parametrize("n,a,s,filt", [(1024, 48, 8, "ramp"), (768, 40, 6, "hann"), (2048, 16, 4, "ramp"),
+                                        (1024, 20, 5, "none")])
+def test_filter_two_workgroup_form_is_bit_identical(gpu, monkeypatch, n, a, s, filt):
+    """The FP32 filter's default form for padded lengths in (2048, 6144] keeps
+    the first slice pair's window in LDS so that two workgroups fit a CU; it
+    runs each pair's FFTs with the same arithmetic as the form that holds the
+    window in registers (RAFT_FFT_FORM=regs), so FBP agrees bit for bit
+    (padded lengths 3072, 2304 and 6144; 5 slices leave a partial group)."""
+    import sscRaft
+
+    y, ang = noisy_sino(n, a, s, seed=n + a)
+    dic = fbp_dic(ang, pad=2, filt=filt, pixel=2.0 / n)
+    monkeypatch.setenv("RAFT_FFT_FORM", "regs")
+    o0 = sscRaft.fbp(y, dic=dic)
+    monkeypatch.delenv("RAFT_FFT_FORM")
+    o1 = sscRaft.fbp(y, dic=dic)
+    np.testing.assert_array_equal(o0, o1)
+
+
 def test_fbp_fp16_storage(gpu):
     import torch
     from sscRaft import device as dv
