@@ -139,7 +139,8 @@ template <> struct Dft<5> : DftGeneric<5> {};
 // Ns is a power of two for every pass before the first odd radix (radices
 // are ordered 8, 4, 2 first), so jm is a mask there.
 template <int R, int THREADS, int E>
-__device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns, int twstep, unsigned nsm, int nss)
+__device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns, int twstep, unsigned nsm, int nss,
+                                         int tid)
 {
     constexpr int MAXB = (E + R - 1) / R;
     const int nb = n / R;
@@ -153,7 +154,7 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns,
     float2 v[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int j = threadIdx.x + b * THREADS;
+        const int j = tid + b * THREADS;
         if (j < nb) {
             const int jm = jmod(j);
             // W^(r ts) = (W^ts)^r: one table twiddle per butterfly, the rest by
@@ -171,7 +172,7 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns,
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-        const int j = threadIdx.x + b * THREADS;
+        const int j = tid + b * THREADS;
         if (j < nb) {
             Dft<R>::run(v[b], tw, n);
             const int jm = jmod(j);
@@ -183,6 +184,12 @@ __device__ __forceinline__ void fft_pass(float2* z, const Tw& tw, int n, int Ns,
     __syncthreads();
 }
 
+// Waves per SIMD asked of the 768-thread LDS FFT kernels (n in (4096, 6144]):
+// six keep them within 80 VGPRs, so two workgroups share a CU (twice ~54 KB
+// of LDS at n = 6144) and one workgroup's barriers overlap the other's passes.
+// Other sizes fit several workgroups per CU at their natural register count.
+template <int THREADS> constexpr int fft_wpe() { return THREADS == 768 ? 6 : 1; }
+
 struct FftK {
     int n, nstages;
     int radix[24];
@@ -193,7 +200,7 @@ struct FftK {
 };
 
 template <int THREADS, int E>
-__device__ void fft_run(float2* z, const Tw& tw, const FftK& p)
+__device__ void fft_run(float2* z, const Tw& tw, const FftK& p, int tid = threadIdx.x)
 {
     int Ns = 1;
     for (int st = 0; st < p.nstages; ++st) {
@@ -201,11 +208,11 @@ __device__ void fft_run(float2* z, const Tw& tw, const FftK& p)
         const int ts = p.twstep[st], ss = p.nss[st];
         const unsigned sm = p.nsm[st];
         switch (R) {
-        case 8: fft_pass<8, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
-        case 4: fft_pass<4, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
-        case 2: fft_pass<2, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
-        case 3: fft_pass<3, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
-        default: fft_pass<5, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss); break;
+        case 8: fft_pass<8, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss, tid); break;
+        case 4: fft_pass<4, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss, tid); break;
+        case 2: fft_pass<2, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss, tid); break;
+        case 3: fft_pass<3, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss, tid); break;
+        default: fft_pass<5, THREADS, E>(z, tw, p.n, Ns, ts, sm, ss, tid); break;
         }
         Ns *= R;
     }
@@ -285,7 +292,7 @@ __device__ __forceinline__ void store_pair(const FilterK& k, const PairIdx& ix, 
 }
 
 template <typename OT, int THREADS, int E>
-__global__ __launch_bounds__(THREADS) void k_filter_fft(FilterK a, FftK p)
+__global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_filter_fft(FilterK a, FftK p)
 {
     extern __shared__ float2 z[]; // [lds_elems(npad)] padded data, then the twiddle tables
     float2* tws = z + lds_elems(a.npad);
@@ -533,7 +540,7 @@ void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t
 // ---------------------------------------------------------------------------
 struct BstPolarK {
     const float* q; // filtered padded rows, interleaved [G][A][n][4]
-    float2* polar;  // [S][2A][n]
+    float2* polar;  // [S][2A][n/2]
     const float2* tw;
     int n, A, S;
 };
@@ -544,7 +551,7 @@ struct BstPolarK {
 // so one complex FFT carries the two: z = left + i right, and with
 // W_k = conj(Z_{(n - k) mod n}), LEFT_k = (Z_k + W_k) / 2, RIGHT_k = (Z_k - W_k) / 2i.
 template <int THREADS, int E>
-__global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
+__global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_bst_polar(BstPolarK a, FftK p)
 {
     extern __shared__ float2 z[];
     float2* tws = z + lds_elems(a.n);
@@ -565,9 +572,12 @@ __global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
     }
     __syncthreads();
     fft_run<THREADS, E>(z, tw, p);
-    float2* outl = a.polar + ((size_t)sl * 2 * a.A + ang) * a.n;
-    float2* outr = outl + (size_t)a.A * a.n;
-    for (int k = threadIdx.x; k < a.n; k += THREADS) {
+    // only radii [0, n/2) are ever interpolated (irho + 1 < n/2 - 1 + 1): rows
+    // of n/2 entries
+    const int h = a.n / 2;
+    float2* outl = a.polar + ((size_t)sl * 2 * a.A + ang) * h;
+    float2* outr = outl + (size_t)a.A * h;
+    for (int k = threadIdx.x; k < h; k += THREADS) {
         const float sig = 2.0f / ((float)a.n * fmaxf(fminf((float)k, (float)(a.n - k)), 0.5f));
         const float2 zk = z[pix(k)];
         const float2 zm = z[pix(k == 0 ? 0 : a.n - k)];
@@ -579,7 +589,7 @@ __global__ __launch_bounds__(THREADS) void k_bst_polar(BstPolarK a, FftK p)
 
 // unnormalised inverse C2C FFT of `rows` rows of length n, in place
 template <int THREADS, int E>
-__global__ __launch_bounds__(THREADS) void k_ifft_rows(float2* data, const float2* twg, FftK p)
+__global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_ifft_rows(float2* data, const float2* twg, FftK p)
 {
     extern __shared__ float2 z[];
     const int n = p.n;
@@ -600,7 +610,7 @@ __global__ __launch_bounds__(THREADS) void k_ifft_rows(float2* data, const float
     }
 }
 
-// polar (2A rows x n radii) -> Cartesian frequency grid s x s (bst.cu:63-99),
+// polar (2A rows x n/2 radii) -> Cartesian frequency grid s x s (bst.cu:63-99),
 // index arithmetic in FP64; the reference's weights (tfrac, 1 - tfrac) with
 // tfrac = iarc - angle are kept as they are
 __global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int s, int S)
@@ -621,9 +631,10 @@ __global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int 
             const float tfrac = (float)((double)iarc - angle);
             iarc = iarc % A2;
             const int uarc = (iarc + 1) % A2;
-            const float2* P = polar + sl * (long long)A2 * n;
-            const float2 a0 = P[(size_t)iarc * n + irho], a1 = P[(size_t)iarc * n + irho + 1];
-            const float2 b0 = P[(size_t)uarc * n + irho], b1 = P[(size_t)uarc * n + irho + 1];
+            const int h = n / 2; // polar rows hold radii [0, n/2)
+            const float2* P = polar + sl * (long long)A2 * h;
+            const float2 a0 = P[(size_t)iarc * h + irho], a1 = P[(size_t)iarc * h + irho + 1];
+            const float2 b0 = P[(size_t)uarc * h + irho], b1 = P[(size_t)uarc * h + irho + 1];
             const float2 i0 = make_float2(a0.x * (1.0f - pfrac) + a1.x * pfrac, a0.y * (1.0f - pfrac) + a1.y * pfrac);
             const float2 i1 = make_float2(b0.x * (1.0f - pfrac) + b1.x * pfrac, b0.y * (1.0f - pfrac) + b1.y * pfrac);
             val = make_float2(i0.x * tfrac + i1.x * (1.0f - tfrac), i0.y * tfrac + i1.y * (1.0f - tfrac));
@@ -644,12 +655,12 @@ __global__ void k_bst_p2c(const float2* polar, float2* cart, int n, int A2, int 
 // kernels, so the values are the same bit for bit.
 constexpr int BST_SPB = 8;
 struct BstRowsK {
-    const float2* polar; // [S][A2][n]
+    const float2* polar; // [S][A2][n/2]
     float2* out;         // [S][s][N]
     int n, A2, s, S, N, L;
 };
 
-template <int THREADS, int E>
+template <int THREADS, int E, int EC>
 __global__ __launch_bounds__(THREADS) void k_bst_rows(BstRowsK a, const float2* twg, FftK p)
 {
     extern __shared__ float2 z[];
@@ -661,16 +672,19 @@ __global__ __launch_bounds__(THREADS) void k_bst_rows(BstRowsK a, const float2* 
     const int tyd = blockIdx.x;
     const int sl0 = blockIdx.y * BST_SPB, sl1 = min(a.S, sl0 + BST_SPB);
     const int ty = (tyd + s - s / 2) % s; // (ty + s/2) % s == tyd
-    // the cells txd = threadIdx.x + j THREADS of this row (THREADS E >= s)
-    int o0[E], o1[E]; // polar offsets of (iarc, irho) and (uarc, irho); -1: zero cell (or past s)
-    float pf[E], tf[E], sg[E];
+    // the cells txd = threadIdx.x + j THREADS of this row (THREADS EC >= s), in
+    // three registers each: oc = polar offset of (iarc, irho), with bit 30 set
+    // when uarc wraps to 0 (-1: zero cell or past s); ps = the radial fraction
+    // with the cell's checkerboard sign in its sign bit; tf = the angular one
+    int oc[EC];
+    float ps[EC], tf[EC];
+    const int h = n / 2;               // polar rows hold radii [0, n/2)
+    const int wrapb = (a.A2 - 1) * h; // (iarc, irho) -> (0, irho)
 #pragma unroll
-    for (int j = 0; j < E; ++j) {
+    for (int j = 0; j < EC; ++j) {
         const int txd = threadIdx.x + j * THREADS;
-        o0[j] = -1;
-        o1[j] = 0;
-        pf[j] = tf[j] = 0.0f;
-        sg[j] = 1.0f;
+        oc[j] = -1;
+        ps[j] = tf[j] = 0.0f;
         if (txd < s) {
             const int tx = (txd + s - s / 2) % s;
             const int posx = tx - s / 2, posy = ty - s / 2;
@@ -678,36 +692,39 @@ __global__ __launch_bounds__(THREADS) void k_bst_rows(BstRowsK a, const float2* 
             const double angle = (double)a.A2 * (0.5 * atan2((double)posy, (double)posx) / M_PI + 0.5);
             const long long irho = (long long)rho;
             int iarc = (int)angle;
-            sg[j] = ((tx & 1) == (ty & 1)) ? 1.0f : -1.0f;
+            float pf = 0.0f;
             if (irho < n / 2 - 1) {
-                pf[j] = (float)(rho - (double)irho);
+                pf = (float)(rho - (double)irho);
                 tf[j] = (float)((double)iarc - angle);
                 iarc = iarc % a.A2;
-                const int uarc = (iarc + 1) % a.A2;
-                o0[j] = iarc * n + (int)irho;
-                o1[j] = uarc * n + (int)irho;
+                oc[j] = (iarc * h + (int)irho) | (iarc + 1 == a.A2 ? (1 << 30) : 0);
             }
+            ps[j] = ((tx & 1) == (ty & 1)) ? pf : -pf;
         }
     }
     for (int sl = sl0; sl < sl1; ++sl) {
-        const float2* P = a.polar + (size_t)sl * a.A2 * n;
+        const float2* P = a.polar + (size_t)sl * a.A2 * h;
         __syncthreads(); // the previous slice's row is read (and the twiddles are in)
 #pragma unroll
-        for (int j = 0; j < E; ++j) {
+        for (int j = 0; j < EC; ++j) {
             const int txd = threadIdx.x + j * THREADS;
             if (txd < s) {
                 float2 val = make_float2(0.0f, 0.0f);
-                if (o0[j] >= 0) {
-                    const float pfrac = pf[j], tfrac = tf[j];
-                    const float2 a0 = P[o0[j]], a1 = P[o0[j] + 1];
-                    const float2 b0 = P[o1[j]], b1 = P[o1[j] + 1];
+                const float pfrac = fabsf(ps[j]);
+                if (oc[j] >= 0) {
+                    const int o0 = oc[j] & ((1 << 30) - 1);
+                    const int o1 = (oc[j] >> 30) ? o0 - wrapb : o0 + h;
+                    const float tfrac = tf[j];
+                    const float2 a0 = P[o0], a1 = P[o0 + 1];
+                    const float2 b0 = P[o1], b1 = P[o1 + 1];
                     const float2 i0 = make_float2(a0.x * (1.0f - pfrac) + a1.x * pfrac,
                                                   a0.y * (1.0f - pfrac) + a1.y * pfrac);
                     const float2 i1 = make_float2(b0.x * (1.0f - pfrac) + b1.x * pfrac,
                                                   b0.y * (1.0f - pfrac) + b1.y * pfrac);
                     val = make_float2(i0.x * tfrac + i1.x * (1.0f - tfrac), i0.y * tfrac + i1.y * (1.0f - tfrac));
                 }
-                const float2 v = make_float2(val.x * sg[j], val.y * sg[j]);
+                const float sg = __builtin_signbit(ps[j]) ? -1.0f : 1.0f;
+                const float2 v = make_float2(val.x * sg, val.y * sg);
                 z[pix(txd)] = make_float2(v.x, -v.y);
             }
         }
@@ -763,12 +780,12 @@ void launch_bst_polar_t(const BstPolarK& k, const FftK& p, size_t lds, hipStream
                        p);
 }
 
-template <int THREADS, int E>
+template <int THREADS, int E, int EC = E>
 void launch_bst_rows_t(const BstRowsK& k, const float2* tw, const FftK& p, size_t lds, hipStream_t st)
 {
-    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_rows<THREADS, E>),
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_rows<THREADS, E, EC>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_bst_rows<THREADS, E>), dim3((unsigned)k.s, (unsigned)((k.S + BST_SPB - 1) / BST_SPB)),
+    hipLaunchKernelGGL((k_bst_rows<THREADS, E, EC>), dim3((unsigned)k.s, (unsigned)((k.S + BST_SPB - 1) / BST_SPB)),
                        dim3(THREADS), lds, st, k, tw, p);
 }
 

@@ -772,13 +772,13 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
     const bool fused = !(fenv && fenv[0] == '0');
     const size_t s2 = (size_t)n * n;
     const size_t cart_elems = fused ? (size_t)n * N : s2;
-    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * n * 8 + (cart_elems + (size_t)n * N) * 8;
+    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * (n / 2) * 8 + (cart_elems + (size_t)n * N) * 8;
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
     B = ((B + 3) / 4) * 4;
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
     float* q = static_cast<float*>(D.workspace<0>((size_t)B * A * n * 4));
-    float2* polar = static_cast<float2*>(D.workspace<1>((size_t)B * 2 * A * n * 8));
+    float2* polar = static_cast<float2*>(D.workspace<1>((size_t)B * 2 * A * (n / 2) * 8));
     float2* cart = static_cast<float2*>(D.workspace<2>((size_t)B * cart_elems * 8));
     float2* cartT = static_cast<float2*>(D.workspace<3>((size_t)B * n * N * 8));
     for (int s0 = 0; s0 < c.S; s0 += B) {

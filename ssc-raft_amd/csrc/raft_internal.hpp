@@ -136,7 +136,7 @@ struct FilterArgs {
 void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
 
 // BST after the row filter: q = filtered padded rows (interleaved [G][A][n][4],
-// window [0, n)), polar [S][2A][n], obj [S][N][N]; tw the twiddle tables of n.
+// window [0, n)), polar [S][2A][n/2], obj [S][N][N]; tw the twiddle tables of n.
 // fused: cart [S][n][N] (the kept columns of the inverse-transformed rows) and
 // cartT [S][N][n]; unfused: cart [S][n][n] (the whole frequency grid) and cartT
 // [S][N][n]
