@@ -237,8 +237,11 @@ __device__ __forceinline__ void kl_flush(const ProjK& a, int g, int slot, int la
     }
 }
 
+// Four waves per SIMD (<= 128 VGPRs): two workgroups per CU (the eEM form
+// held 141 VGPRs and ran one; it now spills two loop invariants, reloaded
+// once per ray block).
 template <int EM>
-__global__ __launch_bounds__(64 * PROJ_ANGLES) void k_project(ProjK a)
+__global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project(ProjK a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int a0 = blockIdx.x * PROJ_ANGLES;
