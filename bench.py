@@ -162,7 +162,7 @@ def cpu_baseline(args, host, ang, N, A):
     ref = oracle.fbp(np.ascontiguousarray(host["y"][:ne]), ang, pfe, pie, prec="f64", nthreads=cores)
     cpu["errors_vs_fp64_oracle"] = dict(
         slices=ne, gpu_f32_rows=_err(host["fbp_f32"], ref), gpu_f16_rows=_err(host["fbp_f16"], ref),
-        cpu_port=_err(port, ref[:ns]))
+        cpu_port=_err(port[:min(ns, ne)], ref[:min(ns, ne)]))
     log(f"FP64 oracle on {ne} slices: {time.perf_counter() - t0:.1f} s; GPU rel-L2 "
         f"{cpu['errors_vs_fp64_oracle']['gpu_f32_rows']['rel_l2']:.2e} (FP32 rows), "
         f"{cpu['errors_vs_fp64_oracle']['gpu_f16_rows']['rel_l2']:.2e} (FP16 rows)")
@@ -617,7 +617,7 @@ def main():
     ap.add_argument("--no-batch-compare", action="store_true",
                     help="skip the second C4 run with every stage per bucket (graph branches)")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=4)
+    ap.add_argument("--cpu-slices", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-next", action="store_true", help="skip the low-pass / BST / flat-dark / tEM timings")
