@@ -737,7 +737,7 @@ __global__ __launch_bounds__(64 * PROJ_ANGLES, 4) void k_project_lds_queue(const
         b = __builtin_amdgcn_readfirstlane(b);
         const int local = item - qu.first[b];
         const int nab = qu.nab[b], RC = descs[b].RC;
-        proj_lds_item<EM, 0>(descs[b], local % nab, (local / nab) % RC, local / (nab * RC), sm);
+        proj_lds_item<EM, PL_PF>(descs[b], local % nab, (local / nab) % RC, local / (nab * RC), sm);
     }
     if (threadIdx.x == 0) {
         __threadfence();
