@@ -93,11 +93,10 @@ template <> struct Acc<__half> {
     }
 };
 
-// LDS of one workgroup: the staged windows of CHUNK angles and their coefficients
+// LDS of one workgroup: the staged windows of CHUNK angles and their bin ranges
 template <int PAIR> constexpr int bp_slot() { return PAIR == 2 ? BP_SLOT_PAIR : BP_SLOT; }
 template <int CHUNK, int PAIR> struct BpSmem {
     uint4 win[PAIR * CHUNK * bp_slot<PAIR>()]; // PAIR halves of CHUNK * slot entries
-    double cA[CHUNK], cB[CHUNK];
     int tl[CHUNK], w[CHUNK], boff[CHUNK];
 };
 
@@ -105,7 +104,7 @@ template <int CHUNK, int PAIR> struct BpSmem {
 // missing second group of a pair reads zeros and stores nothing).  Starts
 // with a barrier (the LDS may still be read by the previous tile of a
 // persistent workgroup) and touches LDS only through sm.
-template <typename QT, int MODE, int CHUNK, int PAIR>
+template <typename QT, int MODE, int CHUNK, int PAIR, int SPF>
 __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSmem<CHUNK, PAIR>& sm)
 {
     static_assert(PAIR == 1 || (PAIR == 2 && Acc<QT>::SB == 4), "pairs are formed of FP32-row groups");
@@ -117,8 +116,6 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSm
     const bool second = PAIR == 2 && (g + 1) * SB1 < a.S; // the pair's second group exists
     constexpr int TX = BP_TILE_X, TY = BP_TILE_Y;
     uint4* win = sm.win;
-    double* s_cA = sm.cA;
-    double* s_cB = sm.cB;
     int* s_tl = sm.tl;
     int* s_w = sm.w;
     int* s_boff = sm.boff;
@@ -163,8 +160,6 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSm
             // any pixel's trunc(u) lies in [floor(umin)-1, floor(umax)+2]
             const int tl = max((int)floor(umin) - 1, a.tlo);
             const int th = min((int)floor(umax) + 2, thi_g);
-            s_cA[tid] = cA;
-            s_cB[tid] = cB;
             s_tl[tid] = tl;
             s_w[tid] = th - tl + 1;
             s_boff[tid] = (tid * SLOT - tl) * (int)sizeof(uint4); // wraps for large tl: used modulo 2^32
@@ -204,10 +199,12 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSm
                 for (int p = 0; p < 4; ++p)
 #pragma unroll
                     for (int s = 0; s < (TWO_LEVEL ? SB : 1); ++s) part[q][p][s] = 0.0f;
-            // 4 angles per trip: their coefficients come from one LDS base by immediate offsets
-#pragma unroll 4
-            for (int al = ag; al < ag + ng; ++al) {
-                const double cA = s_cA[al], cB = s_cB[al];
+            // One angle of the sweep: 8 FP64 bin indices, 8 (pair: 16) gathers.
+            // The angles' FP64 coefficients are the same for every lane, so they
+            // come through scalar loads from the tables in memory (no LDS
+            // instruction: the gathers are what bounds the sweep); the window
+            // offsets stay in LDS (they depend on the tile).
+            auto sweep_angle = [&](int al, double cA, double cB) __attribute__((always_inline)) {
                 // byte offset of bin 0 of this angle's window: entry (al, T - tl)
                 const unsigned boff = (unsigned)s_boff[al];
                 int T[2][4];
@@ -233,6 +230,41 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSm
 #pragma unroll
                         for (int h = 0; h < PAIR; ++h)
                             Acc<QT>::add((TWO_LEVEL ? part[q][p] : tot[q][p]) + h * SB1, v[q][p][h]);
+            };
+            const int aend = ag + ng;
+            if constexpr (SPF == 0) {
+                // coefficients loaded where they are used, 4 angles per trip
+#pragma unroll 4
+                for (int al = ag; al < aend; ++al) sweep_angle(al, a.cA[a0 + al], a.cB[a0 + al]);
+            } else {
+                // scalar-prefetch form: the next trip's 4 coefficients are loaded
+                // while this trip's angles are swept (indices clamped into the
+                // chunk, so every load is in bounds).  Fewer registers (128 at
+                // 16 angles per chunk: four workgroups per CU).
+                double nxA[4], nxB[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = a0 + min(ag + u, nA - 1);
+                    nxA[u] = a.cA[i];
+                    nxB[u] = a.cB[i];
+                }
+#pragma unroll 1
+                for (int al4 = ag; al4 < aend; al4 += 4) {
+                    double cA4[4], cB4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        cA4[u] = nxA[u];
+                        cB4[u] = nxB[u];
+                        const int i = a0 + min(al4 + 4 + u, nA - 1);
+                        nxA[u] = a.cA[i];
+                        nxB[u] = a.cB[i];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (al4 + u >= aend) break;
+                        sweep_angle(al4 + u, cA4[u], cB4[u]);
+                    }
+                }
             }
             if constexpr (TWO_LEVEL) {
 #pragma unroll
@@ -293,11 +325,11 @@ __device__ __forceinline__ void bp_tile(const BpKArgs& a, int tile, int gp, BpSm
 }
 
 // the pair form is held to 256 VGPRs: two waves per SIMD, two workgroups per CU
-template <typename QT, int MODE, int CHUNK, int PAIR>
+template <typename QT, int MODE, int CHUNK, int PAIR, int SPF>
 __global__ __launch_bounds__(256, PAIR) void k_backproject(BpKArgs a)
 {
     __shared__ BpSmem<CHUNK, PAIR> sm;
-    bp_tile<QT, MODE, CHUNK, PAIR>(a, blockIdx.x, blockIdx.y, sm);
+    bp_tile<QT, MODE, CHUNK, PAIR, SPF>(a, blockIdx.x, blockIdx.y, sm);
 }
 
 // Persistent work-queue form: a resident grid takes items from a device
@@ -306,7 +338,7 @@ __global__ __launch_bounds__(256, PAIR) void k_backproject(BpKArgs a)
 // (i - first[b]) / tiles of the volume described by descs[b].  The host orders the segments by cost per
 // item (angles), largest first.  The last workgroup to leave clears the
 // counters for the next launch.
-template <typename QT, int MODE, int CHUNK, int PAIR>
+template <typename QT, int MODE, int CHUNK, int PAIR, int SPF>
 __global__ __launch_bounds__(256, PAIR) void k_backproject_queue(const BpKArgs* __restrict__ descs, BpQueue qu)
 {
     __shared__ BpSmem<CHUNK, PAIR> sm;
@@ -323,7 +355,7 @@ __global__ __launch_bounds__(256, PAIR) void k_backproject_queue(const BpKArgs* 
         b = __builtin_amdgcn_readfirstlane(b);
         const int local = item - qu.first[b];
         const int tiles = qu.tiles[b];
-        bp_tile<QT, MODE, CHUNK, PAIR>(descs[b], local % tiles, local / tiles, sm);
+        bp_tile<QT, MODE, CHUNK, PAIR, SPF>(descs[b], local % tiles, local / tiles, sm);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -514,29 +546,30 @@ void bp_desc_fill(const BpArgs& b, int store, void* dst)
 
 // kernel variant: (FP32 rows, store) x (FBP sums, eEM update) x angles per LDS
 // round x (one group or a pair of FP32-row groups per workgroup)
-template <int CHUNK, int PAIR> static const void* bp_kernel(int store, int mode, bool queue)
+template <int CHUNK, int PAIR, int SPF = 0> static const void* bp_kernel(int store, int mode, bool queue)
 {
     if (mode == 1)
-        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK, PAIR>)
-                     : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK, PAIR>);
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK, PAIR, SPF>)
+                     : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK, PAIR, SPF>);
     if (store == RAFT_STORE_F16) {
-        if (PAIR != 1) fail(RAFT_EINVAL, "internal: the pair form takes FP32 rows");
-        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK, 1>)
-                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK, 1>);
+        if (PAIR != 1 || SPF != 0) fail(RAFT_EINVAL, "internal: the pair and scalar-prefetch forms take FP32 rows");
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK, 1, 0>)
+                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK, 1, 0>);
     }
-    return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK, PAIR>)
-                 : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK, PAIR>);
+    return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK, PAIR, SPF>)
+                 : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK, PAIR, SPF>);
 }
 
 static const void* bp_kernel_any(int store, int mode, int variant, bool queue)
 {
-    if (mode == 1 && store != RAFT_STORE_F32) fail(RAFT_EINVAL, "eEM update runs on FP32 rows");
     if (!bp_variant_supported(variant, mode == 1 ? RAFT_STORE_F32 : store))
         fail(RAFT_EINVAL, "internal: unsupported backprojection variant " + std::to_string(variant));
     switch (variant == 0 ? BP_CHUNK : variant) {
     case 16: return bp_kernel<16, 1>(store, mode, queue);
     case 32: return bp_kernel<32, 1>(store, mode, queue);
     case 48: return bp_kernel<48, 1>(store, mode, queue);
+    case BP_SPF | 16: return bp_kernel<16, 1, 1>(store, mode, queue);
+    case BP_SPF | 32: return bp_kernel<32, 1, 1>(store, mode, queue);
     case BP_PAIR | 16: return bp_kernel<16, 2>(store, mode, queue);
     default: return bp_kernel<32, 2>(store, mode, queue); // BP_PAIR | 32
     }
@@ -550,7 +583,9 @@ bool bp_variant_supported(int variant, int store)
     case 32:
     case 48: return true;
     case BP_PAIR | 16:
-    case BP_PAIR | 32: return store == RAFT_STORE_F32;
+    case BP_PAIR | 32:
+    case BP_SPF | 16:
+    case BP_SPF | 32: return store == RAFT_STORE_F32;
     default: return false;
     }
 }

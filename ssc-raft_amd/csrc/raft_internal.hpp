@@ -74,6 +74,9 @@ constexpr int BP_CHUNK = 32;   // angles staged per LDS round (default; 16 and 4
 // variant flag: FP32 rows, two 4-slice groups per workgroup (8 slices per bin
 // lookup); BP_PAIR | 16 and BP_PAIR | 32 are the autotuner's pair variants
 constexpr int BP_PAIR = 1024;
+// | BP_SPF: the single form whose sweep loads the next 4 angles' coefficients
+// one trip ahead (fewer registers; same sums bit for bit)
+constexpr int BP_SPF = 2048;
 constexpr int BP_GROUP = 16;   // angles per partial sum of the FP32 two-level summation
 constexpr int BP_SLOT = 80;    // LDS entries per staged angle window (>= |(63, 31)| + 5)
 // the pair form's window (two workgroups per CU fit 160 KB of LDS at 32

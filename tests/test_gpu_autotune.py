@@ -1,7 +1,7 @@
 """The tile autotuner on the GPU: the first call of a geometry times every
 candidate (backprojector angle chunks 32/16/48 and, for FP32 rows, the pair
-form that takes two slice groups per workgroup at 32 or 16 angles per round;
-staged-projector ray-chunk counts) and keeps the fastest; every candidate
+form that takes two slice groups per workgroup at 32 or 16 angles per round
+and the scalar-prefetch sweep at 16 or 32; staged-projector ray-chunk counts) and keeps the fastest; every candidate
 computes the same values, so a choice changes speed only -- checked here by
 forcing each candidate through the cache file and comparing outputs bit for
 bit.  The slice counts leave the last pair of slice groups without its second
@@ -17,7 +17,8 @@ from sscRaft import phantom as ph
 pytestmark = pytest.mark.gpu
 
 PAIR = 1024  # BP_PAIR: the pair-form flag of a backprojector variant
-BP_VARIANTS = [16, 32, 48, PAIR | 16, PAIR | 32]
+SPF = 2048  # BP_SPF: the scalar-prefetch sweep flag
+BP_VARIANTS = [16, 32, 48, PAIR | 16, PAIR | 32, SPF | 16, SPF | 32]
 
 
 def _force(tmp_path, monkeypatch, entries):
@@ -79,7 +80,8 @@ def test_eem_and_radon_variants_agree(gpu, tmp_path, monkeypatch):
     rcs = [c for c, _ in choices[pr_key[0]]["ms"]]
     assert len(rcs) >= 2
     assert sorted(c for c, _ in choices[bp_key[0]]["ms"]) == BP_VARIANTS
-    for chunk, rc in [(16, rcs[0]), (48, rcs[-1]), (32, rcs[1]), (PAIR | 32, rcs[0]), (PAIR | 16, rcs[1])]:
+    for chunk, rc in [(16, rcs[0]), (48, rcs[-1]), (32, rcs[1]), (PAIR | 32, rcs[0]), (PAIR | 16, rcs[1]),
+                      (SPF | 16, rcs[0]), (SPF | 32, rcs[-1])]:
         _force(tmp_path, monkeypatch, {bp_key[0]: chunk, pr_key[0]: rc, rd_key[0]: rc})
         x1, kl1, _ = dv.eem_device(y, ang, K)
         assert torch.equal(x1, x0)
