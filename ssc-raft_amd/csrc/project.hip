@@ -402,8 +402,9 @@ __device__ __forceinline__ StepBox step_box(const int4& ub)
         const int pitch16 = (b.w + 15) & ~15;
         b.pitch = b.nr * pitch16 <= PL_CAP ? pitch16 : b.w + ((1 - b.w) & 3);
         // the launch was admitted only if every union fits; the clamp keeps
-        // every LDS access in bounds regardless
-        b.nr = min(b.nr, PL_CAP / b.pitch);
+        // every LDS access in bounds regardless (its division runs only when
+        // it changes nr)
+        if (b.nr * b.pitch > PL_CAP) b.nr = PL_CAP / b.pitch;
     }
     return b;
 }
@@ -660,13 +661,24 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
                     float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f); // this step's sum (two-level summation)
                     const int jmax = ks + PL_K <= a.nk ? PL_K / 2 : (a.nk - ks - kp + 1) >> 1; // ks + kp + 2j < nk
                     if (jmax == PL_K / 2) {
-#pragma unroll 8
-                        for (int j = 0; j < PL_K / 2; ++j) {
-                            const int R = (int)fma(kr, rK, rowi);
-                            const int C = (int)fma(kr, cK, coli);
-                            const unsigned e = tile_off(R, C, pitch16, moff16);
-                            sacc = add4(sacc, *reinterpret_cast<const float4*>(tb + e));
-                            kr += 2.0;
+                        // two samples per trip, both gathers issued before the first
+                        // add, so two LDS reads are in flight; the adds keep the order
+                        // of the one-sample loop (kr + 2 is exact: kr is an integer or
+                        // half-integer far below 2^52)
+#pragma unroll 2
+                        for (int j = 0; j < PL_K / 2; j += 4) {
+                            unsigned e[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const double kq = kr + 2.0 * q;
+                                e[q] = tile_off((int)fma(kq, rK, rowi), (int)fma(kq, cK, coli), pitch16, moff16);
+                            }
+                            float4 x[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const float4*>(tb + e[q]);
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) sacc = add4(sacc, x[q]);
+                            kr += 8.0;
                         }
                     } else {
                         for (int j = 0; j < jmax; ++j) {
