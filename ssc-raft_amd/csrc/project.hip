@@ -661,24 +661,27 @@ __device__ __forceinline__ void proj_lds_item(const ProjK& a, int ab, int rc, in
                     float4 sacc = make_float4(0.f, 0.f, 0.f, 0.f); // this step's sum (two-level summation)
                     const int jmax = ks + PL_K <= a.nk ? PL_K / 2 : (a.nk - ks - kp + 1) >> 1; // ks + kp + 2j < nk
                     if (jmax == PL_K / 2) {
-                        // two samples per trip, both gathers issued before the first
-                        // add, so two LDS reads are in flight; the adds keep the order
-                        // of the one-sample loop (kr + 2 is exact: kr is an integer or
-                        // half-integer far below 2^52)
+                        // GQ samples per trip, all their gathers issued before the
+                        // first add, so GQ LDS reads are in flight; the adds keep the
+                        // order of the one-sample loop (kr + 2q is exact: kr is an
+                        // integer or half-integer far below 2^52).  GQ as timed per
+                        // mode at the 128-register budget: 8 for the eEM form, 4 for
+                        // the others (8 made Radon ~1% slower)
+                        constexpr int GQ = EM == 1 ? 8 : 4;
 #pragma unroll 2
-                        for (int j = 0; j < PL_K / 2; j += 4) {
-                            unsigned e[4];
+                        for (int j = 0; j < PL_K / 2; j += GQ) {
+                            unsigned e[GQ];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) {
+                            for (int q = 0; q < GQ; ++q) {
                                 const double kq = kr + 2.0 * q;
                                 e[q] = tile_off((int)fma(kq, rK, rowi), (int)fma(kq, cK, coli), pitch16, moff16);
                             }
-                            float4 x[4];
+                            float4 x[GQ];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const float4*>(tb + e[q]);
+                            for (int q = 0; q < GQ; ++q) x[q] = *reinterpret_cast<const float4*>(tb + e[q]);
 #pragma unroll
-                            for (int q = 0; q < 4; ++q) sacc = add4(sacc, x[q]);
-                            kr += 8.0;
+                            for (int q = 0; q < GQ; ++q) sacc = add4(sacc, x[q]);
+                            kr += 2.0 * GQ;
                         }
                     } else {
                         for (int j = 0; j < jmax; ++j) {
