@@ -213,14 +213,20 @@ def learned_eem(net, p_hat, y, ang, iterations, cfg=WarmStartConfig, batch=None,
     FBP start needs a projection of its own.  Slices whose network start has a
     higher KL (or more excluded bins) than their FBP start are run again from
     the FBP start.  The result equals warm_start followed by eem_device.
-    finish = (slice indices, iterations): those slices are then run again from
-    their FBP start for that many iterations (slices whose learned start is
-    known not to reach their target).
+    finish = (slice indices, iterations): slices whose learned start is known
+    not to reach their target, and the iterations their FBP start needs.  When
+    that is at most `iterations`, they start from their FBP start in the same
+    run (which gives them `iterations`); otherwise they are run again from it
+    afterwards for that many iterations.
 
     Returns (x [S, N, N], accepted mask)."""
     import torch
 
     x0, _, xf, _ = warm_start(net, p_hat, y, ang, cfg, batch=batch, check=False)
+    if finish is not None and len(finish[0]) > 0 and 0 < finish[1] <= iterations:
+        idx = torch.from_numpy(np.asarray(finish[0], np.int64)).to(x0.device)
+        x0.index_copy_(0, idx, xf.index_select(0, idx))
+        finish = None
     kl_fbp, ex_fbp = eem_kl_device(y, ang, xf)
     if iterations == 0:
         kl_net, ex_net = eem_kl_device(y, ang, x0)
@@ -315,7 +321,7 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
             x, _ = learned_eem(net, p_hat, y, ang, Kl, cfg, finish=finish)
             return x
 
-        learned()  # untimed: tunes and sets up the shapes of the finishing run (as plain's untimed run does)
+        learned()  # untimed: tunes and sets up the shapes (as plain's untimed run does)
         _, t_learned = timed(learned)
         res["learned_seconds"] = t_learned
         res["speedup"] = t_plain / t_learned
@@ -323,7 +329,9 @@ def evaluate(net, count=512, cfg=WarmStartConfig, max_iterations=None, log=print
         res["speedup_note"] = (
             "all slices reach the target from the learned start" if reached.all() else
             f"all {done} of {count} slices counted: {int(reached.sum())} reach the target from the learned start, "
-            f"{int(miss.size)} from their FBP start ({finish[1] if finish else 0} more iterations, timed)")
+            f"{int(miss.size)} from their FBP start"
+            + (f" in the same run (they need {finish[1]} of its {Kl} iterations)" if finish and finish[1] <= Kl
+               else f" ({finish[1] if finish else 0} more iterations, timed)"))
     if Kf is not None:
         def fbp_only():
             x = torch.clamp(fbp_start(p_hat, ang, cfg), min=cfg.floor)

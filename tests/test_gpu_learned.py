@@ -119,3 +119,47 @@ def test_learned_eem_equals_warm_start_then_eem(gpu, monkeypatch):
     x, acc = learned.learned_eem(net, p_hat, y, ang, 5, cfg)
     np.testing.assert_array_equal(acc, acc_ref)
     assert torch.equal(x, ref)
+
+
+
+def test_learned_eem_finishing_slices_start_from_fbp_in_the_same_run(gpu):
+    """Slices listed in `finish` whose FBP start needs no more iterations than
+    the run has start from their FBP start inside the run: the result equals
+    the acceptance rule and eEM over the warm start with those slices' starts
+    replaced by their FBP starts; a finishing count above the run's
+    iterations keeps the separate run afterwards."""
+    import torch
+
+    from sscRaft import learned
+    from sscRaft.device import eem_device, eem_kl_device
+    from sscRaft.nn import UNet
+
+    net = UNet.load(learned.default_weights_path())
+    cfg = learned.WarmStartConfig
+    p_hat, y, _, ang = learned.sinograms(6, cfg.heldout_first + 7100, cfg)
+    x0, _, xf, _ = learned.warm_start(net, p_hat, y, ang, cfg, check=False)
+    kf, ef = eem_kl_device(y, ang, xf)
+
+    def accepted_start(start):
+        kn, en = eem_kl_device(y, ang, start)
+        acc = torch.from_numpy((kn <= kf) & (en <= ef)).cuda()
+        return torch.where(acc[:, None, None], start, xf).contiguous()
+
+    fin = [1, 4]
+    x0f = x0.clone()
+    x0f[fin] = xf[fin]
+    ref = accepted_start(x0f)
+    eem_device(y, ang, 5, obj=ref, with_kl=False)
+    x, acc = learned.learned_eem(net, p_hat, y, ang, 5, cfg, finish=(np.array(fin), 3))
+    assert acc[fin].all()
+    assert torch.equal(x, ref)
+    # more finishing iterations than the run: those slices are run again, from
+    # their FBP start, for 7 iterations after it
+    ref2 = accepted_start(x0.clone())
+    eem_device(y, ang, 5, obj=ref2, with_kl=False)
+    xr = xf[fin].contiguous()
+    eem_device(y[fin].contiguous(), ang, 7, obj=xr, with_kl=False)
+    x2, _ = learned.learned_eem(net, p_hat, y, ang, 5, cfg, finish=(np.array(fin), 7))
+    keep = [i for i in range(6) if i not in fin]
+    assert torch.equal(x2[keep], ref2[keep])
+    assert torch.equal(x2[fin], xr)
