@@ -587,6 +587,96 @@ __global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_bst_polar(BstPo
     }
 }
 
+// One polar transform of k_bst_filter_polar: FFT of z, then the two real
+// transforms it carries, separated and times sigma, into row offset `row` of
+// the first slice (pa) and, if present, of the second (pb)
+template <int THREADS, int E>
+__device__ __forceinline__ void bst_pair_rows(float2* z, const Tw& tw, const FftK& p, float2* pa, float2* pb,
+                                              size_t row, int h, int n, bool two)
+{
+    fft_run<THREADS, E>(z, tw, p);
+    for (int k = threadIdx.x; k < h; k += THREADS) {
+        const float sig = 2.0f / ((float)n * fmaxf(fminf((float)k, (float)(n - k)), 0.5f));
+        const float2 zk = z[pix(k)];
+        const float2 zm = z[pix(k == 0 ? 0 : n - k)];
+        const float hs = 0.5f * sig;
+        pa[row + k] = make_float2((zk.x + zm.x) * hs, (zk.y - zm.y) * hs);
+        if (two) pb[row + k] = make_float2((zk.y + zm.y) * hs, (zm.x - zk.x) * hs);
+    }
+    __syncthreads();
+}
+
+// Filter and polar FFTs in one pass (the default BST form): one workgroup =
+// one slice pair (s0, s0 + 1) of one angle.  The pair's padded rows are
+// filtered as k_filter_fft does (one complex FFT carries both real rows), and
+// the filtered values q_a = Re z wv, q_b = sign Im z wv -- the values the
+// staged rows would hold -- go straight into the polar transforms, so the
+// filtered rows are never written.  The polar rows of both slices come from
+// two complex FFTs: z1 = w (q_a + i q_b) over the left half-rows (mirrored)
+// and z2 = w (q_a + i q_b) over the right half-rows; the two real transforms
+// each carries are separated by Hermitian symmetry as in k_bst_polar (first
+// slice: (Z_k + conj Z_-k) / 2, second: (Z_k - conj Z_-k) / 2i), times sigma.
+// The right half-rows wait in LDS (rh, n/2 entries) while z1 is transformed.
+template <int THREADS, int E>
+__global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_bst_filter_polar(FilterK a, BstPolarK b, FftK p)
+{
+    extern __shared__ float2 z[];
+    const int n = a.npad;
+    float2* tws = z + lds_elems(n);
+    const int ntw = 128 + n / 128 + 1;
+    float2* rh = tws + ntw;
+    for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
+    const Tw tw{tws, tws + 128};
+    const PairIdx ix = pair_index(a);
+    load_pair<THREADS>(a, ix, z);
+    __syncthreads();
+    float sign1 = 1.0f;
+    if (!a.none) {
+        fft_run<THREADS, E>(z, tw, p);
+        for (int k = threadIdx.x; k < n; k += THREADS) {
+            const float2 v = cmul(z[pix(k)], a.H[k]);
+            z[pix(k)] = make_float2(v.x, -v.y);
+        }
+        __syncthreads();
+        fft_run<THREADS, E>(z, tw, p);
+        sign1 = -1.0f;
+    }
+    const int c = n / 2, h = n / 2;
+    const float wv = a.wa[ix.a];
+    // mirrored left half-rows through registers back into z (zeros past h),
+    // right half-rows into rh
+    {
+        constexpr int JL = (E + 1) / 2; // h <= E THREADS / 2
+        float2 lv[JL];
+#pragma unroll
+        for (int j = 0; j < JL; ++j) {
+            const int idx = threadIdx.x + j * THREADS;
+            lv[j] = make_float2(0.0f, 0.0f);
+            if (idx < h) {
+                const float w = idx == 0 ? 2.0f : 4.0f;
+                const float2 zl = z[pix(c - 1 - idx)], zr = z[pix(c + idx)];
+                lv[j] = make_float2(zl.x * wv * w, sign1 * zl.y * wv * w);
+                rh[idx] = make_float2(zr.x * wv * w, sign1 * zr.y * wv * w);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < JL; ++j) {
+            const int idx = threadIdx.x + j * THREADS;
+            if (idx < h) z[pix(idx)] = lv[j];
+        }
+    }
+    for (int idx = h + threadIdx.x; idx < n; idx += THREADS) z[pix(idx)] = make_float2(0.0f, 0.0f);
+    __syncthreads();
+    const bool two = ix.s0 + 1 < a.S;
+    float2* pa = b.polar + (size_t)ix.s0 * 2 * b.A * h;
+    float2* pb = pa + (size_t)2 * b.A * h;
+    bst_pair_rows<THREADS, E>(z, tw, p, pa, pb, (size_t)ix.a * h, h, n, two);
+    for (int idx = threadIdx.x; idx < n; idx += THREADS) z[pix(idx)] = idx < h ? rh[idx] : make_float2(0.0f, 0.0f);
+    __syncthreads();
+    bst_pair_rows<THREADS, E>(z, tw, p, pa, pb, (size_t)(b.A + ix.a) * h, h, n, two);
+}
+
 // unnormalised inverse C2C FFT of `rows` rows of length n, in place
 template <int THREADS, int E>
 __global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_ifft_rows(float2* data, const float2* twg, FftK p)
@@ -780,6 +870,15 @@ void launch_bst_polar_t(const BstPolarK& k, const FftK& p, size_t lds, hipStream
                        p);
 }
 
+template <int THREADS, int E>
+void launch_bst_filter_polar_t(const FilterK& f, const BstPolarK& k, const FftK& p, size_t lds, hipStream_t st)
+{
+    RAFT_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bst_filter_polar<THREADS, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const long long blocks = (long long)((f.S + f.SB - 1) / f.SB) * f.A * (f.SB / 2);
+    hipLaunchKernelGGL((k_bst_filter_polar<THREADS, E>), dim3((unsigned)blocks), dim3(THREADS), lds, st, f, k, p);
+}
+
 template <int THREADS, int E, int EC = E>
 void launch_bst_rows_t(const BstRowsK& k, const float2* tw, const FftK& p, size_t lds, hipStream_t st)
 {
@@ -896,7 +995,12 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
     RAFT_HIP(hipGetLastError());
 }
 
-void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
+bool bst_filter_polar_fits(int n)
+{
+    return sizeof(float2) * (lds_elems(n) + (size_t)(128 + n / 128 + 1) + (size_t)(n / 2)) <= 160 * 1024;
+}
+
+void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st, const FilterArgs* filt)
 {
     if (a.S <= 0) return;
     if (!plan.ok) fail(RAFT_EUNSUPPORTED, "BST needs a padded size n = N(1+pad) <= 16384 with factors 2, 3, 5");
@@ -906,7 +1010,33 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st)
     if ((long long)a.S * 2 * a.A > 0x7fffffffLL || (long long)a.S * s > 0x7fffffffLL)
         fail(RAFT_EUNSUPPORTED, "BST block too large for one launch");
     BstPolarK k{a.q, a.polar, a.tw, n, a.A, a.S};
-    if (n > 8192) launch_bst_polar_t<1024, 16>(k, p, lds, st);
+    if (filt) {
+        // filter + polar FFTs in one pass from the sinogram (k_bst_filter_polar)
+        if (filt->npad != n || filt->S != a.S || filt->A != a.A)
+            fail(RAFT_EINVAL, "BST filter arguments do not match the transform");
+        FilterK f;
+        f.tomo = filt->tomo; f.q = nullptr; f.H = filt->H; f.tw = filt->tw; f.g = nullptr; f.wa = filt->wa;
+        f.N = filt->N; f.A = filt->A; f.S = filt->S; f.npad = n; f.L = filt->L; f.tlo = 0; f.W = n;
+        f.none = filt->none;
+        f.SB = 4;
+        if ((long long)((f.S + f.SB - 1) / f.SB) * f.A * (f.SB / 2) > 0x7fffffffLL)
+            fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one BST launch");
+        const size_t lds_fp = lds + sizeof(float2) * (size_t)(n / 2);
+        if (lds_fp > 160 * 1024) fail(RAFT_EUNSUPPORTED, "padded row too long for the BST filter pass");
+        if (n > 8192) launch_bst_filter_polar_t<1024, 16>(f, k, p, lds_fp, st);
+        else {
+            switch (plan.threads) {
+            case 64: launch_bst_filter_polar_t<64, 8>(f, k, p, lds_fp, st); break;
+            case 128: launch_bst_filter_polar_t<128, 8>(f, k, p, lds_fp, st); break;
+            case 192: launch_bst_filter_polar_t<192, 8>(f, k, p, lds_fp, st); break;
+            case 256: launch_bst_filter_polar_t<256, 8>(f, k, p, lds_fp, st); break;
+            case 384: launch_bst_filter_polar_t<384, 8>(f, k, p, lds_fp, st); break;
+            case 512: launch_bst_filter_polar_t<512, 8>(f, k, p, lds_fp, st); break;
+            case 768: launch_bst_filter_polar_t<768, 8>(f, k, p, lds_fp, st); break;
+            default: launch_bst_filter_polar_t<1024, 8>(f, k, p, lds_fp, st); break;
+            }
+        }
+    } else if (n > 8192) launch_bst_polar_t<1024, 16>(k, p, lds, st);
     else {
         switch (plan.threads) {
         case 64: launch_bst_polar_t<64, 8>(k, p, lds, st); break;

@@ -774,14 +774,20 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
     // gather and the row transforms as separate passes over the grid
     const char* fenv = std::getenv("RAFT_BST_FUSED");
     const bool fused = !(fenv && fenv[0] == '0');
+    // the filter runs inside the polar pass (default, where its LDS fits), so
+    // the filtered padded rows are never written; RAFT_BST_FILTER_FUSED=0
+    // filters into staged rows first, as a separate pass
+    const char* ffenv = std::getenv("RAFT_BST_FILTER_FUSED");
+    const bool ffused = !(ffenv && ffenv[0] == '0') && bst_filter_polar_fits(n);
     const size_t s2 = (size_t)n * n;
     const size_t cart_elems = fused ? (size_t)n * N : s2;
-    const size_t per_slice = (size_t)A * n * 4 + (size_t)2 * A * (n / 2) * 8 + (cart_elems + (size_t)n * N) * 8;
+    const size_t per_slice = (ffused ? 0 : (size_t)A * n * 4) + (size_t)2 * A * (n / 2) * 8 +
+                             (cart_elems + (size_t)n * N) * 8;
     int B = c.blocksize > 0 ? std::min(c.blocksize, c.S) : c.S;
     B = ((B + 3) / 4) * 4;
     const size_t budget = block_budget();
     while (B > 4 && (size_t)B * per_slice > budget) B = (((B + 1) / 2 + 3) / 4) * 4;
-    float* q = static_cast<float*>(D.workspace<0>((size_t)B * A * n * 4));
+    float* q = ffused ? nullptr : static_cast<float*>(D.workspace<0>((size_t)B * A * n * 4));
     float2* polar = static_cast<float2*>(D.workspace<1>((size_t)B * 2 * A * (n / 2) * 8));
     float2* cart = static_cast<float2*>(D.workspace<2>((size_t)B * cart_elems * 8));
     float2* cartT = static_cast<float2*>(D.workspace<3>((size_t)B * n * N * 8));
@@ -796,7 +802,7 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
         fa.wa = dwa.as<float>();
         fa.N = N; fa.A = A; fa.S = nb; fa.npad = n; fa.L = c.L; fa.tlo = 0; fa.W = n;
         fa.none = (c.filter == 0);
-        launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
+        if (!ffused) launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
         BstArgs ba;
         ba.q = q; ba.polar = polar; ba.cart = cart; ba.cartT = cartT;
         ba.obj = d_obj + (size_t)s0 * N * N;
@@ -804,7 +810,7 @@ static void bst_device(int dev, float* d_obj, const float* d_tomo, const float* 
         ba.N = N; ba.A = A; ba.S = nb; ba.L = c.L;
         ba.scale = (float)((double)n * c.pixel * 4.0);
         ba.fused = fused ? 1 : 0;
-        launch_bst(ba, F.plan, st);
+        launch_bst(ba, F.plan, st, ffused ? &fa : nullptr);
     }
     RAFT_HIP(hipStreamSynchronize(st));
 }

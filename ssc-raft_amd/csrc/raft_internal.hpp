@@ -154,7 +154,11 @@ struct BstArgs {
     float scale;        // 4 n pixel
     int fused = 1;      // gather + row inverse FFT in one kernel (k_bst_rows)
 };
-void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st);
+// filt != nullptr: the filter runs inside the polar pass from filt->tomo
+// (a.q unused); otherwise a.q holds the filtered rows
+void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st, const FilterArgs* filt = nullptr);
+// whether the one-pass filter + polar form's LDS fits at padded size n
+bool bst_filter_polar_fits(int n);
 
 struct BpArgs {
     const void* q;      // [G][A][W][SB] staged rows (float SB = 4, or half SB = 8)

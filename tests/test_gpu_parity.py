@@ -621,6 +621,49 @@ def test_bst_fused_rows_match_separate_passes(gpu, monkeypatch, n, a, pad, slice
     np.testing.assert_array_equal(fused, separate)
 
 
+@pytest.mark.parametrize("n,a,pad,slices,filt", [(64, 96, 2, 2, "ramp"), (100, 90, 1, 11, "hann"),
+                                                  (48, 40, 0, 9, "ramp"), (64, 48, 2, 5, "none")])
+def test_bst_filter_in_polar_pass_matches_separate_filter(gpu, monkeypatch, n, a, pad, slices, filt):
+    """The filter inside the polar pass (k_bst_filter_polar, the default)
+    against the filter as a separate pass into staged rows
+    (RAFT_BST_FILTER_FUSED=0): the polar FFTs pair two slices' half-rows
+    instead of one slice's two half-rows, so the sums round differently;
+    within FP32 rounding (rel-L2 1e-6), and both within the oracle's gate.
+    Odd slice counts leave the last pair with one slice."""
+    import sscRaft
+
+    y, ang = noisy_sino(n, a, slices, seed=5 * n + a)
+    dic = fbp_dic(ang, pad=pad, filt=filt, pixel=2.0 / n)
+    dic["method"] = "BST"
+    one_pass = sscRaft.fbp(y, dic=dict(dic))
+    monkeypatch.setenv("RAFT_BST_FILTER_FUSED", "0")
+    separate = sscRaft.fbp(y, dic=dict(dic))
+    assert np.isfinite(one_pass).all()
+    e = rel(one_pass, separate)
+    print(f"bst filter pass n={n} a={a} pad={pad} slices={slices} {filt}: rel-L2 {e:.2e}")
+    assert e <= 1e-6
+
+
+def test_bst_filter_in_polar_pass_above_8192_points(gpu, monkeypatch):
+    """The one-pass BST filter at a padded size above 8192 points (4096 bins,
+    pad 2: 12288, the 16-elements-per-thread FFT) against the separate
+    filter pass, on random rows (an odd slice count)."""
+    import sscRaft
+
+    n, a, S = 4096, 6, 3
+    rng = np.random.default_rng(12288)
+    y = rng.standard_normal((S, a, n)).astype(np.float32)
+    dic = fbp_dic(ph.uniform_angles(a), pad=2, filt="ramp", pixel=2.0 / n)
+    dic["method"] = "BST"
+    one_pass = sscRaft.fbp(y, dic=dict(dic))
+    monkeypatch.setenv("RAFT_BST_FILTER_FUSED", "0")
+    separate = sscRaft.fbp(y, dic=dict(dic))
+    assert np.isfinite(one_pass).all()
+    e = rel(one_pass, separate)
+    print(f"bst filter pass n={n} npad=12288: rel-L2 {e:.2e}")
+    assert e <= 1e-6
+
+
 def test_slab_split_over_repeated_device_ids(gpu):
     """The host-pointer entry points give each entry of gpus[] one contiguous
     slab of slices, each run by its own host thread bound to that device
