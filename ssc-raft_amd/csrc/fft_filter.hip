@@ -628,6 +628,9 @@ __global__ __launch_bounds__(THREADS, fft_wpe<THREADS>()) void k_bst_filter_pola
     for (int k = threadIdx.x; k < ntw; k += THREADS) tws[k] = a.tw[k];
     const Tw tw{tws, tws + 128};
     const PairIdx ix = pair_index(a);
+    // the second pair of a last group with one slice has no slice: it writes
+    // nothing (uniform for the workgroup, before any barrier)
+    if (ix.s0 >= a.S) return;
     load_pair<THREADS>(a, ix, z);
     __syncthreads();
     float sign1 = 1.0f;
