@@ -330,12 +330,12 @@ struct PinnedBuf {
 };
 
 struct Device {
-    static constexpr int kSlots = 12;
+    static constexpr int kSlots = 14;
     std::mutex mu;
     std::map<std::string, std::unique_ptr<FilterTables>> filters;
     DevBuf ws[kSlots];
     DevBuf tab; // per-call tables (see TableSet)
-    PinnedBuf pin[4];                     // host-pointer entry points: staging of two chunks in and out
+    PinnedBuf pin[6];                     // host-pointer entry points: staging of up to three chunks in and out
     std::vector<std::vector<char>> keep;  // host bytes of table uploads in flight (TableSet::keep_alive)
     // the slot is a template argument so that every index is checked at compile time
     template <int SLOT> void* workspace(size_t bytes)
@@ -1859,10 +1859,13 @@ static void fbp_host(int* gpus, int ngpus, float* obj, float* tomo, float* angle
     run_slabs(gpus, ngpus, c.S, [&](int dev, int first, int cnt) {
         if (cnt <= 0) return;
         bind_device(dev);
-        // Pipeline over chunks of whole slice groups, two in flight: the copy
+        // Pipeline over chunks of whole slice groups, NB in flight: the copy
         // pool moves chunk k from the caller's array into page-locked staging
-        // while the GPU filters and backprojects chunk k-1 and the DMA engines
-        // move chunks in and out on their own streams.
+        // while the GPU filters and backprojects the chunks before it and the
+        // DMA engines move chunks in and out on their own streams.  The pool
+        // copies chunk k - (NB - 1) out after issuing chunk k, so with NB = 3
+        // it waits for a chunk whose copies and FBP had two chunk periods to
+        // finish (RAFT_COPY_DEPTH=2: one period).
         const size_t tin = (size_t)c.A * c.N, oout = (size_t)c.N * c.N;
         const size_t target = (size_t)64 << 20; // ~64 MB of sinogram per chunk
         int C = (int)std::max<size_t>(1, target / (tin * 4));
@@ -1870,14 +1873,20 @@ static void fbp_host(int* gpus, int ngpus, float* obj, float* tomo, float* angle
         C = std::max(BP_SLICES_F32, (C / BP_SLICES_F32) * BP_SLICES_F32);
         C = std::min(C, cnt);
         const int nch = (cnt + C - 1) / C;
+        const char* de = std::getenv("RAFT_COPY_DEPTH");
+        const int NB = (de && std::atoi(de) == 2) ? 2 : 3;
         Device& D = device_state(dev);
         std::lock_guard<std::mutex> lk(D.mu);
-        void* pin_in[2] = {D.pin[0].ensure((size_t)C * tin * 4), D.pin[1].ensure((size_t)C * tin * 4)};
-        void* pin_out[2] = {D.pin[2].ensure((size_t)C * oout * 4), D.pin[3].ensure((size_t)C * oout * 4)};
-        float* d_in[2] = {static_cast<float*>(D.workspace<8>((size_t)C * tin * 4)),
-                          static_cast<float*>(D.workspace<9>((size_t)C * tin * 4))};
-        float* d_out[2] = {static_cast<float*>(D.workspace<10>((size_t)C * oout * 4)),
-                           static_cast<float*>(D.workspace<11>((size_t)C * oout * 4))};
+        void* pin_in[3] = {D.pin[0].ensure((size_t)C * tin * 4), D.pin[1].ensure((size_t)C * tin * 4),
+                           NB > 2 ? D.pin[4].ensure((size_t)C * tin * 4) : nullptr};
+        void* pin_out[3] = {D.pin[2].ensure((size_t)C * oout * 4), D.pin[3].ensure((size_t)C * oout * 4),
+                            NB > 2 ? D.pin[5].ensure((size_t)C * oout * 4) : nullptr};
+        float* d_in[3] = {static_cast<float*>(D.workspace<8>((size_t)C * tin * 4)),
+                          static_cast<float*>(D.workspace<9>((size_t)C * tin * 4)),
+                          NB > 2 ? static_cast<float*>(D.workspace<12>((size_t)C * tin * 4)) : nullptr};
+        float* d_out[3] = {static_cast<float*>(D.workspace<10>((size_t)C * oout * 4)),
+                           static_cast<float*>(D.workspace<11>((size_t)C * oout * 4)),
+                           NB > 2 ? static_cast<float*>(D.workspace<13>((size_t)C * oout * 4)) : nullptr};
         Stream s_in, s_comp, s_out;
         std::vector<hipEvent_t> ev;
         auto mk = [&]() {
@@ -1897,34 +1906,40 @@ static void fbp_host(int* gpus, int ngpus, float* obj, float* tomo, float* angle
                 D.keep.clear();
             }
         } cleanup{D, ev, {s_in.s, s_comp.s, s_out.s}};
-        hipEvent_t ev_h2d[2] = {mk(), mk()}, ev_comp[2] = {mk(), mk()}, ev_d2h[2] = {mk(), mk()};
+        hipEvent_t ev_h2d[3], ev_comp[3], ev_d2h[3];
+        for (int b = 0; b < NB; ++b) {
+            ev_h2d[b] = mk();
+            ev_comp[b] = mk();
+            ev_d2h[b] = mk();
+        }
         CopyPool pool(copy_threads());
         advise_huge(obj + (size_t)first * oout, (size_t)cnt * oout * 4);
         auto rows_of = [&](int k) { return std::min(C, cnt - k * C); };
         auto host_out = [&](int k) {
-            const int b = k & 1;
+            const int b = k % NB;
             RAFT_HIP(hipEventSynchronize(ev_d2h[b]));
             pool.copy(obj + (size_t)(first + k * C) * oout, pin_out[b], (size_t)rows_of(k) * oout * 4);
         };
         for (int k = 0; k < nch; ++k) {
-            const int b = k & 1, nb = rows_of(k);
-            if (k >= 2) RAFT_HIP(hipEventSynchronize(ev_h2d[b])); // pin_in[b] read by the copy of chunk k-2
+            const int b = k % NB, nb = rows_of(k);
+            if (k >= NB) RAFT_HIP(hipEventSynchronize(ev_h2d[b])); // pin_in[b] read by the copy of chunk k-NB
             pool.copy(pin_in[b], tomo + (size_t)(first + k * C) * tin, (size_t)nb * tin * 4);
-            if (k >= 2) RAFT_HIP(hipStreamWaitEvent(s_in.s, ev_comp[b], 0)); // d_in[b] read by chunk k-2's FBP
+            if (k >= NB) RAFT_HIP(hipStreamWaitEvent(s_in.s, ev_comp[b], 0)); // d_in[b] read by chunk k-NB's FBP
             RAFT_HIP(hipMemcpyAsync(d_in[b], pin_in[b], (size_t)nb * tin * 4, hipMemcpyHostToDevice, s_in.s));
             RAFT_HIP(hipEventRecord(ev_h2d[b], s_in.s));
             RAFT_HIP(hipStreamWaitEvent(s_comp.s, ev_h2d[b], 0));
-            if (k >= 2) RAFT_HIP(hipStreamWaitEvent(s_comp.s, ev_d2h[b], 0)); // d_out[b] read by chunk k-2's copy
+            if (k >= NB) RAFT_HIP(hipStreamWaitEvent(s_comp.s, ev_d2h[b], 0)); // d_out[b] read by chunk k-NB's copy
             FbpCfg ck = c;
             ck.S = nb;
             fbp_run(D, d_out[b], d_in[b], angles, ck, RAFT_STORE_F32, s_comp.s, false);
             RAFT_HIP(hipEventRecord(ev_comp[b], s_comp.s));
             RAFT_HIP(hipStreamWaitEvent(s_out.s, ev_comp[b], 0));
+            // pin_out[b] was copied out (host_out(k - NB)) before this iteration
             RAFT_HIP(hipMemcpyAsync(pin_out[b], d_out[b], (size_t)nb * oout * 4, hipMemcpyDeviceToHost, s_out.s));
             RAFT_HIP(hipEventRecord(ev_d2h[b], s_out.s));
-            if (k >= 1) host_out(k - 1); // pin_out of chunk k-1 -> the caller's array
+            if (k >= NB - 1) host_out(k - (NB - 1)); // pin_out of chunk k-(NB-1) -> the caller's array
         }
-        host_out(nch - 1);
+        for (int k = std::max(0, nch - (NB - 1)); k < nch; ++k) host_out(k);
     });
 }
 
