@@ -722,6 +722,22 @@ def test_blocked_calls_match_one_block(gpu):
     np.testing.assert_array_equal(sscRaft.radon_RT(x0[:5], ang, [0], pixel=2.0 / n), r0[:5])
 
 
+@pytest.mark.parametrize("depth", ["2", "3"])
+def test_host_pointer_fbp_pipeline_depths(gpu, monkeypatch, depth):
+    """The host-pointer FBP's chunk pipeline (three chunks in flight by
+    default, two with RAFT_COPY_DEPTH=2) over six chunks of 4 slices (the last
+    one partial), so every staging buffer is reused: bit for bit what one
+    block gives."""
+    import sscRaft
+
+    n, a, S = 48, 40, 22
+    y, ang = noisy_sino(n, a, S, seed=29)
+    one = sscRaft.fbp(y, dic=dict(fbp_dic(ang, pad=2, pixel=2.0 / n), blocksize=0))
+    monkeypatch.setenv("RAFT_COPY_DEPTH", depth)
+    chunked = sscRaft.fbp(y, dic=dict(fbp_dic(ang, pad=2, pixel=2.0 / n), blocksize=4))
+    np.testing.assert_array_equal(chunked, one)
+
+
 def test_zero_slices_are_a_no_op(gpu):
     """Device entry points accept an empty slice range and touch nothing."""
     import torch
