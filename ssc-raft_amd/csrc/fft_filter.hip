@@ -227,6 +227,7 @@ struct FilterK {
     const float* wa;
     int N, A, S, npad, L, tlo, W, none;
     int SB; // slices per interleaved group
+    float* rows; // k_filter_fft_entry_lds only: non-null -> windows to rows [S][A][W] instead of q
 };
 
 // block -> (group, angle, pair): rows (s0 = g*SB + 2*pair, a) and (s0 + 1, a)
@@ -429,6 +430,18 @@ __global__ __launch_bounds__(THREADS, WPE) void k_filter_fft_entry_lds(FilterK a
             }
             if (pr == 0) {
                 win[w] = v; // read back by the same thread
+            } else if (a.rows) {
+                // the group's four rows of this angle, each written by this
+                // workgroup only and after all of them were read (in place
+                // when rows is tomo)
+                const float2 h = win[w];
+                const int s0 = g * 4;
+                const size_t ss = (size_t)a.A * a.W;
+                float* r0 = a.rows + ((size_t)s0 * a.A + ang) * a.W + w;
+                if (s0 < a.S) r0[0] = h.x;
+                if (s0 + 1 < a.S) r0[ss] = h.y;
+                if (s0 + 2 < a.S) r0[2 * ss] = v.x;
+                if (s0 + 3 < a.S) r0[3 * ss] = v.y;
             } else {
                 const float2 h = win[w];
                 dst[w] = make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), __float_as_uint(v.x),
@@ -504,7 +517,7 @@ void launch_fft_lds_entry(const FilterK& k, const FftK& p, int blocks, size_t ld
 // >= n/8, with E = 8 elements per thread; n in (8192, 16384] runs 1024
 // threads with E = 16
 template <typename OT>
-void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
+bool launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t lds, hipStream_t st)
 {
     // FP32 rows, n in (2048, 6144]: the two-workgroup form when two of them
     // fit a CU's 160 KB of LDS (RAFT_FFT_FORM=regs keeps the window in
@@ -515,11 +528,11 @@ void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t
         !std::getenv("RAFT_FILTER_PAIRS")) {
         if (p.n <= 3072) launch_fft_lds_entry<384, 3>(k, p, blocks, lw, st);
         else launch_fft_lds_entry<768, 6>(k, p, blocks, lw, st);
-        return;
+        return k.rows != nullptr;
     }
     if (p.n > 8192) {
         launch_fft_t<OT, 1024, 16>(k, p, blocks, lds, st);
-        return;
+        return false;
     }
     switch (threads) {
     case 64: launch_fft_t<OT, 64, 8>(k, p, blocks, lds, st); break;
@@ -531,6 +544,7 @@ void launch_fft(const FilterK& k, const FftK& p, int threads, int blocks, size_t
     case 768: launch_fft_t<OT, 768, 8>(k, p, blocks, lds, st); break;
     default: launch_fft_t<OT, 1024, 8>(k, p, blocks, lds, st); break;
     }
+    return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -966,13 +980,15 @@ FftPlan fft_plan(int n)
     return p;
 }
 
-void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st)
+bool launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st)
 {
-    if (a.S <= 0 || a.A <= 0) return;
+    if (a.S <= 0 || a.A <= 0) return a.rows != nullptr && store == RAFT_STORE_F32;
     FilterK k;
     k.tomo = a.tomo; k.q = a.q; k.H = a.H; k.tw = a.tw; k.g = a.g; k.wa = a.wa;
     k.N = a.N; k.A = a.A; k.S = a.S; k.npad = a.npad; k.L = a.L; k.tlo = a.tlo; k.W = a.W; k.none = a.none;
     k.SB = (store == RAFT_STORE_F16) ? 8 : 4;
+    k.rows = store == RAFT_STORE_F32 ? a.rows : nullptr;
+    bool direct = false;
     const long long groups = (a.S + k.SB - 1) / k.SB;
     const long long blocks = groups * a.A * (k.SB / 2);
     if (blocks > 0x7fffffffLL) fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one filter launch");
@@ -982,7 +998,7 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
         const FftK p = fftk_of(plan);
         const int threads = plan.ok ? plan.threads : 256;
         if (store == RAFT_STORE_F16) launch_fft<__half>(k, p, threads, (int)blocks, lds_fft, st);
-        else launch_fft<float>(k, p, threads, (int)blocks, lds_fft, st);
+        else direct = launch_fft<float>(k, p, threads, (int)blocks, lds_fft, st);
     } else {
         if (lds > 160 * 1024) fail(RAFT_EUNSUPPORTED, "padded row too long for the direct filter path");
         if (store == RAFT_STORE_F16) {
@@ -996,6 +1012,7 @@ void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipS
         }
     }
     RAFT_HIP(hipGetLastError());
+    return direct;
 }
 
 bool bst_filter_polar_fits(int n)
@@ -1022,6 +1039,7 @@ void launch_bst(const BstArgs& a, const FftPlan& plan, hipStream_t st, const Fil
         f.N = filt->N; f.A = filt->A; f.S = filt->S; f.npad = n; f.L = filt->L; f.tlo = 0; f.W = n;
         f.none = filt->none;
         f.SB = 4;
+        f.rows = nullptr;
         if ((long long)((f.S + f.SB - 1) / f.SB) * f.A * (f.SB / 2) > 0x7fffffffLL)
             fail(RAFT_EUNSUPPORTED, "too many sinogram rows for one BST launch");
         const size_t lds_fp = lds + sizeof(float2) * (size_t)(n / 2);

@@ -854,11 +854,15 @@ static void lowpass_device(int dev, float* d_tomo, const float* paramf, const in
         fa.wa = dwa.as<float>();
         fa.N = N; fa.A = A; fa.S = nb; fa.npad = c.npad; fa.L = c.L; fa.tlo = c.L; fa.W = N;
         fa.none = 0;
+        // in place where the filter form writes rows directly (FP32 whole
+        // entries with the window in LDS), else through q
+        fa.rows = d_tomo + (size_t)s0 * A * N;
+        bool direct = false;
         {
             ProfScope ps(KF_FILTER, st);
-            launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
+            direct = launch_fbp_filter(fa, F.plan, RAFT_STORE_F32, st);
         }
-        launch_from_interleaved(q, d_tomo + (size_t)s0 * A * N, nb, A, N, st);
+        if (!direct) launch_from_interleaved(q, d_tomo + (size_t)s0 * A * N, nb, A, N, st);
     }
     RAFT_HIP(hipStreamSynchronize(st));
 }

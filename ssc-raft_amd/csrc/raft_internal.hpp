@@ -135,8 +135,11 @@ struct FilterArgs {
     const float* wa;    // [A] per-angle weight (dtheta * angular scale)
     int N, A, S, npad, L, tlo, W;
     int none;           // filter 'none': no filtering, no 1/(npad h)
+    float* rows = nullptr; // FP32, W = N: where the kernel form allows, the window is written
+                           // straight to rows [S][A][W] (may be tomo itself) instead of q
 };
-void launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
+// true when the filtered windows went straight to a.rows (else to a.q)
+bool launch_fbp_filter(const FilterArgs& a, const FftPlan& plan, int store, hipStream_t st);
 
 // BST after the row filter: q = filtered padded rows (interleaved [G][A][n][4],
 // window [0, n)), polar [S][2A][n/2], obj [S][N][N]; tw the twiddle tables of n.

@@ -448,6 +448,32 @@ def test_filter_largest_fft_sizes(gpu, n, pad):
     assert e <= FP32_TOL
 
 
+@pytest.mark.parametrize("n,a,s,filt", [(1024, 16, 5, "hann"), (2048, 8, 3, "ramp")])
+def test_lowpass_rows_written_in_place(gpu, monkeypatch, n, a, s, filt):
+    """Padded sizes 3072 and 6144 take the FP32 filter form that writes the
+    filtered rows straight back into the data (no staged rows, no copy back):
+    against the oracle, and bit for bit against the form that goes through
+    staged rows (RAFT_FFT_FORM=regs).  Odd slice counts leave the last group
+    of four partial."""
+    import sscRaft
+
+    rng = np.random.default_rng(n + s)
+    tomo = rng.standard_normal((s, a, n)).astype(np.float32)
+    dic = {"gpu": [0], "filter": filt, "padding": 2, "regularization": 0.0, "beta/delta": 0.0,
+           "energy[eV]": 2.0e4, "z2[m]": 0.4, "detectorPixel[m]": 0.3, "blocksize": 0}
+    from sscRaft.processing.filters import lowpass_params
+
+    pf, pi = lowpass_params(n, a, s, sscRaft.SetDictionary(dict(dic), ("gpu",), (), ()))
+    ref = oracle.lowpass(tomo, pf, pi)
+    out = sscRaft.lowpass(tomo.copy(), dic=dict(dic))
+    e = rel(out, ref)
+    print(f"lowpass n={n} pad=2 (npad {3 * n}) slices={s}: rel-L2 {e:.2e}")
+    assert e <= FP32_TOL
+    monkeypatch.setenv("RAFT_FFT_FORM", "regs")
+    staged = sscRaft.lowpass(tomo.copy(), dic=dict(dic))
+    np.testing.assert_array_equal(out, staged)
+
+
 def test_lowpass_device_in_place_and_offset_ignored(gpu):
     import torch
     from sscRaft import device as dv
