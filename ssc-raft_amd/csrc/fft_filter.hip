@@ -393,7 +393,7 @@ __global__ __launch_bounds__(THREADS) void k_filter_fft_entry(FilterK a, FftK p)
 // workgroup's barriers overlap the other's passes.  The arithmetic of each
 // pair is that of k_filter_fft, so the stored entries are the same bit for
 // bit.
-template <int THREADS, int E, int WPE>
+template <int THREADS, int E, int WPE, bool ROWS>
 __global__ __launch_bounds__(THREADS, WPE) void k_filter_fft_entry_lds(FilterK a, FftK p)
 {
     extern __shared__ float2 z[];
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(THREADS, WPE) void k_filter_fft_entry_lds(FilterK a
             }
             if (pr == 0) {
                 win[w] = v; // read back by the same thread
-            } else if (a.rows) {
+            } else if (ROWS) {
                 // the group's four rows of this angle, each written by this
                 // workgroup only and after all of them were read (in place
                 // when rows is tomo)
@@ -505,12 +505,22 @@ void launch_fft_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipSt
     hipLaunchKernelGGL((k_filter_fft<OT, THREADS, E>), dim3(blocks), dim3(THREADS), lds, st, k, p);
 }
 
+template <int THREADS, int WPE, bool ROWS>
+void launch_fft_lds_entry_t(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
+{
+    const void* fn = reinterpret_cast<const void*>(&k_filter_fft_entry_lds<THREADS, 8, WPE, ROWS>);
+    RAFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_filter_fft_entry_lds<THREADS, 8, WPE, ROWS>), dim3(blocks / 2), dim3(THREADS), lds, st, k,
+                       p);
+}
+
+// the rows form is its own instantiation, so the staged-row form's code is
+// unchanged by it
 template <int THREADS, int WPE>
 void launch_fft_lds_entry(const FilterK& k, const FftK& p, int blocks, size_t lds, hipStream_t st)
 {
-    const void* fn = reinterpret_cast<const void*>(&k_filter_fft_entry_lds<THREADS, 8, WPE>);
-    RAFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_filter_fft_entry_lds<THREADS, 8, WPE>), dim3(blocks / 2), dim3(THREADS), lds, st, k, p);
+    if (k.rows) launch_fft_lds_entry_t<THREADS, WPE, true>(k, p, blocks, lds, st);
+    else launch_fft_lds_entry_t<THREADS, WPE, false>(k, p, blocks, lds, st);
 }
 
 // THREADS = the smallest of 64, 128, 192, 256, 384, 512, 768, 1024 that is
