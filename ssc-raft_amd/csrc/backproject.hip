@@ -552,9 +552,9 @@ template <int CHUNK, int PAIR, int SPF = 0> static const void* bp_kernel(int sto
         return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 1, CHUNK, PAIR, SPF>)
                      : reinterpret_cast<const void*>(&k_backproject<float, 1, CHUNK, PAIR, SPF>);
     if (store == RAFT_STORE_F16) {
-        if (PAIR != 1 || SPF != 0) fail(RAFT_EINVAL, "internal: the pair and scalar-prefetch forms take FP32 rows");
-        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK, 1, 0>)
-                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK, 1, 0>);
+        if (PAIR != 1) fail(RAFT_EINVAL, "internal: the pair form takes FP32 rows");
+        return queue ? reinterpret_cast<const void*>(&k_backproject_queue<__half, 0, CHUNK, 1, SPF>)
+                     : reinterpret_cast<const void*>(&k_backproject<__half, 0, CHUNK, 1, SPF>);
     }
     return queue ? reinterpret_cast<const void*>(&k_backproject_queue<float, 0, CHUNK, PAIR, SPF>)
                  : reinterpret_cast<const void*>(&k_backproject<float, 0, CHUNK, PAIR, SPF>);
@@ -582,10 +582,10 @@ bool bp_variant_supported(int variant, int store)
     case 16:
     case 32:
     case 48: return true;
-    case BP_PAIR | 16:
-    case BP_PAIR | 32:
     case BP_SPF | 16:
-    case BP_SPF | 32: return store == RAFT_STORE_F32;
+    case BP_SPF | 32: return true;
+    case BP_PAIR | 16:
+    case BP_PAIR | 32: return store == RAFT_STORE_F32;
     default: return false;
     }
 }

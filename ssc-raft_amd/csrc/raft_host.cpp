@@ -199,15 +199,15 @@ static std::string bp_tune_key(int n, int A, int W, int store, int mode, int gro
     snprintf(k, sizeof k, "bp/n=%d/A=%d/W=%d/store=%d/mode=%d/groups=%d", n, A, W, store, mode, groups);
     return k;
 }
-// backprojector variants the tuner times: angle chunks; for FP32 rows also the
-// scalar-prefetch sweep and the pair form (two slice groups per workgroup)
+// backprojector variants the tuner times: angle chunks, the scalar-prefetch
+// sweep; for FP32 rows also the pair form (two slice groups per workgroup)
 // where its narrower window holds the tile's bins (rr = bin steps per pixel
 // step); every variant gives the same sums
 static const std::vector<int>& bp_cands(int store, double rr = 1.0)
 {
     static const std::vector<int> f32 = {BP_CHUNK, 16, 48, BP_PAIR | 32, BP_PAIR | 16, BP_SPF | 16, BP_SPF | 32};
     static const std::vector<int> f32_nopair = {BP_CHUNK, 16, 48, BP_SPF | 16, BP_SPF | 32};
-    static const std::vector<int> f16 = {BP_CHUNK, 16, 48};
+    static const std::vector<int> f16 = {BP_CHUNK, 16, 48, BP_SPF | 16, BP_SPF | 32};
     const bool pair_fits = rr * std::hypot((double)(BP_TILE_X - 1), (double)(BP_TILE_Y - 1)) + 5.0 <= BP_SLOT_PAIR;
     if (store == RAFT_STORE_F16) return f16;
     return pair_fits ? f32 : f32_nopair;

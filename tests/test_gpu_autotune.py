@@ -57,6 +57,36 @@ def test_tuner_times_candidates_and_every_chunk_agrees(gpu, tmp_path, monkeypatc
     assert np.linalg.norm(ref - oracle.fbp(y, ang, pf, pi)) / np.linalg.norm(oracle.fbp(y, ang, pf, pi)) < 1e-5
 
 
+BP_VARIANTS_F16 = [16, 32, 48, SPF | 16, SPF | 32]
+
+
+def test_fp16_rows_variants_agree(gpu, tmp_path, monkeypatch):
+    """FBP with FP16-stored rows: the tuner times the angle chunks and the
+    scalar-prefetch sweep, and every forced variant gives the same image bit
+    for bit."""
+    import torch
+    from sscRaft import device as dv
+
+    n, a, s = 160, 150, 9
+    y, ang = ph.poisson_log(oracle.radon(ph.phantoms(n, s, config=7), ph.uniform_angles(a)), 1e4, seed=6), \
+        ph.uniform_angles(a)
+    paramf, parami = oracle.fbp_params(n, a, s, pad=2, filter_type=7, pixel=2.0 / n)
+    t = torch.from_numpy(y).cuda()
+    monkeypatch.setenv("RAFT_AUTOTUNE_CACHE", str(tmp_path / "fresh16.json"))
+    dv.autotune_clear()
+    ref = dv.fbp_device(t, ang, paramf, parami, store="f16").cpu().numpy()
+    choices = dv.autotune_choices()
+    keys = [k for k in choices if k.startswith(f"bp/n={n}/A={a}/") and "/store=1/" in k]
+    assert len(keys) == 1, choices
+    rec = choices[keys[0]]
+    assert sorted(c for c, _ in rec["ms"]) == BP_VARIANTS_F16 and rec["choice"] in BP_VARIANTS_F16
+    print(f"tuned {keys[0]}: {rec}")
+    for chunk in BP_VARIANTS_F16:
+        _force(tmp_path, monkeypatch, {keys[0]: chunk})
+        out = dv.fbp_device(t, ang, paramf, parami, store="f16").cpu().numpy()
+        np.testing.assert_array_equal(out, ref)
+
+
 def test_eem_and_radon_variants_agree(gpu, tmp_path, monkeypatch):
     """eEM iterates are bit-identical whichever backprojector chunk and
     projector ray-chunk count run (the KL differs only in summation order);
